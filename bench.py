@@ -1,0 +1,222 @@
+"""Benchmark of the omega-B97M-V / COSX SCF-cycle hot path on one MI355X.
+
+One step = one SCF cycle of the hot path for the synthetic (H2O)_27 cluster
+(def2-SVP, 648 AOs): COSX K(1/r) + K(erf, omega = 0.3) on the SGX grid and
+the VV10 term of nr_nlc_vxc on the VV10 grid (both the project's own
+level-1 Becke grid, 218 566 points), in the mixed-precision form (FP32
+integrals and VV10 pair sums; FP64 AO values, GEMMs, weights, K and vmat),
+with the density matrix already resident in HBM (wb_k_build_dev /
+wb_nlc_vxc_dev).  Also reported: the all-FP64 form of the same kernels, the
+per-cycle error of the mixed form against it, p50/p99 per-cycle latency, the
+exchange kernel's FP32 roofline fraction (HIP events on the library stream)
+and the FP64 CPU oracle timed on a bounded sample on the host cores.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'pyscf-wb97mv-fast_amd'))
+
+OMEGA = 0.3
+NLC = (6.0, 0.01)
+K_TOL = 1e-11
+# FP32 flops per primitive-pair evaluation (one grid point) by class, FMA = 2:
+# Boys, Hermite recursion and E contraction (pruned), and per task and point
+# the F / G contraction 4 * nsph_a * nsph_b
+FLOPS_PRIM = {(0, 0): 43, (1, 0): 89, (1, 1): 223, (2, 0): 182, (2, 1): 516,
+              (2, 2): 1256, (3, 0): 358, (3, 1): 1081, (3, 2): 2674, (3, 3): 5688}
+PEAK_FP32_PACKED = 157.3e12        # MI355X published FP32 vector peak
+PEAK_FP32_VALU = 78.6e12           # without packed FP32 (not used by this build)
+PHASES = ['ao', 'fgemm', 'screen', 'exchange', 'kgemm', 'nlc_ao', 'nlc_rho', 'vv10',
+          'nlc_fock']
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(seed=0):
+    from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
+    t = time.perf_counter()
+    mol = mole.Mole(geometry.water_cluster(27, seed=seed), 'def2-svp')
+    g = grids.Grids(mol, level=1).build()
+    dm = guess.huckel_density(mol)
+    log('workload: nao %d, %d grid points, setup %.1f s' % (mol.nao, g.size,
+                                                            time.perf_counter() - t))
+    return mol, g, dm
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--fp64-steps', type=int, default=2)
+    ap.add_argument('--vv10-tol', type=float, default=0.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(the grid-sharded K with one nao^2 all-reduce is a design note)',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pyscf_wb97mv_fast_amd import lib
+    from pyscf_wb97mv_fast_amd.basis import pack_mol
+
+    mol, g, dm = build_workload()
+    nao = mol.nao
+    L = lib.load()
+    ctx = lib.Context(0)
+    basis = lib.DeviceBasis(ctx, pack_mol(mol))
+    grid = lib.DeviceGrid(ctx, g.coords, g.weights)
+    dev = torch.device('cuda:0')
+    d_dm = torch.tensor(dm, dtype=torch.float64, device=dev)
+    d_K = torch.zeros((2, nao, nao), dtype=torch.float64, device=dev)
+    d_V = torch.zeros((nao, nao), dtype=torch.float64, device=dev)
+    d_e = torch.zeros(2, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    om = numpy.array([0.0, OMEGA])
+
+    def cycle(prec):
+        lib.check(L.wb_k_build_dev(ctx.h, basis.h, grid.h, d_dm.data_ptr(), 2, lib.ptr(om),
+                                   K_TOL, prec, d_K.data_ptr()))
+        lib.check(L.wb_nlc_vxc_dev(ctx.h, basis.h, grid.h, d_dm.data_ptr(), NLC[0], NLC[1],
+                                   args.vv10_tol, prec, d_e.data_ptr(), d_V.data_ptr()))
+        ctx.sync()
+
+    def run(prec, steps, warmup, profile=False):
+        for _ in range(warmup):
+            cycle(prec)
+        lib.check(L.wb_ctx_set_profile(ctx.h, 1 if profile else 0))
+        torch.cuda.synchronize()
+        ctx.sync()
+        times = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t = time.perf_counter()
+            cycle(prec)
+            times.append(time.perf_counter() - t)
+        ctx.sync()
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        ph = numpy.zeros(len(PHASES))
+        lib.check(L.wb_profile_read(ctx.h, lib.ptr(ph), len(PHASES), 1))
+        lib.check(L.wb_ctx_set_profile(ctx.h, 0))
+        out = (d_K.cpu().numpy().copy(), d_e.cpu().numpy().copy(), d_V.cpu().numpy().copy())
+        return total, numpy.array(times), ph / max(steps, 1), out
+
+    # mixed precision: the measured configuration
+    total, times, ph, res32 = run(lib.PREC['mixed'], args.steps, args.warmup, profile=True)
+    value = args.steps / total
+    log('mixed: %.2f ms/cycle, phases (ms) %s' % (1e3 * total / args.steps,
+                                                 dict(zip(PHASES, numpy.round(ph, 3)))))
+    work = numpy.zeros(20, dtype=numpy.int64)
+    lib.check(L.wb_k_work(ctx.h, lib.ptr(work)))
+    kept = numpy.zeros(2, dtype=numpy.int64)
+    lib.check(L.wb_k_stats(ctx.h, grid.h, lib.ptr(kept)))
+    flops = 0.0
+    for la in range(4):
+        for lb in range(la + 1):
+            c = la * (la + 1) // 2 + lb
+            ntask, nprim = int(work[2 * c]), int(work[2 * c + 1])
+            flops += 64.0 * (nprim * FLOPS_PRIM[(la, lb)] + ntask * 4 * (2 * la + 1) * (2 * lb + 1))
+    # the exchange phase covers both operators (the same task lists)
+    md_ms = ph[3] / 2.0
+    achieved = flops / (md_ms * 1e-3) if md_ms > 0 else 0.0
+
+    # all-FP64 form of the same kernels
+    total64, times64, ph64, res64 = run(lib.PREC['fp64'], args.fp64_steps, 1, profile=True)
+    value64 = args.fp64_steps / total64
+    log('fp64: %.2f ms/cycle, phases (ms) %s' % (1e3 * total64 / args.fp64_steps,
+                                                dict(zip(PHASES, numpy.round(ph64, 3)))))
+    dK = res32[0] - res64[0]
+    dEk = [0.25 * float(numpy.einsum('ij,ji->', dm, dK[i])) for i in range(2)]
+    dEx = -0.25 * float(numpy.einsum('ij,ji->', dm, 0.15 * dK[0] + 0.85 * dK[1]))
+    dEnl = float(res32[1][1] - res64[1][1])
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mol, g, dm)
+
+    line = {
+        'metric': 'COSX+VV10 SCF cycles/s (mixed FP32/FP64, |dE| <= 1e-6 Ha vs FP64)',
+        'value': round(value, 4), 'unit': 'cycles/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(1e3 * total / args.steps, 3),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32+f64', 'data': 'synthetic',
+        'config': {'workload': '(H2O)27 def2-SVP: COSX K(1/r) + K(erf, 0.3) + VV10 '
+                               'nr_nlc_vxc per cycle, dm resident in HBM',
+                   'nao': nao, 'grid_points': int(g.size), 'grid': 'own Becke level 1',
+                   'k_tile_tol': K_TOL, 'vv10_tile_tol': args.vv10_tol,
+                   'guess': 'extended Hueckel'},
+        'latency_ms': {'p50': round(1e3 * float(numpy.percentile(times, 50)), 3),
+                       'p99': round(1e3 * float(numpy.percentile(times, 99)), 3)},
+        'fp64': {'value': round(value64, 4), 'ms_per_step': round(1e3 * total64 / args.fp64_steps, 3),
+                 'speedup_mixed_vs_fp64': round(value / value64, 3)},
+        'accuracy_vs_fp64': {'dE_K_1/r': dEk[0], 'dE_K_erf': dEk[1], 'dE_x': dEx,
+                             'dE_nl': dEnl, 'max_abs_dK': float(numpy.abs(dK).max()),
+                             'max_abs_dVnl': float(numpy.abs(res32[2] - res64[2]).max())},
+        'phases_ms': dict(zip(PHASES, [round(float(v), 4) for v in ph])),
+        'roofline': {'bound': 'valu', 'kernel': 'md_k_kernel (all classes, one operator)',
+                     'achieved': round(achieved / 1e12, 3), 'peak': PEAK_FP32_PACKED / 1e12,
+                     'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_PACKED, 4),
+                     'frac_vs_nonpacked_78.6': round(achieved / PEAK_FP32_VALU, 4),
+                     'traffic': None, 'tasks_kept': int(kept[1]), 'tasks_total': int(kept[0]),
+                     'algorithmic_flops_per_launch': flops},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(mol, g, dm, n_k=192, n_v=1024, seed=1):
+    """FP64 oracle on the host cores, bounded sample: K(1/r)+K(erf) on n_k
+    grid points and the VV10 pair sum for n_v outer points, scaled to the
+    full grid (both are linear in the sampled outer points)."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    ob = oracle.OracleBasis.from_mol(mol)
+    rng = numpy.random.default_rng(seed)
+    n = g.size
+    sel = rng.choice(n, n_k, replace=False)
+    t = time.perf_counter()
+    for omega in (0.0, OMEGA):
+        oracle.k_build(ob, g.coords[sel], g.weights[sel], dm, omega, pair_cut=1e-14)
+    tk = (time.perf_counter() - t) * n / n_k
+    # VV10 inputs from the oracle's own density on the grid (FP64)
+    rho = numpy.empty((4, n))
+    for i0 in range(0, n, 16384):
+        ao = oracle.ao_eval(ob, g.coords[i0:i0 + 16384], deriv=1)
+        tmp = ao[0] @ dm
+        rho[0, i0:i0 + 16384] = numpy.einsum('pm,pm->p', tmp, ao[0])
+        for d in range(3):
+            rho[d + 1, i0:i0 + 16384] = 2 * numpy.einsum('pm,pm->p', tmp, ao[d + 1])
+    ind = rho[0] >= oracle.THRESH
+    q, W0, kappa = oracle.vv10_pointwise(rho[0][ind], rho[1:4][:, ind], g.weights[ind], NLC)[:3]
+    m = int(ind.sum())
+    outer = rng.choice(m, min(n_v, m), replace=False)
+    t = time.perf_counter()
+    oracle.vv10_sum_outer(g.coords[ind], q, W0, kappa, outer)
+    tv = (time.perf_counter() - t) * m / len(outer)
+    cores = len(os.sched_getaffinity(0))
+    log('cpu oracle: K %.1f s + VV10 %.1f s per cycle (%d threads)' % (tk, tv, cores))
+    return {'value': round(1.0 / (tk + tv), 6), 'unit': 'cycles/s', 'cores': cores,
+            'kind': 'port',
+            'sample': 'FP64 C oracle (OpenMP), K(1/r)+K(erf) on %d of %d grid points and the '
+                      'VV10 sum for %d of %d outer points, scaled linearly; stock PySCF is '
+                      'not installed' % (n_k, n, len(outer), m),
+            'seconds_per_cycle': {'k_both': round(tk, 2), 'vv10': round(tv, 2)}}
+
+
+if __name__ == '__main__':
+    main()

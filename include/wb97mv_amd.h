@@ -1,0 +1,139 @@
+/*
+ * wb97mv_amd -- C ABI of the MI355X (gfx950) omega-B97M-V / COSX hot path.
+ *
+ * Plain pointers and sizes only.  Host pointers are caller-owned,
+ * C-contiguous and not retained past the call; device state lives behind
+ * the opaque handles.  Every function returns a status (WB_OK, ...); the
+ * message of the last failure on the calling thread is wb_last_error().
+ * Calls on different contexts may run concurrently from different host
+ * threads; one context (one HIP stream) is not reentrant.
+ *
+ * Reference interfaces replaced (Cedrus810/pyscf-wb97mv-fast):
+ *   COSX exchange K   pyscf.sgx.sgx_jk.get_k_only as hooked in
+ *                     gpu/install.py:293-330 -> gpu/sgx_k.py:983-991
+ *                     (GpuKBuilder.get_k, gpu/sgx_k.py:663-751)
+ *   VV10              NumInt.nr_nlc_vxc as hooked in gpu/install.py:276-291
+ *                     -> gpu/vv10.py:268-342 (gpu_nr_nlc_vxc)
+ *   VV10 kernel sum   gpu/vv10.py:167-240 (Vv10Kernel.kernel_sum)
+ *   AO values         gpu/ao_eval.py:186-222 (AoEvaluator.eval)
+ *   shell pairs       gpu/shellpairs.py:184-437 (ShellPairs, built natively
+ *                     inside wb_basis_create)
+ */
+#ifndef WB97MV_AMD_H
+#define WB97MV_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WB_ABI_VERSION 1
+
+/* status codes; the reference raises gpu.xc.Unsupported (gpu/xc.py:30-32)
+ * and retries on OOM (gpu/install.py:99-109) */
+#define WB_OK 0
+#define WB_UNSUPPORTED 1
+#define WB_OOM 2
+#define WB_DEVICE_ERROR 3
+#define WB_INVALID 4
+
+/* precision of the integral / VV10 pair arithmetic; AO values, GEMMs,
+ * weights and accumulations across tiles are FP64 in both */
+#define WB_PREC_MIXED 0
+#define WB_PREC_FP64 1
+
+typedef void* wb_ctx_t;
+typedef void* wb_basis_t;
+typedef void* wb_grid_t;
+
+const char* wb_last_error(void);
+int wb_version(void);
+
+/* Context on one device; stream = NULL creates a private non-blocking
+ * stream (the reference's per-flow streams, gpu/install.py:118-135). */
+int wb_ctx_create(int device, void* stream, wb_ctx_t* out);
+int wb_ctx_destroy(wb_ctx_t ctx);
+/* number of G partial copies of the exchange kernel (default 4) */
+int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
+/* mixed form: accumulate G in FP64 (1, default) or FP32 (0) */
+int wb_ctx_set_g64(wb_ctx_t ctx, int on);
+int wb_sync(wb_ctx_t ctx);
+float wb_last_ms(wb_ctx_t ctx);
+/* Phase timing with HIP events on the context's stream (on = 1); read
+ * accumulated milliseconds of the phases {AO, F GEMM, screening, exchange
+ * kernels, K GEMM, NLC AO, NLC density, VV10 pair sum (with pointwise and
+ * compaction), NLC potential + Fock} into ms[0..n). */
+int wb_ctx_set_profile(wb_ctx_t ctx, int on);
+int wb_profile_read(wb_ctx_t ctx, double* ms, int n, int reset);
+/* Work of the last K build per (la, lb) class c = la(la+1)/2 + lb:
+ * out[2c] kept tasks (64-point sub-block x shell pair), out[2c+1] their
+ * primitive pairs (x 64 points = primitive-pair evaluations). */
+int wb_k_work(wb_ctx_t ctx, long long* out20);
+
+/* Segmented spherical basis (a generally contracted shell is passed as one
+ * shell per contraction column), l <= 3.  coefs carry the libcint
+ * normalization (mol._libcint_ctr_coeff).  Shell pairs whose max
+ * |c_a c_b| K_AB <= pair_cut are dropped (the reference uses 1e-14,
+ * gpu/shellpairs.py:290-294).  Replaces ShellPack + ShellPairs
+ * (gpu/ao_eval.py:57-131, gpu/shellpairs.py:187-437). */
+int wb_basis_create(wb_ctx_t ctx, int nshell, const int* l, const int* nprim,
+                    const double* center, const double* exps, const double* coefs,
+                    double pair_cut, wb_basis_t* out);
+/* out4 = {nao, nshell, kept shell pairs, primitive pairs} */
+int wb_basis_info(wb_basis_t basis, long long* out4);
+int wb_basis_destroy(wb_basis_t basis);
+
+/* Integration grid (bohr), resident on the device; points are reordered
+ * internally into spatially compact 64-point sub-blocks.  Replaces
+ * _GridState (gpu/sgx_k.py:496-542). */
+int wb_grid_create(wb_ctx_t ctx, int npts, const double* coords, const double* weights,
+                   wb_grid_t* out);
+/* out3 = {npts, padded points, sub-blocks} */
+int wb_grid_info(wb_grid_t grid, long long* out3);
+int wb_grid_destroy(wb_grid_t grid);
+
+/* COSX exchange for nop operators (omega = 0: 1/r, > 0: erf(omega r)/r):
+ * K_op = X^T (w o G_op), G_op[g] = A_op(g) F[g], F = X qd, with qd = Q dm
+ * (nao x nao, row-major).  Returns the raw K (nop x nao x nao) before
+ * get_k_only's Q^T and symmetrization (gpu/sgx_k.py:736-741).  Tasks
+ * (64-point sub-block, shell pair) whose bound is below tile_tol are
+ * skipped (0 keeps all; the reference default 1e-11). */
+int wb_k_build(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd, int nop,
+               const double* omegas, double tile_tol, int precision, double* k_out);
+/* the same with device pointers (inputs resident in HBM) */
+int wb_k_build_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_qd,
+                   int nop, const double* omegas, double tile_tol, int precision,
+                   double* d_k);
+/* out2 = {tasks considered, tasks kept} of the last K build on grid */
+int wb_k_stats(wb_ctx_t ctx, wb_grid_t grid, long long* out2);
+
+/* VV10 term of nr_nlc_vxc for one RKS density matrix on grid:
+ * out2 = {nelec, excsum}, vmat (nao x nao).  (b, C) = nlc_coeff (6.0, 0.01
+ * for wB97M-V).  tile_tol > 0 enables tile-pair pruning of the double sum. */
+int wb_nlc_vxc(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* dm, double b,
+               double C, double tile_tol, int precision, double* out2, double* vmat);
+int wb_nlc_vxc_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_dm,
+                   double b, double C, double tile_tol, int precision, double* d_out2,
+                   double* d_vmat);
+
+/* VV10 kernel sums on given points (coords n x 3): F = -1.5 sum T, U, W as
+ * libdft's VXC_vv10nlc returns them (gpu/vv10.py:167-240). */
+int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, const double* W0,
+                const double* kappa, double tile_tol, int precision, double* F, double* U,
+                double* W);
+
+/* AO values (deriv 0) or values + gradients (deriv 1), FP64, out
+ * (1 + 3 deriv, npts, nao) as PySCF's eval_gto (gpu/ao_eval.py:186-222). */
+int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, int deriv,
+               double* out);
+
+/* Test hook (no reference counterpart): the FP64 MFMA GEMMs on host data.
+ * kind 0: C[M][N] = A[M][K] B[K][N]; kind 1: C[M][N] = sum_k A[M][k]
+ * B[N][k] w[k] (w may be NULL); kinds 2 / 3: the same with the 64 x 64
+ * tile kernels. */
+int wb_test_gemm(wb_ctx_t ctx, int kind, int M, int N, int K, const double* A, const double* B,
+                 const double* w, double* C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -179,6 +179,19 @@ def vv10_sum(coords, q, W0, kappa):
     return F, U, W
 
 
+def vv10_sum_outer(coords, q, W0, kappa, outer):
+    """F/U/W for the outer points ``outer`` (indices) against all points."""
+    coords = numpy.ascontiguousarray(coords, dtype=numpy.float64).reshape(-1, 3)
+    n = len(coords)
+    outer = numpy.ascontiguousarray(outer, dtype=numpy.int32)
+    args = [numpy.ascontiguousarray(a, dtype=numpy.float64) for a in (q, W0, kappa)]
+    m = len(outer)
+    F, U, W = numpy.empty(m), numpy.empty(m), numpy.empty(m)
+    lib().orc_vv10_sum_outer(ctypes.c_int(n), _ptr(coords), *[_ptr(a) for a in args],
+                             ctypes.c_int(m), _ptr(outer), _ptr(F), _ptr(U), _ptr(W))
+    return F, U, W
+
+
 def vv10_pointwise(rho0, grad, weights, nlc_pars):
     """numint._vv10nlc's per-point quantities (gpu/vv10.py:243-265)."""
     b, C = nlc_pars

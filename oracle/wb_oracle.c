@@ -364,11 +364,24 @@ void orc_k_build(const orc_basis *bs, int npts, const double *coords,
 }
 
 /* ------------------------------------------------------------ VV10 ----- */
+void orc_vv10_sum_outer(int n, const double *coords, const double *q, const double *W0,
+                        const double *kappa, int nout, const int *outer, double *F,
+                        double *U, double *W);
 void orc_vv10_sum(int n, const double *coords, const double *q, const double *W0,
                   const double *kappa, double *F, double *U, double *W)
 {
+    orc_vv10_sum_outer(n, coords, q, W0, kappa, n, NULL, F, U, W);
+}
+
+/* F/U/W for the outer points listed in outer[0..nout) (all points when
+ * outer == NULL), every point as inner point */
+void orc_vv10_sum_outer(int n, const double *coords, const double *q, const double *W0,
+                        const double *kappa, int nout, const int *outer, double *F,
+                        double *U, double *W)
+{
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
+    for (int o = 0; o < nout; ++o) {
+        const int i = outer ? outer[o] : o;
         double f = 0.0, u = 0.0, w = 0.0;
         const double xi = coords[3 * i], yi = coords[3 * i + 1], zi = coords[3 * i + 2];
         for (int j = 0; j < n; ++j) {
@@ -384,8 +397,8 @@ void orc_vv10_sum(int n, const double *coords, const double *q, const double *W0
             u += Tu;
             w += Tu * r2;
         }
-        F[i] = -1.5 * f;
-        U[i] = u;
-        W[i] = w;
+        F[o] = -1.5 * f;
+        U[o] = u;
+        W[o] = w;
     }
 }

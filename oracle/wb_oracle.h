@@ -35,6 +35,10 @@ void orc_k_build(const orc_basis *bs, int npts, const double *coords,
 void orc_vv10_sum(int n, const double *coords, const double *q, const double *W0,
                   const double *kappa, double *F, double *U, double *W);
 
+void orc_vv10_sum_outer(int n, const double *coords, const double *q, const double *W0,
+                        const double *kappa, int nout, const int *outer, double *F,
+                        double *U, double *W);
+
 #ifdef __cplusplus
 }
 #endif

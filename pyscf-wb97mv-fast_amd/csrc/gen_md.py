@@ -1,0 +1,224 @@
+"""Generates md_gen.h: straight-line McMurchie-Davidson code per (la, lb).
+
+For every shell-pair class la >= lb (l <= 3) it emits
+
+  md_prim_<la><lb>(dx, dy, dz, R0, E, V)
+      one primitive pair at one grid point: the pruned Hermite Coulomb
+      recursion R^0_tuv (t+u+v <= L, L = la+lb) from R0[n] = c_n F_n(T),
+      then V[a][b] += sum_tuv Ex[ax][bx][t] Ey[ay][by][u] Ez[az][bz][v] R_tuv
+      factorized as z -> y -> x contractions.  E is the primitive pair's
+      coefficient record (uniform across the wave, read through the scalar
+      cache); V is the Cartesian block accumulated over the contraction.
+  md_fcart_<l>(Fs, Fc)      Fc[c] = sum_m T[m][c] Fs[m]   (sph -> cart)
+  md_gsph_<l>(Gc, Gs)       Gs[m] = sum_c T[m][c] Gc[c]   (cart -> sph)
+      (in FP32 every non-trivial constant product is compensated with the
+      constant's rounding remainder, wb_cm)
+
+with every index a compile-time constant, so all arrays live in registers.
+E layout per axis: ((i * (lb+1) + j) * (L+1) + t); axes x, y, z follow.
+The math follows the reference's MD recurrences (gpu/shellpairs.py:76-113,
+447-485) and its Cartesian order (lx, then ly descending).
+"""
+import os
+import sys
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), '..'))
+from pyscf_wb97mv_fast_amd.cart2sph import cart2sph, cart_components  # noqa: E402
+
+LMAX = 3
+
+
+def _lit(v):
+    return repr(float(v))
+
+
+def gen_r(L):
+    """Lines computing R_t_u_v (n = 0) for t+u+v <= L from R0[n]."""
+    lines = []
+    done = {}
+
+    def name(n, t, u, v):
+        return 'r%d_%d_%d_%d' % (n, t, u, v)
+
+    def need(n, t, u, v):
+        key = (n, t, u, v)
+        if key in done:
+            return done[key]
+        if t == u == v == 0:
+            done[key] = 'R0[%d]' % n
+            return done[key]
+        if t > 0:
+            a = need(n + 1, t - 1, u, v)
+            expr = 'dx * %s' % a
+            if t > 1:
+                b = need(n + 1, t - 2, u, v)
+                expr = 'fma(dx, %s, (real)%d * %s)' % (a, t - 1, b)
+        elif u > 0:
+            a = need(n + 1, t, u - 1, v)
+            expr = 'dy * %s' % a
+            if u > 1:
+                b = need(n + 1, t, u - 2, v)
+                expr = 'fma(dy, %s, (real)%d * %s)' % (a, u - 1, b)
+        else:
+            a = need(n + 1, t, u, v - 1)
+            expr = 'dz * %s' % a
+            if v > 1:
+                b = need(n + 1, t, u, v - 2)
+                expr = 'fma(dz, %s, (real)%d * %s)' % (a, v - 1, b)
+        nm = name(n, t, u, v)
+        lines.append('    const real %s = %s;' % (nm, expr))
+        done[key] = nm
+        return nm
+
+    out = {}
+    for t in range(L + 1):
+        for u in range(L + 1 - t):
+            for v in range(L + 1 - t - u):
+                out[(t, u, v)] = need(0, t, u, v)
+    return lines, out
+
+
+def gen_prim(la, lb):
+    L = la + lb
+    ca, cb = cart_components(la), cart_components(lb)
+    nb = len(cb)
+    naxis = (la + 1) * (lb + 1) * (L + 1)
+
+    def eidx(axis, i, j, t):
+        return axis * naxis + (i * (lb + 1) + j) * (L + 1) + t
+
+    lines = ['template <typename real>',
+             '__device__ __forceinline__ void md_prim_%d%d(real dx, real dy, real dz,'
+             % (la, lb),
+             '    const real* __restrict__ R0, const real* __restrict__ E,'
+             ' real* __restrict__ V)',
+             '{']
+    rl, R = gen_r(L)
+    lines += rl
+
+    erl = 3 * naxis
+
+    def emul(axis, i, j, t, x):
+        if i == 0 and j == 0 and t == 0:
+            return x                      # E^{00}_0 = 1
+        return 'wb_emul<real, %d>(E, %d, %s)' % (erl, eidx(axis, i, j, t), x)
+
+    def efma(axis, i, j, t, x, acc):
+        if i == 0 and j == 0 and t == 0:
+            return '%s + %s' % (acc, x)
+        return 'wb_efma<real, %d>(E, %d, %s, %s)' % (erl, eidx(axis, i, j, t), x, acc)
+
+    # which (az,bz) / (ay,by,az,bz) combinations are used
+    zpairs = sorted({(a[2], b[2]) for a in ca for b in cb})
+    sz = {}
+    for (az, bz) in zpairs:
+        # t + u <= la + lb - az - bz
+        rem = L - az - bz
+        for t in range(rem + 1):
+            for u in range(rem + 1 - t):
+                terms = [(v, R[(t, u, v)]) for v in range(az + bz + 1)]
+                nm = 'sz_%d%d_%d_%d' % (az, bz, t, u)
+                expr = emul(2, az, bz, terms[0][0], terms[0][1])
+                for v, rn in terms[1:]:
+                    expr = efma(2, az, bz, v, rn, expr)
+                lines.append('    const real %s = %s;' % (nm, expr))
+                sz[(az, bz, t, u)] = nm
+    yz = {}
+    yzpairs = sorted({(a[1], b[1], a[2], b[2]) for a in ca for b in cb})
+    for (ay, by, az, bz) in yzpairs:
+        tmax = L - ay - by - az - bz
+        for t in range(tmax + 1):
+            terms = [(u, sz[(az, bz, t, u)]) for u in range(ay + by + 1)]
+            nm = 'syz_%d%d%d%d_%d' % (ay, by, az, bz, t)
+            expr = emul(1, ay, by, terms[0][0], terms[0][1])
+            for u, sn in terms[1:]:
+                expr = efma(1, ay, by, u, sn, expr)
+            lines.append('    const real %s = %s;' % (nm, expr))
+            yz[(ay, by, az, bz, t)] = nm
+    for i, a in enumerate(ca):
+        for j, b in enumerate(cb):
+            ax, bx = a[0], b[0]
+            acc = 'V[%d]' % (i * nb + j)
+            expr = acc
+            for t in range(ax + bx + 1):
+                expr = efma(0, ax, bx, t, yz[(a[1], b[1], a[2], b[2], t)], expr)
+            lines.append('    %s = %s;' % (acc, expr))
+    lines.append('}')
+    return lines
+
+
+def _cm(v, x):
+    """Constant product v * x: in FP32 as fma(v_hi, x, v_lo * x) with the
+    FP32 rounding remainder v_lo, so the constant's rounding (the same at
+    every grid point) does not bias the result; plain in FP64."""
+    if float(v) in (1.0, -1.0, 2.0, -2.0, 3.0, -3.0, 4.0, -4.0):
+        return '(real)%s * %s' % (_lit(v), x)
+    import numpy
+    hi = float(numpy.float32(v))
+    lo = float(numpy.float32(v - hi))
+    return 'wb_cm<real>(%sf, %sf, %s, %s)' % (repr(hi), repr(lo), _lit(v), x)
+
+
+def gen_transforms(l):
+    T = cart2sph(l)
+    nsph, nc = T.shape
+    lines = ['template <typename real>',
+             '__device__ __forceinline__ void md_fcart_%d(const real* __restrict__ Fs,'
+             ' real* __restrict__ Fc)' % l, '{']
+    for c in range(nc):
+        terms = [(m, T[m, c]) for m in range(nsph) if T[m, c] != 0.0]
+        expr = ' + '.join(_cm(v, 'Fs[%d]' % m) for m, v in terms) or '(real)0'
+        lines.append('    Fc[%d] = %s;' % (c, expr))
+    lines.append('}')
+    lines += ['template <typename real>',
+              '__device__ __forceinline__ void md_gsph_%d(const real* __restrict__ Gc,'
+              ' real* __restrict__ Gs)' % l, '{']
+    for m in range(nsph):
+        terms = [(c, T[m, c]) for c in range(nc) if T[m, c] != 0.0]
+        expr = ' + '.join(_cm(v, 'Gc[%d]' % c) for c, v in terms)
+        lines.append('    Gs[%d] = %s;' % (m, expr))
+    lines.append('}')
+    return lines
+
+
+def main(out_path):
+    lines = ['// generated by gen_md.py -- do not edit',
+             '#pragma once',
+             '// c * x for a constant c = hi + lo (FP32) or d (FP64)',
+             'template <typename real>',
+             '__device__ __forceinline__ real wb_cm(float hi, float lo, double d, real x);',
+             'template <> __device__ __forceinline__ float wb_cm<float>(float hi, float lo,'
+             ' double, float x) { return fmaf(hi, x, lo * x); }',
+             'template <> __device__ __forceinline__ double wb_cm<double>(float, float,'
+             ' double d, double x) { return d * x; }',
+             '// E-coefficient products; with WB_ECOMP the FP32 record holds the',
+             '// rounding remainders after the ERL leading values',
+             '#ifndef WB_ECOMP',
+             '#define WB_ECOMP 0',
+             '#endif',
+             'template <typename real, int ERL>',
+             '__device__ __forceinline__ real wb_efma(const real* __restrict__ E, int k, real x,'
+             ' real acc) {',
+             '    if constexpr (WB_ECOMP && sizeof(real) == 4) return fma(E[k], x, fma(E[k + ERL], x, acc));',
+             '    else return fma(E[k], x, acc);',
+             '}',
+             'template <typename real, int ERL>',
+             '__device__ __forceinline__ real wb_emul(const real* __restrict__ E, int k, real x) {',
+             '    if constexpr (WB_ECOMP && sizeof(real) == 4) return fma(E[k], x, E[k + ERL] * x);',
+             '    else return E[k] * x;',
+             '}']
+    for l in range(LMAX + 1):
+        lines += gen_transforms(l)
+    for la in range(LMAX + 1):
+        for lb in range(la + 1):
+            lines += gen_prim(la, lb)
+    text = '\n'.join(lines) + '\n'
+    os.makedirs(os.path.dirname(out_path), exist_ok=True)
+    old = open(out_path).read() if os.path.exists(out_path) else None
+    if old != text:
+        with open(out_path, 'w') as f:
+            f.write(text)
+
+
+if __name__ == '__main__':
+    main(sys.argv[1])

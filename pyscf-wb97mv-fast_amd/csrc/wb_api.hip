@@ -1,0 +1,896 @@
+// C ABI of the wb97mv_amd library and its host-side orchestration.
+//
+// Host preprocessing (FP64, once per basis / grid) and the per-call device
+// pipelines:
+//   K build:  X = AO(grid) -> F = X (QD) -> screening -> per class MD kernel
+//             (G copies) -> K = X^T (w o sum G)           (gpu/sgx_k.py:663-751)
+//   NLC:      X, dX = AO(grid) -> T = X D -> rho, grad rho -> VV10 pointwise,
+//             compaction, double sum -> potential -> vmat   (gpu/vv10.py:268-342)
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include "wb_boys.h"
+#include "wb_internal.h"
+#ifndef WB_ECOMP
+#define WB_ECOMP 0
+#endif
+
+namespace wb {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+// ---------------------------------------------------------------- Boys ----
+static long double boys_ld(int m, long double T)
+{
+    // F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
+    if (T == 0.0L) return 1.0L / (2 * m + 1);
+    long double term = 1.0L / (2 * m + 1), sum = term;
+    for (int k = 1; k < 2000; ++k) {
+        term *= 2.0L * T / (2 * m + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-21L) break;
+    }
+    return expl(-T) * sum;
+}
+
+void boys_tables(std::vector<float>& t32, std::vector<double>& t64)
+{
+    t32.assign(B32_NODES * B32_ROW, 0.f);
+    for (int k = 0; k < B32_NODES; ++k) {
+        const long double T = 0.25L * k;
+        for (int m = 0; m <= 13; ++m) {
+            const long double v = m <= 12 ? boys_ld(m, T) : expl(-T);
+            const float hi = (float)v;
+            t32[k * B32_ROW + m] = hi;
+            t32[k * B32_ROW + B32_LO + m] = (float)(v - (long double)hi);
+        }
+    }
+    t64.assign(B64_NODES * B64_ROW, 0.0);
+    for (int k = 0; k < B64_NODES; ++k) {
+        const long double T = 0.125L * k;
+        for (int m = 0; m <= 14; ++m) t64[k * B64_ROW + m] = (double)boys_ld(m, T);
+        t64[k * B64_ROW + 15] = (double)expl(-T);
+    }
+}
+
+// ------------------------------------------------------ basis / pairs -----
+static int cart_idx_list(int l, int comps[][3])
+{
+    int c = 0;
+    for (int lx = l; lx >= 0; --lx)
+        for (int ly = l - lx; ly >= 0; --ly) {
+            comps[c][0] = lx; comps[c][1] = ly; comps[c][2] = l - lx - ly;
+            ++c;
+        }
+    return c;
+}
+
+// E[i][j][t] for one axis (McMurchie-Davidson, gpu/shellpairs.py:76-113)
+static void hermite_axis(int la, int lb, double p, double xpa, double xpb,
+                         double E[4][4][7])
+{
+    std::memset(E, 0, sizeof(double) * 4 * 4 * 7);
+    const double h = 0.5 / p;
+    E[0][0][0] = 1.0;
+    for (int i = 0; i <= la; ++i)
+        for (int j = 0; j <= lb; ++j) {
+            if (i == 0 && j == 0) continue;
+            int ii = i, jj = j;
+            double x;
+            if (j == 0) { ii = i - 1; x = xpa; } else { jj = j - 1; x = xpb; }
+            const int tp = ii + jj;
+            for (int t = 0; t <= i + j; ++t) {
+                double v = 0.0;
+                if (t > 0) v += h * E[ii][jj][t - 1];
+                if (t <= tp) v += x * E[ii][jj][t];
+                if (t + 1 <= tp) v += (t + 1) * E[ii][jj][t + 1];
+                E[i][j][t] = v;
+            }
+        }
+}
+
+// sup_{s >= 0} (s + a)^i (s + b)^j exp(-c s^2): log-concave, so the maximum
+// is at the root of i/(s+a) + j/(s+b) - 2 c s (bisection)
+static double poly_gauss_sup(int i, int j, double a, double b, double c)
+{
+    if (i == 0 && j == 0) return 1.0;
+    auto dlog = [&](double s) { return i / (s + a + 1e-300) + j / (s + b + 1e-300) - 2 * c * s; };
+    double lo = 0.0, hi = 1.0;
+    while (dlog(hi) > 0) hi *= 2;
+    if (dlog(0.0) <= 0) hi = 0.0;
+    for (int it = 0; it < 200 && hi > 0; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (dlog(mid) > 0) lo = mid; else hi = mid;
+    }
+    const double s = hi;
+    return std::pow(s + a, i) * std::pow(s + b, j) * std::exp(-c * s * s);
+}
+
+static double rowsum_c2s_max(int l)
+{
+    // max over spherical components m of sum_c |T[m][c]| of the real solid
+    // harmonics: s 1/(2 sqrt(pi)), p sqrt(3/(4 pi)), d the z^2 row
+    // 4 sqrt(5/(16 pi)), f the z^3 row 8 sqrt(7/(16 pi))
+    switch (l) {
+    case 0: return 0.5 / std::sqrt(M_PI);
+    case 1: return std::sqrt(3.0 / (4 * M_PI));
+    case 2: return 4.0 * std::sqrt(5.0 / (16 * M_PI));
+    default: return 8.0 * std::sqrt(7.0 / (16 * M_PI));
+    }
+}
+
+static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
+                        const double* center, const double* exps, const double* coefs,
+                        double pair_cut)
+{
+    B.nshell = nshell;
+    B.pair_cut = pair_cut;
+    B.shells.resize(nshell);
+    int p0 = 0, ao = 0;
+    for (int s = 0; s < nshell; ++s) {
+        if (l[s] < 0 || l[s] > LMAX) throw Error(WB_UNSUPPORTED, "shell angular momentum > 3");
+        if (nprim[s] < 1 || nprim[s] > 64) throw Error(WB_UNSUPPORTED, "bad primitive count");
+        ShellInfo& si = B.shells[s];
+        si.cx = center[3 * s]; si.cy = center[3 * s + 1]; si.cz = center[3 * s + 2];
+        si.l = l[s]; si.nprim = nprim[s]; si.prim0 = p0; si.ao0 = ao;
+        p0 += nprim[s];
+        ao += 2 * l[s] + 1;
+    }
+    B.nao = ao;
+    B.exps.assign(exps, exps + p0);
+    B.coefs.assign(coefs, coefs + p0);
+
+    for (int la = 0; la <= LMAX; ++la)
+        for (int lb = 0; lb <= la; ++lb) {
+            ClassData& C = B.cls[class_id(la, lb)];
+            C.la = la; C.lb = lb;
+            C.erl = 3 * (la + 1) * (lb + 1) * (la + lb + 1);
+        }
+    // pairs: A = the shell of larger l (larger index on ties), sorted by (A, B)
+    std::vector<std::pair<int, int>> order;
+    for (int i = 0; i < nshell; ++i)
+        for (int j = 0; j <= i; ++j) {
+            int a = i, b = j;
+            if (l[j] > l[i]) std::swap(a, b);
+            order.emplace_back(a, b);
+        }
+    std::sort(order.begin(), order.end());
+    int comps_a[10][3], comps_b[10][3];
+    long long total = 0;
+    for (auto [a, b] : order) {
+        const ShellInfo &A = B.shells[a], &Bs = B.shells[b];
+        const int la = A.l, lb = Bs.l;
+        ClassData& C = B.cls[class_id(la, lb)];
+        const double AB[3] = {A.cx - Bs.cx, A.cy - Bs.cy, A.cz - Bs.cz};
+        const double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+        // shell-pair magnitude cutoff (gpu/shellpairs.py:290-294)
+        double mag = 0.0;
+        for (int i = 0; i < A.nprim; ++i)
+            for (int j = 0; j < Bs.nprim; ++j) {
+                const double ea = exps[A.prim0 + i], eb = exps[Bs.prim0 + j];
+                const double v = std::fabs(coefs[A.prim0 + i] * coefs[Bs.prim0 + j]) *
+                                 std::exp(-ea * eb / (ea + eb) * ab2);
+                mag = std::max(mag, v);
+            }
+        if (mag <= pair_cut) continue;
+        ++total;
+        PairRec pr{};
+        pr.a0 = A.ao0; pr.b0 = Bs.ao0;
+        pr.pp0 = C.nprims;
+        pr.shA = a; pr.shB = b;
+        pr.diag = (a == b);
+        cart_idx_list(la, comps_a);
+        cart_idx_list(lb, comps_b);
+        PairScreen ps{};
+        double pmin = 1e300, beta = 0.0;
+        double Pd[3] = {0, 0, 0};
+        std::vector<double> Ps;
+        const double rs = rowsum_c2s_max(la) * rowsum_c2s_max(lb);
+        for (int i = 0; i < A.nprim; ++i)
+            for (int j = 0; j < Bs.nprim; ++j) {
+                const double ea = exps[A.prim0 + i], eb = exps[Bs.prim0 + j];
+                const double p = ea + eb, mu = ea * eb / p;
+                const double K = std::exp(-mu * ab2);
+                const double cc = coefs[A.prim0 + i] * coefs[Bs.prim0 + j];
+                const double P[3] = {(ea * A.cx + eb * Bs.cx) / p, (ea * A.cy + eb * Bs.cy) / p,
+                                     (ea * A.cz + eb * Bs.cz) / p};
+                const double PA[3] = {P[0] - A.cx, P[1] - A.cy, P[2] - A.cz};
+                const double PB[3] = {P[0] - Bs.cx, P[1] - Bs.cy, P[2] - Bs.cz};
+                C.P.insert(C.P.end(), P, P + 3);
+                C.p.push_back(p);
+                C.pref.push_back(cc * K * 2.0 * M_PI / p);
+                double E[3][4][4][7];
+                for (int d = 0; d < 3; ++d) hermite_axis(la, lb, p, PA[d], PB[d], E[d]);
+                const int L = la + lb;
+                for (int d = 0; d < 3; ++d)
+                    for (int ii = 0; ii <= la; ++ii)
+                        for (int jj = 0; jj <= lb; ++jj)
+                            for (int t = 0; t <= L; ++t) C.E.push_back(E[d][ii][jj][t]);
+                // screening: |chi_a chi_b| <= |cc| K M exp(-0.75 p |r-P|^2),
+                // M = max over Cartesian components of prod_axes
+                //     sup_s (s+|PA|)^i (s+|PB|)^j exp(-0.25 p s^2)
+                double M = 0.0;
+                for (int u = 0; u < ncart(la); ++u)
+                    for (int v = 0; v < ncart(lb); ++v) {
+                        double m = 1.0;
+                        for (int d = 0; d < 3; ++d)
+                            m *= poly_gauss_sup(comps_a[u][d], comps_b[v][d], std::fabs(PA[d]),
+                                                std::fabs(PB[d]), 0.25 * p);
+                        M = std::max(M, m);
+                    }
+                beta += std::fabs(cc) * K * M * 2.0 * M_PI / (0.75 * p) * rs;
+                if (p < pmin) { pmin = p; Pd[0] = P[0]; Pd[1] = P[1]; Pd[2] = P[2]; }
+                Ps.insert(Ps.end(), P, P + 3);
+            }
+        double delta = 0.0;
+        for (size_t m = 0; m < Ps.size() / 3; ++m) {
+            const double dx = Ps[3 * m] - Pd[0], dy = Ps[3 * m + 1] - Pd[1], dz = Ps[3 * m + 2] - Pd[2];
+            delta = std::max(delta, std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        ps.cx = (float)Pd[0]; ps.cy = (float)Pd[1]; ps.cz = (float)Pd[2];
+        ps.delta = (float)(delta * (1 + 1e-6) + 1e-5);
+        ps.beta = (float)(beta * (1 + 1e-5));
+        ps.q = (float)(0.75 * pmin * (1 - 1e-6));
+        pr.npp = C.nprims + A.nprim * Bs.nprim - pr.pp0;
+        C.nprims += A.nprim * Bs.nprim;
+        C.pairs.push_back(pr);
+        C.scr.push_back(ps);
+        C.npairs++;
+    }
+    B.npairs_total = total;
+}
+
+// synchronous upload (setup paths; the host buffer may die on return)
+static void upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s)
+{
+    d.ensure(bytes);
+    if (bytes) {
+        WB_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+        WB_HIP(hipStreamSynchronize(s));
+    }
+}
+
+static void upload_basis(Basis& B, hipStream_t s)
+{
+    upload(B.d_shells, B.shells.data(), sizeof(ShellInfo) * B.nshell, s);
+    upload(B.d_exps, B.exps.data(), sizeof(double) * B.exps.size(), s);
+    upload(B.d_coefs, B.coefs.data(), sizeof(double) * B.coefs.size(), s);
+    std::vector<int> shao(2 * B.nshell);
+    for (int i = 0; i < B.nshell; ++i) {
+        shao[i] = B.shells[i].ao0;
+        shao[B.nshell + i] = 2 * B.shells[i].l + 1;
+    }
+    upload(B.d_shao, shao.data(), sizeof(int) * shao.size(), s);
+    for (auto& C : B.cls) {
+        if (!C.npairs) continue;
+        upload(C.d_pairs, C.pairs.data(), sizeof(PairRec) * C.npairs, s);
+        upload(C.d_scr, C.scr.data(), sizeof(PairScreen) * C.npairs, s);
+        upload(C.d_P, C.P.data(), sizeof(double) * C.P.size(), s);
+#if WB_ECOMP
+        // per primitive pair: the erl leading FP32 values, then their remainders
+        std::vector<float> e32(2 * C.E.size());
+        for (size_t m = 0; m < C.E.size() / C.erl; ++m)
+            for (int i = 0; i < C.erl; ++i) {
+                const double v = C.E[m * C.erl + i];
+                const float hi = (float)v;
+                e32[2 * m * C.erl + i] = hi;
+                e32[2 * m * C.erl + C.erl + i] = (float)(v - (double)hi);
+            }
+#else
+        std::vector<float> e32(C.E.begin(), C.E.end());
+#endif
+        upload(C.d_E32, e32.data(), sizeof(float) * e32.size(), s);
+        upload(C.d_E64, C.E.data(), sizeof(double) * C.E.size(), s);
+    }
+    WB_HIP(hipStreamSynchronize(s));
+}
+
+// per-omega primitive-pair operator constants [tp, c_0..c_L]:
+// theta = w^2/(w^2+p) (1 for 1/r), tp = theta p, c_n = pref sqrt(theta)(-2 theta p)^n,
+// formed in FP64; the FP32 record holds (hi, lo) pairs [tp, c_0..c_L] so the
+// kernel forms each constant product with one rounding (fma(hi, x, lo x))
+static const void* op_consts(ClassData& C, double omega, bool f32, hipStream_t s)
+{
+    auto it = C.opc.find(omega);
+    if (it == C.opc.end()) {
+        const int L = C.la + C.lb, crl = L + 2;
+        std::vector<double> c64((size_t)C.nprims * crl);
+        for (int m = 0; m < C.nprims; ++m) {
+            const double p = C.p[m];
+            const double th = omega > 0 ? omega * omega / (omega * omega + p) : 1.0;
+            c64[(size_t)m * crl] = th * p;
+            double c = C.pref[m] * std::sqrt(th);
+            for (int n = 0; n <= L; ++n) {
+                c64[(size_t)m * crl + 1 + n] = c;
+                c *= -2.0 * th * p;
+            }
+        }
+        std::vector<float> c32(2 * c64.size());
+        for (size_t i = 0; i < c64.size(); ++i) {
+            const float hi = (float)c64[i];
+            c32[2 * i] = hi;
+            c32[2 * i + 1] = (float)(c64[i] - (double)hi);
+        }
+        auto b32 = std::make_shared<DevBuf>(), b64 = std::make_shared<DevBuf>();
+        upload(*b32, c32.data(), sizeof(float) * c32.size(), s);
+        upload(*b64, c64.data(), sizeof(double) * c64.size(), s);
+        WB_HIP(hipStreamSynchronize(s));
+        it = C.opc.emplace(omega, std::make_pair(b32, b64)).first;
+    }
+    return f32 ? it->second.first->p : it->second.second->p;
+}
+
+// ---------------------------------------------------------------- grid ----
+// k-d split into leaves of exactly 64 points (the last may be short): each
+// split cuts the longest extent at a multiple of 64 points.
+static void kd_order(std::vector<int>& idx, int b, int e, const double* xyz)
+{
+    const int n = e - b;
+    if (n <= SUB) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = b; i < e; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], xyz[3 * idx[i] + d]);
+            hi[d] = std::max(hi[d], xyz[3 * idx[i] + d]);
+        }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d) if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+    const int nblk = (n + SUB - 1) / SUB;
+    const int mid = b + (nblk / 2) * SUB;
+    std::nth_element(idx.begin() + b, idx.begin() + mid, idx.begin() + e,
+                     [&](int x, int y) { return xyz[3 * x + ax] < xyz[3 * y + ax]; });
+    kd_order(idx, b, mid, xyz);
+    kd_order(idx, mid, e, xyz);
+}
+
+static void build_grid(Grid& G, int npts, const double* xyz, const double* w, hipStream_t s)
+{
+    G.npts = npts;
+    G.nsub = std::max(1, (npts + SUB - 1) / SUB);
+    G.Ppad = G.nsub * SUB;
+    G.perm.resize(npts);
+    std::iota(G.perm.begin(), G.perm.end(), 0);
+    kd_order(G.perm, 0, npts, xyz);
+    std::vector<double> hx(3 * (size_t)G.Ppad), hw(G.Ppad, 0.0);
+    std::vector<float4> sub(G.nsub);
+    double o[3] = {0, 0, 0};
+    for (int i = 0; i < npts; ++i)
+        for (int d = 0; d < 3; ++d) o[d] += xyz[3 * i + d];
+    for (int d = 0; d < 3; ++d) G.origin[d] = npts ? o[d] / npts : 0.0;
+    for (int sb = 0; sb < G.nsub; ++sb) {
+        const int b = sb * SUB, e = std::min(npts, b + SUB);
+        double c[3] = {0, 0, 0};
+        for (int i = b; i < e; ++i)
+            for (int d = 0; d < 3; ++d) c[d] += xyz[3 * G.perm[i] + d];
+        const int cnt = std::max(1, e - b);
+        for (int d = 0; d < 3; ++d) c[d] /= cnt;
+        double r = 0.0;
+        for (int i = b; i < b + SUB; ++i) {
+            const int src = (i < npts) ? G.perm[i] : (npts ? G.perm[b < npts ? b : 0] : -1);
+            for (int d = 0; d < 3; ++d) hx[(size_t)d * G.Ppad + i] = src >= 0 ? xyz[3 * src + d] : 0.0;
+            hw[i] = (i < npts) ? w[G.perm[i]] : 0.0;
+            if (i < e) {
+                const double dx = xyz[3 * src] - c[0], dy = xyz[3 * src + 1] - c[1],
+                             dz = xyz[3 * src + 2] - c[2];
+                r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        }
+        sub[sb] = make_float4((float)c[0], (float)c[1], (float)c[2],
+                              (float)(r * (1 + 1e-6) + 1e-5));
+    }
+    G.w_sorted = hw;
+    upload(G.d_xyz, hx.data(), sizeof(double) * hx.size(), s);
+    upload(G.d_w, hw.data(), sizeof(double) * hw.size(), s);
+    upload(G.d_sub, sub.data(), sizeof(float4) * sub.size(), s);
+    WB_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------ profiling ---
+void Ctx::start()
+{
+    if (!profile) return;
+    if (!t0) WB_HIP(hipEventCreate(&t0));
+    WB_HIP(hipEventRecord(t0, stream));
+    marks.clear();
+}
+
+void Ctx::mark(int phase)
+{
+    if (!profile) return;
+    hipEvent_t e;
+    WB_HIP(hipEventCreate(&e));
+    WB_HIP(hipEventRecord(e, stream));
+    marks.emplace_back(phase, e);
+}
+
+// accumulate the recorded phases into phase_ms (synchronizes)
+static void collect(Ctx& c)
+{
+    if (!c.profile || c.marks.empty()) return;
+    WB_HIP(hipEventSynchronize(c.marks.back().second));
+    hipEvent_t prev = c.t0;
+    for (auto& [ph, e] : c.marks) {
+        float ms = 0.f;
+        WB_HIP(hipEventElapsedTime(&ms, prev, e));
+        c.phase_ms[ph] += ms;
+        if (prev != c.t0) (void)hipEventDestroy(prev);
+        prev = e;
+    }
+    if (prev != c.t0) (void)hipEventDestroy(prev);
+    c.marks.clear();
+}
+
+// ------------------------------------------------------------- K build ----
+static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
+                        const double* omegas, double tol, int prec, double* d_K)
+{
+    const bool f32 = (prec == WB_PREC_MIXED);
+    const size_t rsz = f32 ? 4 : 8;
+    const int nao = B.nao, Ppad = G.Ppad;
+    hipStream_t s = c.stream;
+    c.start();
+    double* X = (double*)c.ws_X.ensure(sizeof(double) * nao * (size_t)Ppad);
+    launch_ao_eval(s, B, G, 0, X);
+    c.mark(PH_AO);
+    double* QDT = (double*)c.ws_D.ensure(sizeof(double) * nao * (size_t)nao);
+    transpose(s, nao, d_QD, QDT);
+    void* F = c.ws_F.ensure(rsz * nao * (size_t)Ppad);
+    gemm_nn_fast(s, nao, Ppad, nao, QDT, X, F, f32);
+    c.mark(PH_FGEMM);
+    // screening and task lists
+    const int* d_sh = B.d_shao.as<int>();
+    float* fmax = (float*)c.ws_fmax.ensure(sizeof(float) * (size_t)G.nsub * B.nshell);
+    launch_fmax(s, F, f32, G.d_w.as<double>(), nao, Ppad, G.nsub, d_sh, d_sh + B.nshell, B.nshell, fmax);
+    size_t ntask = 0;
+    for (auto& C : B.cls) ntask += (size_t)C.npairs * G.nsub;
+    int* counts = (int*)c.ws_counts.ensure(sizeof(int) * (size_t)NCLASS * G.nsub);
+    zero_bytes(s, counts, sizeof(int) * (size_t)NCLASS * G.nsub);
+    auto* work = (unsigned long long*)c.ws_work.ensure(sizeof(unsigned long long) * 2 * NCLASS);
+    zero_bytes(s, work, sizeof(unsigned long long) * 2 * NCLASS);
+    int* ids = (int*)c.ws_tasks.ensure(sizeof(int) * std::max<size_t>(ntask, 1));
+    size_t toff = 0;
+    std::vector<size_t> cls_off(NCLASS);
+    for (int k = 0; k < NCLASS; ++k) {
+        cls_off[k] = toff;
+        launch_tasks(s, B.cls[k], G, fmax, B.nshell, (float)tol, counts + (size_t)k * G.nsub,
+                     ids + toff, work + 2 * k);
+        toff += (size_t)B.cls[k].npairs * G.nsub;
+    }
+    c.tasks_total = (long long)ntask;
+    const int ncopy = c.ncopy;
+    const size_t gstride = (size_t)nao * Ppad;
+    const bool g64 = !f32 || c.g64;
+    const size_t gsz = g64 ? 8 : 4;
+    void* Gb = c.ws_G.ensure(gsz * gstride * ncopy);
+    const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
+    c.mark(PH_SCREEN);
+    for (int op = 0; op < nop; ++op) {
+        zero_bytes(s, Gb, gsz * gstride * ncopy);
+        for (int k = 0; k < NCLASS; ++k) {
+            ClassData& C = B.cls[k];
+            if (!C.npairs) continue;
+            const void* Cc = op_consts(C, omegas[op], f32, s);
+            launch_md(s, C, G, f32, g64, F, Gb, gstride, ncopy, counts + (size_t)k * G.nsub,
+                      ids + cls_off[k], Cc, boys);
+        }
+        c.mark(PH_MD);
+        gemm_nt_fast(s, nao, nao, Ppad, X, Gb, !g64, ncopy, gstride, G.d_w.as<double>(),
+                       d_K + (size_t)op * nao * nao, c.ws_Kp);
+        c.mark(PH_KGEMM);
+    }
+    collect(c);
+}
+
+static long long count_kept(Ctx& c, Grid& G)
+{
+    std::vector<int> h((size_t)NCLASS * G.nsub);
+    WB_HIP(hipMemcpyAsync(h.data(), c.ws_counts.p, sizeof(int) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    WB_HIP(hipStreamSynchronize(c.stream));
+    long long s = 0;
+    for (int v : h) s += v;
+    return s;
+}
+
+// ------------------------------------------------------------------ NLC ---
+static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, double Cc,
+                    double tol, int prec, double* d_out2, double* d_vmat)
+{
+    const bool f64 = (prec == WB_PREC_FP64);
+    const int nao = B.nao, Ppad = G.Ppad;
+    hipStream_t s = c.stream;
+    c.start();
+    const size_t plane = (size_t)nao * Ppad;
+    double* X4 = (double*)c.ws_Xv.ensure(sizeof(double) * 4 * plane);
+    launch_ao_eval(s, B, G, 1, X4);
+    c.mark(PH_NLC_AO);
+    double* T = (double*)c.ws_T.ensure(sizeof(double) * plane);
+    gemm_nn_fast(s, nao, Ppad, nao, d_dm, X4, T, false);   // T = D X (D symmetric)
+    double* rho = (double*)c.ws_rho.ensure(sizeof(double) * 4 * Ppad);
+    launch_rho(s, nao, Ppad, G.npts, T, X4, rho);
+    c.mark(PH_NLC_RHO);
+    double* wv = (double*)c.ws_vv.ensure(sizeof(double) * 5 * Ppad);
+    double* Bm = T;    // T is dead after rho: reuse for B
+    vv10_flow(s, nao, G.nsub, Ppad, G.d_xyz.as<double>(), G.d_w.as<double>(), G.origin, X4, rho,
+              b, Cc, tol, f64, c.ws_B, wv, wv + 4 * Ppad, Bm, d_out2,
+              c.profile ? &c : nullptr);
+    double* V = (double*)c.ws_K.ensure(sizeof(double) * nao * nao);
+    gemm_nt_fast(s, nao, nao, Ppad, X4, Bm, false, 1, 0, nullptr, V, c.ws_Kp);
+    symadd(s, nao, V, d_vmat);
+    c.mark(PH_NLC_FOCK);
+    collect(c);
+}
+
+}  // namespace wb
+
+using namespace wb;
+
+#define WB_TRY(...)                                                      \
+    try {                                                                \
+        __VA_ARGS__;                                                     \
+        return WB_OK;                                                    \
+    } catch (const wb::Error& e) {                                       \
+        set_last_error(e.what());                                        \
+        return e.code;                                                   \
+    } catch (const std::bad_alloc&) {                                    \
+        set_last_error("host allocation failed");                        \
+        return WB_OOM;                                                   \
+    } catch (const std::exception& e) {                                  \
+        set_last_error(e.what());                                        \
+        return WB_DEVICE_ERROR;                                          \
+    }
+
+extern "C" {
+
+const char* wb_last_error(void) { return g_last_error.c_str(); }
+
+int wb_version(void) { return WB_ABI_VERSION; }
+
+int wb_ctx_create(int device, void* stream, wb_ctx_t* out)
+{
+    WB_TRY({
+        if (!out) throw Error(WB_INVALID, "null output");
+        auto* c = new Ctx();
+        c->device = device;
+        WB_HIP(hipSetDevice(device));
+        if (stream) {
+            c->stream = (hipStream_t)stream;
+        } else {
+            WB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+        std::vector<float> t32;
+        std::vector<double> t64;
+        boys_tables(t32, t64);
+        upload(c->d_b32, t32.data(), sizeof(float) * t32.size(), c->stream);
+        upload(c->d_b64, t64.data(), sizeof(double) * t64.size(), c->stream);
+        WB_HIP(hipStreamSynchronize(c->stream));
+        *out = c;
+    })
+}
+
+int wb_ctx_destroy(wb_ctx_t ctx)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c) return WB_OK;
+        WB_HIP(hipSetDevice(c->device));
+        WB_HIP(hipStreamSynchronize(c->stream));
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+    })
+}
+
+int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy)
+{
+    WB_TRY({
+        if (ncopy < 1 || ncopy > 64) throw Error(WB_INVALID, "ncopy out of range");
+        ((Ctx*)ctx)->ncopy = ncopy;
+    })
+}
+
+int wb_ctx_set_g64(wb_ctx_t ctx, int on)
+{
+    WB_TRY({ ((Ctx*)ctx)->g64 = on != 0; })
+}
+
+int wb_basis_create(wb_ctx_t ctx, int nshell, const int* l, const int* nprim,
+                    const double* center, const double* exps, const double* coefs,
+                    double pair_cut, wb_basis_t* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        WB_HIP(hipSetDevice(c->device));
+        auto* b = new Basis();
+        try {
+            build_basis(*b, nshell, l, nprim, center, exps, coefs, pair_cut);
+            upload_basis(*b, c->stream);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    })
+}
+
+int wb_basis_info(wb_basis_t basis, long long* out4)
+{
+    WB_TRY({
+        auto* b = (Basis*)basis;
+        long long np = 0;
+        for (auto& C : b->cls) np += C.nprims;
+        out4[0] = b->nao;
+        out4[1] = b->nshell;
+        out4[2] = b->npairs_total;
+        out4[3] = np;
+    })
+}
+
+int wb_basis_destroy(wb_basis_t basis)
+{
+    WB_TRY({ delete (Basis*)basis; })
+}
+
+int wb_grid_create(wb_ctx_t ctx, int npts, const double* coords, const double* weights,
+                   wb_grid_t* out)
+{
+    WB_TRY({
+        if (npts < 1) throw Error(WB_INVALID, "empty grid");
+        auto* c = (Ctx*)ctx;
+        WB_HIP(hipSetDevice(c->device));
+        auto* g = new Grid();
+        try {
+            build_grid(*g, npts, coords, weights, c->stream);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
+    })
+}
+
+int wb_grid_info(wb_grid_t grid, long long* out3)
+{
+    WB_TRY({
+        auto* g = (Grid*)grid;
+        out3[0] = g->npts;
+        out3[1] = g->Ppad;
+        out3[2] = g->nsub;
+    })
+}
+
+int wb_grid_destroy(wb_grid_t grid)
+{
+    WB_TRY({ delete (Grid*)grid; })
+}
+
+int wb_k_build_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_qd,
+                   int nop, const double* omegas, double tile_tol, int precision,
+                   double* d_k)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (nop < 1 || nop > 8) throw Error(WB_INVALID, "nop out of range");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        WB_HIP(hipSetDevice(c->device));
+        k_build_dev(*c, *(Basis*)basis, *(Grid*)grid, d_qd, nop, omegas, tile_tol, precision, d_k);
+    })
+}
+
+int wb_k_build(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd, int nop,
+               const double* omegas, double tile_tol, int precision, double* k_out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* b = (Basis*)basis;
+        if (nop < 1 || nop > 8) throw Error(WB_INVALID, "nop out of range");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        WB_HIP(hipSetDevice(c->device));
+        const size_t nn = (size_t)b->nao * b->nao;
+        double* d_in = (double*)c->ws_K.ensure(sizeof(double) * nn * (1 + nop));
+        WB_HIP(hipMemcpyAsync(d_in, qd, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        k_build_dev(*c, *b, *(Grid*)grid, d_in, nop, omegas, tile_tol, precision, d_in + nn);
+        WB_HIP(hipMemcpyAsync(k_out, d_in + nn, sizeof(double) * nn * nop, hipMemcpyDeviceToHost,
+                              c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+        c->ms_last = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    })
+}
+
+int wb_k_stats(wb_ctx_t ctx, wb_grid_t grid, long long* out2)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        out2[0] = c->tasks_total;
+        out2[1] = count_kept(*c, *(Grid*)grid);
+    })
+}
+
+int wb_nlc_vxc_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_dm,
+                   double b, double C, double tile_tol, int precision, double* d_out2,
+                   double* d_vmat)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        WB_HIP(hipSetDevice(c->device));
+        nlc_dev(*c, *(Basis*)basis, *(Grid*)grid, d_dm, b, C, tile_tol, precision, d_out2, d_vmat);
+    })
+}
+
+int wb_nlc_vxc(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* dm, double b,
+               double C, double tile_tol, int precision, double* out2, double* vmat)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* bs = (Basis*)basis;
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        WB_HIP(hipSetDevice(c->device));
+        const size_t nn = (size_t)bs->nao * bs->nao;
+        double* din = (double*)c->ws_io.ensure(sizeof(double) * (2 * nn + 2));
+        WB_HIP(hipMemcpyAsync(din, dm, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+        nlc_dev(*c, *bs, *(Grid*)grid, din, b, C, tile_tol, precision, din + 2 * nn, din + nn);
+        WB_HIP(hipMemcpyAsync(vmat, din + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipMemcpyAsync(out2, din + 2 * nn, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, const double* W0,
+                const double* kappa, double tile_tol, int precision, double* F, double* U,
+                double* W)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (n < 0) throw Error(WB_INVALID, "negative size");
+        if (n == 0) return WB_OK;
+        WB_HIP(hipSetDevice(c->device));
+        const bool f64 = precision == WB_PREC_FP64;
+        // points in their given order, tiles of 64 consecutive points
+        double o[3] = {0, 0, 0};
+        for (int i = 0; i < n; ++i) for (int d = 0; d < 3; ++d) o[d] += coords[3 * i + d];
+        for (int d = 0; d < 3; ++d) o[d] /= n;
+        const int ntiles = (n + 63) / 64;
+        std::vector<float4> ts(2 * (size_t)ntiles);
+        std::vector<double> pt(4 * (size_t)n), pk(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            for (int d = 0; d < 3; ++d) pt[4 * i + d] = coords[3 * i + d] - o[d];
+            pt[4 * i + 3] = W0[i];
+            pk[2 * i] = kappa[i];
+            pk[2 * i + 1] = q[i];
+        }
+        for (int t = 0; t < ntiles; ++t) {
+            const int b = t * 64, e = std::min(n, b + 64);
+            double c3[3] = {0, 0, 0}, r = 0, wm = 1e300, km = 1e300, qs = 0;
+            for (int i = b; i < e; ++i) for (int d = 0; d < 3; ++d) c3[d] += pt[4 * i + d] / (e - b);
+            for (int i = b; i < e; ++i) {
+                double dd = 0;
+                for (int d = 0; d < 3; ++d) dd += (pt[4 * i + d] - c3[d]) * (pt[4 * i + d] - c3[d]);
+                r = std::max(r, std::sqrt(dd));
+                wm = std::min(wm, W0[i]); km = std::min(km, kappa[i]); qs += q[i];
+            }
+            ts[2 * t] = make_float4((float)c3[0], (float)c3[1], (float)c3[2], (float)(r * 1.0001 + 1e-5));
+            ts[2 * t + 1] = make_float4((float)(wm * 0.9999), (float)(km * 0.9999), (float)(qs * 1.0001), 0.f);
+        }
+        DevBuf dp, dk, dt, dout;
+        if (f64) {
+            upload(dp, pt.data(), sizeof(double) * pt.size(), c->stream);
+            upload(dk, pk.data(), sizeof(double) * pk.size(), c->stream);
+        } else {
+            std::vector<float> p32(pt.begin(), pt.end()), k32(pk.begin(), pk.end());
+            upload(dp, p32.data(), sizeof(float) * p32.size(), c->stream);
+            upload(dk, k32.data(), sizeof(float) * k32.size(), c->stream);
+            WB_HIP(hipStreamSynchronize(c->stream));
+        }
+        upload(dt, ts.data(), sizeof(float4) * ts.size(), c->stream);
+        double* out = (double*)dout.ensure(sizeof(double) * 3 * (size_t)n);
+        launch_vv10_sum(c->stream, n, dp.p, dk.p, dt.as<float>(), ntiles, (float)tile_tol, f64,
+                        out, out + n, out + 2 * (size_t)n);
+        WB_HIP(hipMemcpyAsync(F, out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipMemcpyAsync(U, out + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipMemcpyAsync(W, out + 2 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, int deriv,
+               double* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* b = (Basis*)basis;
+        if (npts < 1) throw Error(WB_INVALID, "no points");
+        WB_HIP(hipSetDevice(c->device));
+        const int Ppad = (npts + 63) / 64 * 64;
+        std::vector<double> hx(3 * (size_t)Ppad, 0.0);
+        for (int i = 0; i < Ppad; ++i)
+            for (int d = 0; d < 3; ++d) hx[(size_t)d * Ppad + i] = coords[3 * (i < npts ? i : 0) + d];
+        DevBuf dx, dX;
+        upload(dx, hx.data(), sizeof(double) * hx.size(), c->stream);
+        const int nc = deriv ? 4 : 1;
+        const size_t plane = (size_t)b->nao * Ppad;
+        double* X = (double*)dX.ensure(sizeof(double) * nc * plane);
+        launch_ao_eval_pts(c->stream, *b, dx.as<double>(), Ppad, deriv, X);
+        std::vector<double> h(nc * plane);
+        WB_HIP(hipMemcpyAsync(h.data(), X, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+        for (int k = 0; k < nc; ++k)
+            for (int m = 0; m < b->nao; ++m)
+                for (int g = 0; g < npts; ++g)
+                    out[((size_t)k * npts + g) * b->nao + m] = h[k * plane + (size_t)m * Ppad + g];
+    })
+}
+
+int wb_ctx_set_profile(wb_ctx_t ctx, int on)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        c->profile = on != 0;
+        for (double& v : c->phase_ms) v = 0.0;
+    })
+}
+
+int wb_profile_read(wb_ctx_t ctx, double* ms, int n, int reset)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        for (int i = 0; i < n && i < PH_COUNT; ++i) ms[i] = c->phase_ms[i];
+        if (reset) for (double& v : c->phase_ms) v = 0.0;
+    })
+}
+
+int wb_k_work(wb_ctx_t ctx, long long* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        unsigned long long h[2 * NCLASS] = {0};
+        if (c->ws_work.p) {
+            WB_HIP(hipMemcpyAsync(h, c->ws_work.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+            WB_HIP(hipStreamSynchronize(c->stream));
+        }
+        for (int i = 0; i < 2 * NCLASS; ++i) out[i] = (long long)h[i];
+    })
+}
+
+int wb_test_gemm(wb_ctx_t ctx, int kind, int M, int N, int K, const double* A, const double* B,
+                 const double* w, double* C)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        WB_HIP(hipSetDevice(c->device));
+        DevBuf dA, dB, dw, dC, ws;
+        const size_t nb = (size_t)K * N;
+        upload(dA, A, sizeof(double) * (size_t)M * K, c->stream);
+        upload(dB, B, sizeof(double) * nb, c->stream);
+        if (w) upload(dw, w, sizeof(double) * K, c->stream);
+        dC.ensure(sizeof(double) * (size_t)M * N);
+        if (kind == 0)
+            gemm_nn_fast(c->stream, M, N, K, dA.as<double>(), dB.as<double>(), dC.p, false);
+        else if (kind == 1)
+            gemm_nt_fast(c->stream, M, N, K, dA.as<double>(), dB.p, false, 1, 0,
+                         w ? dw.as<double>() : nullptr, dC.as<double>(), ws);
+        else if (kind == 2)
+            gemm_nn(c->stream, M, N, K, dA.as<double>(), dB.as<double>(), dC.p, false);
+        else
+            gemm_nt_reduce(c->stream, M, N, K, dA.as<double>(), dB.p, false, 1, 0,
+                           w ? dw.as<double>() : nullptr, dC.as<double>(), ws);
+        WB_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * (size_t)M * N, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_sync(wb_ctx_t ctx)
+{
+    WB_TRY({ WB_HIP(hipStreamSynchronize(((Ctx*)ctx)->stream)); })
+}
+
+float wb_last_ms(wb_ctx_t ctx) { return ((Ctx*)ctx)->ms_last; }
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// Internal data structures and kernel launchers of the wb97mv_amd library.
+#pragma once
+#include <map>
+#include <memory>
+#include "wb_common.h"
+
+namespace wb {
+
+// ---------------------------------------------------------------- basis ----
+struct ShellInfo {            // device copy, one per (segmented) shell
+    double cx, cy, cz;
+    int l, nprim, prim0, ao0;
+};
+
+struct PairRec {              // one shell pair of a class, la >= lb
+    int a0, b0;               // AO offsets of shells A and B
+    int pp0, npp;             // primitive pairs [pp0, pp0 + npp) in the class arrays
+    int shA, shB;             // shell indices (screening)
+    int diag, pad;            // diag: A == B
+};
+
+struct PairScreen {           // screening data per pair (pair order of its class)
+    float cx, cy, cz;         // centre of the most diffuse primitive pair
+    float delta;              // max distance of any primitive centre from (cx,cy,cz)
+    float beta;               // sum of per-primitive bound prefactors (rounded up)
+    float q;                  // 0.75 * min p (rounded down)
+    float pad0, pad1;
+};
+
+struct ClassData {
+    int la = 0, lb = 0, npairs = 0, nprims = 0;
+    std::vector<PairRec> pairs;
+    std::vector<PairScreen> scr;
+    std::vector<double> P;            // (nprims, 3) primitive-pair centres
+    std::vector<double> p;            // (nprims,) exponents p
+    std::vector<double> pref;         // (nprims,) c_a c_b K_AB 2 pi / p
+    std::vector<double> E;            // (nprims, erl) Hermite E records
+    int erl = 0;                      // E record length
+    DevBuf d_pairs, d_scr, d_P, d_E32, d_E64;
+    std::map<double, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;  // omega -> (C32, C64)
+};
+
+struct Basis {
+    int nshell = 0, nao = 0;
+    std::vector<ShellInfo> shells;
+    std::vector<double> exps, coefs;
+    DevBuf d_shells, d_exps, d_coefs, d_shao;   // d_shao: ao0[nshell], nao[nshell]
+    ClassData cls[NCLASS];
+    double pair_cut = 1e-14;
+    long long npairs_total = 0;
+};
+
+// ----------------------------------------------------------------- grid ----
+struct Grid {
+    int npts = 0, Ppad = 0, nsub = 0;
+    std::vector<int> perm;             // sorted position -> caller's point index
+    std::vector<double> w_sorted;      // host copy (weights in sorted order)
+    double origin[3] = {0, 0, 0};      // centroid (VV10 FP32 frame)
+    DevBuf d_xyz;                      // double [3][Ppad] SoA, padding = first point of block
+    DevBuf d_w;                        // double [Ppad], padding 0
+    DevBuf d_sub;                      // float4 [nsub] centre (relative to nothing: absolute) + radius
+};
+
+// --------------------------------------------------------------- context ---
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    DevBuf d_b32, d_b64;               // Boys tables
+    // workspaces (grow-only)
+    DevBuf ws_X, ws_Xv, ws_F, ws_G, ws_tasks, ws_counts, ws_fmax, ws_Kp, ws_K, ws_D;
+    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io;
+    // statistics of the last K build
+    long long tasks_total = 0, tasks_kept = 0;
+    float ms_last = 0.f;
+    int ncopy = 4;
+    bool g64 = true;           // mixed form: G accumulated in FP64
+    // per-class work of the last K build: kept tasks, kept primitive pairs
+    DevBuf ws_work;
+    // phase timing (HIP events on this context's stream), off by default
+    bool profile = false;
+    std::vector<std::pair<int, hipEvent_t>> marks;     // (phase id, event after it)
+    hipEvent_t t0 = nullptr;
+    double phase_ms[16] = {0};
+    void mark(int phase);
+    void start();
+};
+
+enum Phase { PH_AO = 0, PH_FGEMM, PH_SCREEN, PH_MD, PH_KGEMM, PH_NLC_AO, PH_NLC_RHO,
+             PH_VV10, PH_NLC_FOCK, PH_COUNT };
+
+// --------------------------------------------------------- launchers --------
+void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X);
+void launch_ao_eval_pts(hipStream_t s, const Basis& b, const double* xyz, int Ppad,
+                        int deriv, double* X);
+
+// C[M][N] (+)= A[M][K] B[K][N], row-major; out float or double
+void gemm_nn(hipStream_t s, int M, int N, int K, const double* A, const double* B,
+             void* C, bool out_f32);
+// C[M][N] = sum_k A[M][k] * Bsum[N][k] * (w ? w[k] : 1), Bsum = sum of nb copies
+// (stride bstride) of type float (b_f32) or double; split over K, deterministic.
+void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A,
+                    const void* B, bool b_f32, int nb, size_t bstride,
+                    const double* w, double* C, DevBuf& ws);
+
+// 128 x 128 tile versions (the production paths); gemm_nt_fast folds the
+// copies and weights of B into one FP64 operand first when needed
+void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B,
+                  void* C, bool out_f32);
+void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B,
+                  bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws);
+
+void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
+                 int Ppad, int nsub, const int* sh_ao0, const int* sh_nao, int nshell,
+                 float* fmax);
+void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
+                  int nshell, float tol, int* counts, int* ids, unsigned long long* work);
+// f32: FP32 integrals (else FP64); g64: accumulate G in FP64 (always with FP64 integrals)
+void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
+               const void* F, void* G, size_t gstride, int ncopy,
+               const int* counts, const int* ids, const void* C, const void* boys);
+
+// VV10
+void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T,
+                const double* X, double* rho /* [4][Ppad] */);
+// pts: (x, y, z, W0) and pk: (kappa, q) per point, float or double (f64);
+// tiles: per 64-point tile two float4 (centre xyz, radius | W0min, kmin, qsum, 0)
+void launch_vv10_sum(hipStream_t s, int n, const void* pts, const void* pk,
+                     const float* tiles, int ntiles, float tol, bool f64, double* F,
+                     double* U, double* W);
+
+void zero_bytes(hipStream_t s, void* p, size_t bytes);
+void transpose(hipStream_t s, int n, const double* a, double* b);
+void symadd(hipStream_t s, int n, const double* a, double* c);
+// The VV10 part of nr_nlc_vxc after the density: pointwise terms, compaction,
+// double sum, potential -> B[nu][g] (vmat = X^T B + (X^T B)^T) and
+// sums = (nelec, excsum).
+struct Ctx;
+void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, const double* w,
+               const double* origin, const double* X4, const double* rho, double b, double C,
+               double tol, bool f64, DevBuf& ws, double* wv, double* e, double* Bout,
+               double* sums, Ctx* prof);
+
+}  // namespace wb

@@ -1,0 +1,313 @@
+// Fused McMurchie-Davidson COSX exchange kernels and their screening.
+//
+// For one shell-pair class (la >= lb) and one operator (1/r or erf(w r)/r),
+// a wave owns one sub-block of 64 grid points (one point per lane) and one
+// chunk of that sub-block's screened task list.  Per task (shell pair A, B)
+// it forms the Cartesian block
+//     V_ab(g) = sum_prims sum_tuv Ex Ey Ez R_tuv(p, P - g)
+// in registers (never stored), and contracts it with F (F = X Q D):
+//     G[g, B] += A(g)^T F[g, A],   G[g, A] += A(g) F[g, B]  (A != B)
+// Tasks are sorted by A, so G[A] accumulates in registers along a run of
+// tasks with the same A and is written once per run.  Each chunk writes its
+// own copy of G (no atomics, fixed order, bit-reproducible); the copies are
+// summed inside the K GEMM.  The reference does the same contraction with
+// one warp per 32-point sub-block (gpu/sgx_k.py:335-400).
+#include "wb_boys.h"
+#include "wb_internal.h"
+#include "md_gen.h"
+
+namespace wb {
+
+template <int L> struct MdDispatch;
+
+template <int LA, int LB, typename real>
+__device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R0,
+                                        const real* E, real* V);
+
+#define WB_MD_PRIM(la, lb)                                                        \
+    template <> __device__ __forceinline__ void md_prim<la, lb, float>(           \
+        float dx, float dy, float dz, const float* R0, const float* E, float* V) { \
+        md_prim_##la##lb<float>(dx, dy, dz, R0, E, V);                           \
+    }                                                                             \
+    template <> __device__ __forceinline__ void md_prim<la, lb, double>(          \
+        double dx, double dy, double dz, const double* R0, const double* E,       \
+        double* V) {                                                              \
+        md_prim_##la##lb<double>(dx, dy, dz, R0, E, V);                          \
+    }
+WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
+WB_MD_PRIM(2, 2) WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1) WB_MD_PRIM(3, 2) WB_MD_PRIM(3, 3)
+#undef WB_MD_PRIM
+
+template <int l, typename real>
+__device__ __forceinline__ void fcart(const real* Fs, real* Fc) {
+    if constexpr (l == 0) md_fcart_0<real>(Fs, Fc);
+    else if constexpr (l == 1) md_fcart_1<real>(Fs, Fc);
+    else if constexpr (l == 2) md_fcart_2<real>(Fs, Fc);
+    else md_fcart_3<real>(Fs, Fc);
+}
+template <int l, typename real>
+__device__ __forceinline__ void gsph(const real* Gc, real* Gs) {
+    if constexpr (l == 0) md_gsph_0<real>(Gc, Gs);
+    else if constexpr (l == 1) md_gsph_1<real>(Gc, Gs);
+    else if constexpr (l == 2) md_gsph_2<real>(Gc, Gs);
+    else md_gsph_3<real>(Gc, Gs);
+}
+
+constexpr int MD_WAVES = 4;
+
+// real: integral arithmetic; gacc: G accumulation (registers and memory)
+template <int LA, int LB, typename real, typename gacc>
+__global__ __launch_bounds__(64 * MD_WAVES)
+void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
+                 gacc* __restrict__ G, size_t gstride,
+                 const int* __restrict__ counts, const int* __restrict__ ids, int cap,
+                 const PairRec* __restrict__ pairs, const double* __restrict__ ppP,
+                 const real* __restrict__ ppE, const real* __restrict__ ppC,
+                 const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk)
+{
+    constexpr int L = LA + LB;
+    constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
+    constexpr int NSA = 2 * LA + 1, NSB = 2 * LB + 1;
+    constexpr bool F32 = sizeof(real) == 4;
+    constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1) * ((WB_ECOMP && F32) ? 2 : 1);
+    constexpr int CRL = (L + 2) * (F32 ? 2 : 1);   // FP32: (hi, lo) pairs
+    constexpr int TAB = F32 ? B32_NODES * B32_ROW : 1;
+    __shared__ real s_tab[TAB];
+    const real* tab = boys_g;
+    if constexpr (F32) {
+        for (int i = threadIdx.x; i < TAB; i += blockDim.x) s_tab[i] = boys_g[i];
+        __syncthreads();
+        tab = s_tab;
+    }
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int worker = blockIdx.x * MD_WAVES + wave;
+    if (worker >= nsub * nchunk) return;
+    const int s = worker % nsub, j = worker / nsub;
+    const int cnt = counts[s];
+    const int t0 = (int)(((long long)cnt * j) / nchunk);
+    const int t1 = (int)(((long long)cnt * (j + 1)) / nchunk);
+    if (t0 >= t1) return;
+    const int* tl = ids + (size_t)s * cap;
+    gacc* Gc = G + (size_t)j * gstride;
+    const int g = s * SUB + lane;
+    const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
+
+    int curA = -1;
+    real FaC[NCA];
+    gacc acc[NCA];
+    int a0 = 0;
+    for (int t = t0; t < t1; ++t) {
+        const int k = tl[t];
+        const PairRec pr = pairs[k];
+        if (pr.shA != curA) {
+            if (curA >= 0) {
+                gacc gs[NSA];
+                gsph<LA, gacc>(acc, gs);
+#pragma unroll
+                for (int m = 0; m < NSA; ++m) Gc[(size_t)(a0 + m) * Ppad + g] += gs[m];
+            }
+            curA = pr.shA;
+            a0 = pr.a0;
+            real fa[NSA];
+#pragma unroll
+            for (int m = 0; m < NSA; ++m) fa[m] = F[(size_t)(a0 + m) * Ppad + g];
+            fcart<LA, real>(fa, FaC);
+#pragma unroll
+            for (int c = 0; c < NCA; ++c) acc[c] = (gacc)0;
+        }
+        real V[NCA * NCB];
+#pragma unroll
+        for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)0;
+        for (int m = pr.pp0; m < pr.pp0 + pr.npp; ++m) {
+            const real dx = (real)(ppP[3 * m] - px);
+            const real dy = (real)(ppP[3 * m + 1] - py);
+            const real dz = (real)(ppP[3 * m + 2] - pz);
+            const real* C = ppC + (size_t)m * CRL;
+            const real r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            real Fb[L + 1], R0[L + 1];
+            if constexpr (F32) {
+                Boys<real>::template eval<L>(fmaf(C[0], r2, C[1] * r2), tab, Fb);
+#pragma unroll
+                for (int n = 0; n <= L; ++n) R0[n] = fmaf(C[2 + 2 * n], Fb[n], C[3 + 2 * n] * Fb[n]);
+            } else {
+                Boys<real>::template eval<L>(C[0] * r2, tab, Fb);
+#pragma unroll
+                for (int n = 0; n <= L; ++n) R0[n] = C[1 + n] * Fb[n];
+            }
+            md_prim<LA, LB, real>(dx, dy, dz, R0, ppE + (size_t)m * ERL, V);
+        }
+        real fb[NSB], FbC[NCB];
+#pragma unroll
+        for (int m = 0; m < NSB; ++m) fb[m] = F[(size_t)(pr.b0 + m) * Ppad + g];
+        fcart<LB, real>(fb, FbC);
+        real gbc[NCB];
+#pragma unroll
+        for (int b = 0; b < NCB; ++b) {
+            real v = (real)0;
+#pragma unroll
+            for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + b], FaC[a], v);
+            gbc[b] = v;
+        }
+        real gbs[NSB];
+        gsph<LB, real>(gbc, gbs);
+#pragma unroll
+        for (int m = 0; m < NSB; ++m) Gc[(size_t)(pr.b0 + m) * Ppad + g] += (gacc)gbs[m];
+        if (!pr.diag) {
+#pragma unroll
+            for (int a = 0; a < NCA; ++a) {
+                real v = (real)0;
+#pragma unroll
+                for (int b = 0; b < NCB; ++b) v = fma(V[a * NCB + b], FbC[b], v);
+                acc[a] += (gacc)v;
+            }
+        }
+    }
+    gacc gs[NSA];
+    gsph<LA, gacc>(acc, gs);
+#pragma unroll
+    for (int m = 0; m < NSA; ++m) Gc[(size_t)(a0 + m) * Ppad + g] += gs[m];
+}
+
+template <int LA, int LB, typename real, typename gacc>
+static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
+                        void* G, size_t gstride, int ncopy, const int* counts,
+                        const int* ids, const void* C, const void* boys)
+{
+    const int workers = g.nsub * ncopy;
+    const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
+    const real* E = (sizeof(real) == 4) ? c.d_E32.as<real>() : c.d_E64.as<real>();
+    hipLaunchKernelGGL((md_k_kernel<LA, LB, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES), 0, s,
+                       g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, counts, ids,
+                       c.npairs, c.d_pairs.as<PairRec>(), c.d_P.as<double>(), E,
+                       (const real*)C, (const real*)boys, g.Ppad, g.nsub, ncopy);
+    WB_HIP(hipGetLastError());
+}
+
+void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
+               const void* F, void* G, size_t gstride, int ncopy, const int* counts,
+               const int* ids, const void* C, const void* boys)
+{
+    if (c.npairs == 0) return;
+#define WB_CASE(XA, XB)                                                                \
+    if (c.la == XA && c.lb == XB) {                                                    \
+        if (f32 && !g64)                                                               \
+            md_launch_t<XA, XB, float, float>(s, c, g, F, G, gstride, ncopy, counts, ids, C, boys); \
+        else if (f32)                                                                  \
+            md_launch_t<XA, XB, float, double>(s, c, g, F, G, gstride, ncopy, counts, ids, C, boys); \
+        else                                                                           \
+            md_launch_t<XA, XB, double, double>(s, c, g, F, G, gstride, ncopy, counts, ids, C, boys); \
+        return;                                                                        \
+    }
+    WB_CASE(0, 0) WB_CASE(1, 0) WB_CASE(1, 1) WB_CASE(2, 0) WB_CASE(2, 1)
+    WB_CASE(2, 2) WB_CASE(3, 0) WB_CASE(3, 1) WB_CASE(3, 2) WB_CASE(3, 3)
+#undef WB_CASE
+    throw Error(WB_UNSUPPORTED, "shell-pair class beyond (f, f)");
+}
+
+// ------------------------------------------------------------ screening ----
+// fmax[s][sh] = max_{g in s} w_g * sum_{mu in sh} |F[mu][g]|
+template <typename real>
+__global__ __launch_bounds__(256)
+void fmax_kernel(const real* __restrict__ F, const double* __restrict__ w, int Ppad,
+                 int nsub, const int* __restrict__ sh_ao0, const int* __restrict__ sh_nao,
+                 int nshell, float* __restrict__ fmax)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = blockIdx.x * 4 + wave;
+    if (s >= nsub) return;
+    const int lane = threadIdx.x & 63;
+    const int g = s * SUB + lane;
+    const double wg = w[g];
+    for (int sh = 0; sh < nshell; ++sh) {
+        const int a0 = sh_ao0[sh], n = sh_nao[sh];
+        double v = 0.0;
+        for (int m = 0; m < n; ++m) v += fabs((double)F[(size_t)(a0 + m) * Ppad + g]);
+        float f = (float)(v * wg);
+        f = wave_max(f);
+        if (lane == 0) fmax[(size_t)s * nshell + sh] = f * 1.0001f;
+    }
+}
+
+void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao, int Ppad,
+                 int nsub, const int* sh_ao0, const int* sh_nao, int nshell, float* fmax)
+{
+    const int blocks = (nsub + 3) / 4;
+    if (f32)
+        hipLaunchKernelGGL(fmax_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                           (const float*)F, w, Ppad, nsub, sh_ao0, sh_nao, nshell, fmax);
+    else
+        hipLaunchKernelGGL(fmax_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                           (const double*)F, w, Ppad, nsub, sh_ao0, sh_nao, nshell, fmax);
+    WB_HIP(hipGetLastError());
+}
+
+// One wave per (sub-block): the kept pair ids of one class, in pair order.
+// est = beta * F0bound(q d_min^2) * max(fmax[A], fmax[B]); F0(x) <= min(1,
+// sqrt(pi/(4x))).  tol <= 0 keeps every pair.
+// work[0] += kept tasks, work[1] += their primitive pairs (integer atomics:
+// the totals do not depend on the order)
+__global__ __launch_bounds__(256)
+void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__ pairs,
+                 int npairs, const float4* __restrict__ sub, int nsub,
+                 const float* __restrict__ fmax, int nshell, float tol,
+                 int* __restrict__ counts, int* __restrict__ ids,
+                 unsigned long long* __restrict__ work)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int s = blockIdx.x * 4 + wave;
+    if (s >= nsub) return;
+    const int lane = threadIdx.x & 63;
+    const float4 sb = sub[s];
+    const float* fm = fmax + (size_t)s * nshell;
+    int* out = ids + (size_t)s * npairs;
+    int n = 0;
+    unsigned long long npp = 0;
+    for (int k0 = 0; k0 < npairs; k0 += 64) {
+        const int k = k0 + lane;
+        bool keep = false;
+        if (k < npairs) {
+            if (tol <= 0.f) {
+                keep = true;
+            } else {
+                const PairScreen ps = scr[k];
+                const PairRec pr = pairs[k];
+                const float dx = ps.cx - sb.x, dy = ps.cy - sb.y, dz = ps.cz - sb.z;
+                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                float d = dist - sb.w - ps.delta;
+                d -= 1e-5f * (dist + sb.w + ps.delta) + 1e-6f;   // FP32 safety
+                float f0 = 1.0f;
+                if (d > 0.f) f0 = fminf(1.0f, 0.8862269254527580f / (sqrtf(ps.q) * d));
+                const float est = ps.beta * f0 * fmaxf(fm[pr.shA], fm[pr.shB]);
+                keep = est * 1.001f >= tol;
+            }
+        }
+        const unsigned long long bal = __ballot(keep);
+        const int pos = n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+        if (keep) {
+            out[pos] = k;
+            npp += (unsigned long long)pairs[k].npp;
+        }
+        n += __popcll(bal);
+    }
+    for (int o = 32; o > 0; o >>= 1) npp += __shfl_xor(npp, o, 64);
+    if (lane == 0) {
+        counts[s] = n;
+        atomicAdd(work, (unsigned long long)n);
+        atomicAdd(work + 1, npp);
+    }
+}
+
+void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
+                  int nshell, float tol, int* counts, int* ids, unsigned long long* work)
+{
+    if (c.npairs == 0) return;
+    const int blocks = (g.nsub + 3) / 4;
+    hipLaunchKernelGGL(task_kernel, dim3(blocks), dim3(256), 0, s, c.d_scr.as<PairScreen>(),
+                       c.d_pairs.as<PairRec>(), c.npairs, g.d_sub.as<float4>(), g.nsub, fmax,
+                       nshell, tol, counts, ids, work);
+    WB_HIP(hipGetLastError());
+}
+
+}  // namespace wb

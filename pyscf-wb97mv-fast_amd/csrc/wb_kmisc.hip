@@ -1,0 +1,1028 @@
+// AO evaluation, FP64 MFMA GEMMs, GGA density and the VV10 double sum.
+#include "wb_internal.h"
+#include "md_gen.h"
+
+namespace wb {
+
+// ------------------------------------------------------------ AO values ---
+// X[c][mu][g] (c = 0 values, 1..3 gradients), grid-major so that the 64
+// lanes of a wave (64 consecutive points) store 512 contiguous bytes.
+// Follows the evaluation of the reference's AoEvaluator (gpu/ao_eval.py:
+// 255-329): radial sum in FP64 with exp in FP64, Cartesian powers, then
+// cart -> sph (libcint order).
+template <int l, int DERIV>
+__device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double rad,
+                                         double drad, double* __restrict__ X, size_t plane,
+                                         int ao0, int Ppad, int g)
+{
+    constexpr int NC = (l + 1) * (l + 2) / 2, NS = 2 * l + 1;
+    double px[l + 2], py[l + 2], pz[l + 2];
+    px[0] = py[0] = pz[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < l + 2; ++k) {
+        px[k] = px[k - 1] * dx;
+        py[k] = py[k - 1] * dy;
+        pz[k] = pz[k - 1] * dz;
+    }
+    double cv[NC], gx[NC], gy[NC], gz[NC];
+    int c = 0;
+#pragma unroll
+    for (int lx = l; lx >= 0; --lx) {
+#pragma unroll
+        for (int ly = l - lx; ly >= 0; --ly) {
+            const int lz = l - lx - ly;
+            const double poly = px[lx] * py[ly] * pz[lz];
+            cv[c] = poly * rad;
+            if (DERIV) {
+                gx[c] = py[ly] * pz[lz] * ((lx ? lx * px[lx - 1] : 0.0) * rad + px[lx + 1] * drad);
+                gy[c] = px[lx] * pz[lz] * ((ly ? ly * py[ly - 1] : 0.0) * rad + py[ly + 1] * drad);
+                gz[c] = px[lx] * py[ly] * ((lz ? lz * pz[lz - 1] : 0.0) * rad + pz[lz + 1] * drad);
+            }
+            ++c;
+        }
+    }
+    double sv[NS];
+    if constexpr (l == 0) md_gsph_0<double>(cv, sv);
+    else if constexpr (l == 1) md_gsph_1<double>(cv, sv);
+    else if constexpr (l == 2) md_gsph_2<double>(cv, sv);
+    else md_gsph_3<double>(cv, sv);
+#pragma unroll
+    for (int m = 0; m < NS; ++m) X[(size_t)(ao0 + m) * Ppad + g] = sv[m];
+    if (DERIV) {
+        const double* src[3] = {gx, gy, gz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if constexpr (l == 0) md_gsph_0<double>(src[ax], sv);
+            else if constexpr (l == 1) md_gsph_1<double>(src[ax], sv);
+            else if constexpr (l == 2) md_gsph_2<double>(src[ax], sv);
+            else md_gsph_3<double>(src[ax], sv);
+#pragma unroll
+            for (int m = 0; m < NS; ++m)
+                X[(ax + 1) * plane + (size_t)(ao0 + m) * Ppad + g] = sv[m];
+        }
+    }
+}
+
+template <int DERIV>
+__global__ __launch_bounds__(256)
+void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
+               const double* __restrict__ exps, const double* __restrict__ coefs,
+               const double* __restrict__ xyz, int Ppad, double* __restrict__ X, size_t plane)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= Ppad) return;
+    const double x = xyz[g], y = xyz[Ppad + g], z = xyz[2 * Ppad + g];
+    const int s0 = blockIdx.y * spb, s1 = min(nshell, s0 + spb);
+    for (int is = s0; is < s1; ++is) {
+        const ShellInfo sh = shells[is];
+        const double dx = x - sh.cx, dy = y - sh.cy, dz = z - sh.cz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double rad = 0.0, drad = 0.0;
+        for (int i = 0; i < sh.nprim; ++i) {
+            const double a = exps[sh.prim0 + i];
+            const double ar2 = a * r2;
+            if (ar2 < 740.0) {
+                const double ce = coefs[sh.prim0 + i] * exp(-ar2);
+                rad += ce;
+                drad -= 2.0 * a * ce;
+            }
+        }
+        switch (sh.l) {
+        case 0: ao_shell<0, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
+        case 1: ao_shell<1, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
+        case 2: ao_shell<2, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
+        default: ao_shell<3, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
+        }
+    }
+}
+
+void launch_ao_eval_pts(hipStream_t s, const Basis& b, const double* xyz, int Ppad, int deriv,
+                        double* X)
+{
+    const int spb = 16;
+    dim3 grid((Ppad + 255) / 256, (b.nshell + spb - 1) / spb);
+    const size_t plane = (size_t)b.nao * Ppad;
+    if (deriv)
+        hipLaunchKernelGGL(ao_kernel<1>, grid, dim3(256), 0, s, b.d_shells.as<ShellInfo>(),
+                           b.nshell, spb, b.d_exps.as<double>(), b.d_coefs.as<double>(), xyz,
+                           Ppad, X, plane);
+    else
+        hipLaunchKernelGGL(ao_kernel<0>, grid, dim3(256), 0, s, b.d_shells.as<ShellInfo>(),
+                           b.nshell, spb, b.d_exps.as<double>(), b.d_coefs.as<double>(), xyz,
+                           Ppad, X, plane);
+    WB_HIP(hipGetLastError());
+}
+
+void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X)
+{
+    launch_ao_eval_pts(s, b, g.d_xyz.as<double>(), g.Ppad, deriv, X);
+}
+
+// ---------------------------------------------------------- FP64 GEMMs ----
+// v_mfma_f64_16x16x4_f64: lane l holds A[l%16][l/16], B[l/16][l%16] and
+// C[4*r + l/16][l%16] in accumulator element r = 0..3.  A 256-thread block computes a 64x64
+// tile, each wave a 32x32 quarter (2x2 MFMA tiles), BK = 16 per LDS stage.
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int GB_M = 64, GB_N = 64, GB_K = 16;
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Loads a 64 (rows) x 16 (k) tile of a row-major matrix with k contiguous
+// into S[k][row]; out-of-range entries are zero.
+__device__ __forceinline__ void load_rows_k(const double* __restrict__ A, int lda, int rows,
+                                            int kdim, int r0, int k0,
+                                            double (*S)[GB_M + 1])
+{
+    const int t = threadIdx.x;
+    const int r = t >> 2, kq = (t & 3) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int rr = r0 + r, kk = k0 + kq + i;
+        S[kq + i][r] = (rr < rows && kk < kdim) ? A[(size_t)rr * lda + kk] : 0.0;
+    }
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(256)
+void gemm_nn_kernel(int M, int N, int K, const double* __restrict__ A,
+                    const double* __restrict__ B, OutT* __restrict__ C)
+{
+    __shared__ double As[GB_K][GB_M + 1];
+    __shared__ double Bs[GB_K][GB_N + 1];
+    const int m0 = blockIdx.y * GB_M, n0 = blockIdx.x * GB_N;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+    for (int k0 = 0; k0 < K; k0 += GB_K) {
+        load_rows_k(A, K, M, K, m0, k0, As);
+        {   // B[k][n], n contiguous
+            const int kr = t >> 4, nq = (t & 15) * 4;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kk = k0 + kr, nn = n0 + nq + i;
+                Bs[kr][nq + i] = (kk < K && nn < N) ? B[(size_t)kk * N + nn] : 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk += 4) {
+            const int kr = kk + (lane >> 4), c = lane & 15;
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[kr][wm + i * 16 + c];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn + j * 16 + c];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int mm = m0 + wm + i * 16 + 4 * r + (lane >> 4);
+                const int nn = n0 + wn + j * 16 + (lane & 15);
+                if (mm < M && nn < N) C[(size_t)mm * N + nn] = (OutT)acc[i][j][r];
+            }
+}
+
+void gemm_nn(hipStream_t s, int M, int N, int K, const double* A, const double* B, void* C,
+             bool out_f32)
+{
+    dim3 grid((N + GB_N - 1) / GB_N, (M + GB_M - 1) / GB_M);
+    if (out_f32)
+        hipLaunchKernelGGL(gemm_nn_kernel<float>, grid, dim3(256), 0, s, M, N, K, A, B,
+                           (float*)C);
+    else
+        hipLaunchKernelGGL(gemm_nn_kernel<double>, grid, dim3(256), 0, s, M, N, K, A, B,
+                           (double*)C);
+    WB_HIP(hipGetLastError());
+}
+
+// Partial C tiles of one K split: Cp[z][M][N] = sum_{k in split z} A[m][k] B'[n][k],
+// B'[n][k] = w[k] * sum_c Bc[n][k].
+template <typename BT>
+__global__ __launch_bounds__(256)
+void gemm_nt_kernel(int M, int N, int K, int kchunk, const double* __restrict__ A,
+                    const BT* __restrict__ B, int nb, size_t bstride,
+                    const double* __restrict__ w, double* __restrict__ Cp)
+{
+    __shared__ double As[GB_K][GB_M + 1];
+    __shared__ double Bs[GB_K][GB_N + 1];
+    const int m0 = blockIdx.y * GB_M, n0 = blockIdx.x * GB_N, z = blockIdx.z;
+    const int kb = z * kchunk, ke = min(K, kb + kchunk);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+    for (int k0 = kb; k0 < ke; k0 += GB_K) {
+        load_rows_k(A, K, M, ke, m0, k0, As);
+        {
+            const int r = t >> 2, kq = (t & 3) * 4;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int nn = n0 + r, kk = k0 + kq + i;
+                double v = 0.0;
+                if (nn < N && kk < ke) {
+                    const BT* p = B + (size_t)nn * K + kk;
+                    for (int c = 0; c < nb; ++c) v += (double)p[c * bstride];
+                    if (w) v *= w[kk];
+                }
+                Bs[kq + i][r] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk += 4) {
+            const int kr = kk + (lane >> 4), c = lane & 15;
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[kr][wm + i * 16 + c];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn + j * 16 + c];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    double* out = Cp + (size_t)z * M * N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int mm = m0 + wm + i * 16 + 4 * r + (lane >> 4);
+                const int nn = n0 + wn + j * 16 + (lane & 15);
+                if (mm < M && nn < N) out[(size_t)mm * N + nn] = acc[i][j][r];
+            }
+}
+
+__global__ void split_reduce_kernel(int MN, int nsplit, const double* __restrict__ Cp,
+                                    double* __restrict__ C)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= MN) return;
+    double v = 0.0;
+    for (int z = 0; z < nsplit; ++z) v += Cp[(size_t)z * MN + i];
+    C[i] = v;
+}
+
+void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A, const void* B,
+                    bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws)
+{
+    const int tiles = ((M + GB_M - 1) / GB_M) * ((N + GB_N - 1) / GB_N);
+    int nsplit = std::max(1, std::min(4096 / std::max(tiles, 1), K / (GB_K * 32)));
+    int kchunk = (K + nsplit - 1) / nsplit;
+    kchunk = (kchunk + GB_K - 1) / GB_K * GB_K;
+    nsplit = (K + kchunk - 1) / kchunk;
+    double* Cp = (double*)ws.ensure(sizeof(double) * (size_t)nsplit * M * N);
+    dim3 grid((N + GB_N - 1) / GB_N, (M + GB_M - 1) / GB_M, nsplit);
+    if (b_f32)
+        hipLaunchKernelGGL(gemm_nt_kernel<float>, grid, dim3(256), 0, s, M, N, K, kchunk, A,
+                           (const float*)B, nb, bstride, w, Cp);
+    else
+        hipLaunchKernelGGL(gemm_nt_kernel<double>, grid, dim3(256), 0, s, M, N, K, kchunk, A,
+                           (const double*)B, nb, bstride, w, Cp);
+    WB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
+                       nsplit, Cp, C);
+    WB_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------- 128x128 FP64 GEMMs ----
+// Block tile 128 x 128, 4 waves of 64 x 64 (4 x 4 MFMA tiles), BK = 16;
+// the next K slice is fetched into registers while the current one is
+// consumed from LDS (two LDS buffers, one barrier per slice).  Block ids are
+// remapped so that consecutive work items run on the same XCD (blocks are
+// dealt round-robin over the 8 XCDs), which keeps the operand slices a
+// group of tiles shares in one L2.
+constexpr int T_M = 128, T_N = 128, T_K = 16, T_PAD = 2;
+constexpr int NXCD = 8;
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
+{
+    const int per = (nblocks + NXCD - 1) / NXCD;
+    const int xcd = bid % NXCD, local = bid / NXCD;
+    // XCDs with an index below nblocks % NXCD get one block more
+    const int rem = nblocks % NXCD;
+    const int base = (rem == 0 || xcd < rem) ? xcd * per : rem * per + (xcd - rem) * (per - 1);
+    return base + local;
+}
+
+// Row-major rows x K (k contiguous) slice [r0, r0+128) x [k0, k0+16) into
+// registers: thread t holds row t/2, k half (t%2)*8 .. +8.
+__device__ __forceinline__ void fetch_rows(const double* __restrict__ A, int lda, int rows,
+                                           int kend, int r0, int k0, double (&v)[8])
+{
+    const int t = threadIdx.x, r = r0 + (t >> 1), kb = k0 + (t & 1) * 8;
+    if (r < rows && kb + 8 <= kend) {
+        const double2* p = (const double2*)(A + (size_t)r * lda + kb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { double2 q = p[i]; v[2 * i] = q.x; v[2 * i + 1] = q.y; }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (r < rows && kb + i < kend) ? A[(size_t)r * lda + kb + i] : 0.0;
+    }
+}
+__device__ __forceinline__ void store_rows(double (*S)[T_M + T_PAD], const double (&v)[8])
+{
+    const int t = threadIdx.x, r = t >> 1, kq = (t & 1) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) S[kq + i][r] = v[i];
+}
+// K x N (n contiguous) slice [k0, k0+16) x [n0, n0+128): thread t holds k
+// row t/16, n columns (t%16)*8 .. +8
+__device__ __forceinline__ void fetch_cols(const double* __restrict__ B, int ldb, int kdim,
+                                           int ncols, int k0, int n0, double (&v)[8])
+{
+    const int t = threadIdx.x, k = k0 + (t >> 4), nb = n0 + (t & 15) * 8;
+    if (k < kdim && nb + 8 <= ncols) {
+        const double2* p = (const double2*)(B + (size_t)k * ldb + nb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { double2 q = p[i]; v[2 * i] = q.x; v[2 * i + 1] = q.y; }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (k < kdim && nb + i < ncols) ? B[(size_t)k * ldb + nb + i] : 0.0;
+    }
+}
+__device__ __forceinline__ void store_cols(double (*S)[T_N + T_PAD], const double (&v)[8])
+{
+    const int t = threadIdx.x, k = t >> 4, nq = (t & 15) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) S[k][nq + i] = v[i];
+}
+
+__device__ __forceinline__ void mma_slice(const double (*As)[T_M + T_PAD],
+                                          const double (*Bs)[T_N + T_PAD], int wm, int wn,
+                                          d4 (&acc)[4][4])
+{
+    const int lane = threadIdx.x & 63, c = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < T_K; kk += 4) {
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[kk + kq][wm + i * 16 + c];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[kk + kq][wn + j * 16 + c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+    }
+}
+
+template <typename OutT>
+__device__ __forceinline__ void store_tile(OutT* __restrict__ C, int ldc, int M, int N, int m0,
+                                           int n0, int wm, int wn, const d4 (&acc)[4][4])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int mm = m0 + wm + i * 16 + 4 * r + (lane >> 4);
+                const int nn = n0 + wn + j * 16 + (lane & 15);
+                if (mm < M && nn < N) C[(size_t)mm * ldc + nn] = (OutT)acc[i][j][r];
+            }
+}
+
+// C[M][N] = A[M][K] B[K][N]; grid: m tiles fastest (they share the B slice)
+template <typename OutT>
+__global__ __launch_bounds__(256, 2)
+void gemm_nn128_kernel(int M, int N, int K, const double* __restrict__ A,
+                       const double* __restrict__ B, OutT* __restrict__ C, int mt)
+{
+    __shared__ double As[2][T_K][T_M + T_PAD];
+    __shared__ double Bs[2][T_K][T_N + T_PAD];
+    const int nblocks = gridDim.x;
+    const int w = xcd_remap(blockIdx.x, nblocks);
+    const int m0 = (w % mt) * T_M, n0 = (w / mt) * T_N;
+    const int wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+    double va[8], vb[8];
+    fetch_rows(A, K, M, K, m0, 0, va);
+    fetch_cols(B, N, K, N, 0, n0, vb);
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += T_K) {
+        store_rows(As[buf], va);
+        store_cols(Bs[buf], vb);
+        __syncthreads();
+        if (k0 + T_K < K) {
+            fetch_rows(A, K, M, K, m0, k0 + T_K, va);
+            fetch_cols(B, N, K, N, k0 + T_K, n0, vb);
+        }
+        mma_slice(As[buf], Bs[buf], wm, wn, acc);
+        buf ^= 1;
+    }
+    store_tile(C, N, M, N, m0, n0, wm, wn, acc);
+}
+
+// Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k]; grid: tiles fastest
+// within a split (they share the A and B slices of that split)
+__global__ __launch_bounds__(256, 2)
+void gemm_nt128_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
+                       const double* __restrict__ A, const double* __restrict__ B,
+                       double* __restrict__ Cp)
+{
+    __shared__ double As[2][T_K][T_M + T_PAD];
+    __shared__ double Bs[2][T_K][T_N + T_PAD];
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = w % ntile, z = w / ntile;
+    const int m0 = (tile % mt) * T_M, n0 = (tile / mt) * T_N;
+    const int kb = z * kchunk, ke = min(K, kb + kchunk);
+    const int wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    d4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+    double va[8], vb[8];
+    fetch_rows(A, K, M, ke, m0, kb, va);
+    fetch_rows(B, K, N, ke, n0, kb, vb);
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += T_K) {
+        store_rows(As[buf], va);
+        store_rows((double (*)[T_M + T_PAD])Bs[buf], vb);
+        __syncthreads();
+        if (k0 + T_K < ke) {
+            fetch_rows(A, K, M, ke, m0, k0 + T_K, va);
+            fetch_rows(B, K, N, ke, n0, k0 + T_K, vb);
+        }
+        mma_slice(As[buf], Bs[buf], wm, wn, acc);
+        buf ^= 1;
+    }
+    store_tile(Cp + (size_t)z * M * N, N, M, N, m0, n0, wm, wn, acc);
+}
+
+// out[i] = w[i % K] * sum_c src[c][i]  (the G copies and the grid weights
+// folded into one FP64 operand), i over nrow x K
+template <typename BT>
+__global__ __launch_bounds__(256)
+void fold_copies_kernel(size_t n, int K, const BT* __restrict__ src, int nb, size_t stride,
+                        const double* __restrict__ w, double* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    for (int c = 0; c < nb; ++c) v += (double)src[c * stride + i];
+    if (w) v *= w[i % K];
+    out[i] = v;
+}
+
+void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B, void* C,
+                  bool out_f32)
+{
+    const int mt = (M + T_M - 1) / T_M, nt = (N + T_N - 1) / T_N;
+    dim3 grid(mt * nt);
+    if (out_f32)
+        hipLaunchKernelGGL(gemm_nn128_kernel<float>, grid, dim3(256), 0, s, M, N, K, A, B,
+                           (float*)C, mt);
+    else
+        hipLaunchKernelGGL(gemm_nn128_kernel<double>, grid, dim3(256), 0, s, M, N, K, A, B,
+                           (double*)C, mt);
+    WB_HIP(hipGetLastError());
+}
+
+void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B, bool b_f32,
+                  int nb, size_t bstride, const double* w, double* C, DevBuf& ws)
+{
+    // fold the copies / weights into one FP64 operand when needed
+    const double* Bd = (const double*)B;
+    const size_t nbk = (size_t)N * K;
+    const int mt = (M + T_M - 1) / T_M, ntile = mt * ((N + T_N - 1) / T_N);
+    int nsplit = std::max(1, std::min(96, K / (T_K * 64)));
+    int kchunk = (K + nsplit - 1) / nsplit;
+    kchunk = (kchunk + T_K - 1) / T_K * T_K;
+    nsplit = (K + kchunk - 1) / kchunk;
+    const bool fold = b_f32 || nb != 1 || w != nullptr;
+    const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
+    char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * nbk : 0));
+    double* Cp = (double*)base;
+    if (fold) {
+        double* Bf = (double*)(base + part_bytes);
+        const unsigned blocks = (unsigned)((nbk + 255) / 256);
+        if (b_f32)
+            hipLaunchKernelGGL(fold_copies_kernel<float>, dim3(blocks), dim3(256), 0, s, nbk, K,
+                               (const float*)B, nb, bstride, w, Bf);
+        else
+            hipLaunchKernelGGL(fold_copies_kernel<double>, dim3(blocks), dim3(256), 0, s, nbk, K,
+                               (const double*)B, nb, bstride, w, Bf);
+        WB_HIP(hipGetLastError());
+        Bd = Bf;
+    }
+    hipLaunchKernelGGL(gemm_nt128_kernel, dim3(ntile * nsplit), dim3(256), 0, s, M, N, K, kchunk,
+                       mt, ntile, A, Bd, Cp);
+    WB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
+                       nsplit, Cp, C);
+    WB_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------- GGA density ---
+// rho[0][g] = sum_mu T[mu][g] X[mu][g], rho[1+d][g] = 2 sum_mu T[mu][g] dX_d[mu][g]
+__global__ __launch_bounds__(256)
+void rho_kernel(int nao, int Ppad, const double* __restrict__ T, const double* __restrict__ X,
+                double* __restrict__ rho)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= Ppad) return;
+    const size_t plane = (size_t)nao * Ppad;
+    double r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    for (int m = 0; m < nao; ++m) {
+        const size_t o = (size_t)m * Ppad + g;
+        const double tv = T[o];
+        r0 = fma(tv, X[o], r0);
+        r1 = fma(tv, X[plane + o], r1);
+        r2 = fma(tv, X[2 * plane + o], r2);
+        r3 = fma(tv, X[3 * plane + o], r3);
+    }
+    rho[g] = r0;
+    rho[Ppad + g] = 2.0 * r1;
+    rho[2 * Ppad + g] = 2.0 * r2;
+    rho[3 * Ppad + g] = 2.0 * r3;
+}
+
+void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T, const double* X,
+                double* rho)
+{
+    (void)npts;
+    hipLaunchKernelGGL(rho_kernel, dim3((Ppad + 255) / 256), dim3(256), 0, s, nao, Ppad, T, X,
+                       rho);
+    WB_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------- VV10 sum -------
+// Outer points i: two per lane (128 per wave).  Inner points j stream
+// through LDS in tiles of VJ = 256 (staged by the whole block); each wave
+// decides per 64-point sub-tile, from the tile statistics and its own
+// i-range (wave reductions), whether the whole sub-tile can be skipped:
+//   g >= Rmin^2 W0min_I + kmin_I, g' >= Rmin^2 W0min_J + kmin_J,
+//   F <= qsum_J / (g g' (g+g')), U <= 2F/g, W <= U Rmax^2
+// (the reference's host-side tile-pair bound, gpu/vv10.py:21-31, 147-164).
+// Per pair: T = q_j / (g g' (g+g')), U += T (1/g + 1/(g+g')), W += that R^2,
+// with one reciprocal (refined by a Newton step) per pair.  Sums over 8
+// inner points are FP32, beyond that FP64, in a fixed order.
+constexpr int VJ = 256;
+constexpr int VG = 8;      // inner points per FP32 partial sum
+
+template <typename real>
+__device__ __forceinline__ real vv_rcp(real x);
+template <> __device__ __forceinline__ float vv_rcp<float>(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+template <> __device__ __forceinline__ double vv_rcp<double>(double x) { return 1.0 / x; }
+
+template <typename real> struct VP4 { real x, y, z, w; };   // coords, W0
+template <typename real> struct VP2 { real k, q; };         // kappa, q = rho w
+
+template <typename real>
+__device__ __forceinline__ void vv10_body(int n, const VP4<real>* __restrict__ pt,
+                                          const VP2<real>* __restrict__ pk,
+                                          const float4* __restrict__ ts, int ntiles, float tol,
+                                          double* __restrict__ Fo, double* __restrict__ Uo,
+                                          double* __restrict__ Wo)
+{
+    __shared__ VP4<real> s_pt[VJ];
+    __shared__ VP2<real> s_pk[VJ];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ibase = (blockIdx.x * 4 + wave) * 128;
+    const int i0 = ibase + lane, i1 = ibase + 64 + lane;
+    const bool v0 = i0 < n, v1 = i1 < n;
+    const VP4<real> dummy{0, 0, 0, 1};
+    const VP2<real> dk{1, 0};
+    const VP4<real> a = v0 ? pt[i0] : dummy;
+    const VP4<real> b = v1 ? pt[i1] : a;
+    const VP2<real> ka = v0 ? pk[i0] : dk;
+    const VP2<real> kb = v1 ? pk[i1] : ka;
+    // wave statistics of the outer points (for pruning)
+    float cx = 0, cy = 0, cz = 0, rI = 0, wminI = 0, kminI = 0;
+    if (tol > 0.f) {
+        const float cnt = (float)__popcll(__ballot(v0)) + (float)__popcll(__ballot(v1));
+        float sx = (v0 ? (float)a.x : 0.f) + (v1 ? (float)b.x : 0.f);
+        float sy = (v0 ? (float)a.y : 0.f) + (v1 ? (float)b.y : 0.f);
+        float sz = (v0 ? (float)a.z : 0.f) + (v1 ? (float)b.z : 0.f);
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+            sz += __shfl_xor(sz, o, 64);
+        }
+        cx = sx / fmaxf(cnt, 1.f); cy = sy / fmaxf(cnt, 1.f); cz = sz / fmaxf(cnt, 1.f);
+        auto dist = [&](const VP4<real>& p) {
+            const float ex = (float)p.x - cx, ey = (float)p.y - cy, ez = (float)p.z - cz;
+            return sqrtf(ex * ex + ey * ey + ez * ez);
+        };
+        float r = fmaxf(v0 ? dist(a) : 0.f, v1 ? dist(b) : 0.f);
+        float wm = fminf(v0 ? (float)a.w : 3e38f, v1 ? (float)b.w : 3e38f);
+        float km = fminf(v0 ? (float)ka.k : 3e38f, v1 ? (float)kb.k : 3e38f);
+        for (int o = 32; o > 0; o >>= 1) {
+            r = fmaxf(r, __shfl_xor(r, o, 64));
+            wm = fminf(wm, __shfl_xor(wm, o, 64));
+            km = fminf(km, __shfl_xor(km, o, 64));
+        }
+        rI = r * 1.0001f + 1e-5f; wminI = wm * 0.9999f; kminI = km * 0.9999f;
+    }
+    double F0 = 0, U0 = 0, W0 = 0, F1 = 0, U1 = 0, W1 = 0;
+    for (int j0 = 0; j0 < n; j0 += VJ) {
+        __syncthreads();
+        {
+            const int jj = j0 + threadIdx.x;
+            s_pt[threadIdx.x] = jj < n ? pt[jj] : dummy;
+            s_pk[threadIdx.x] = jj < n ? pk[jj] : dk;
+        }
+        __syncthreads();
+        for (int sub = 0; sub < VJ / 64; ++sub) {
+            const int jt = (j0 >> 6) + sub;
+            if (jt >= ntiles) break;
+            if (tol > 0.f) {
+                const float4 c = ts[2 * jt], st = ts[2 * jt + 1];   // c.xyz r | wmin kmin qsum
+                const float d = sqrtf((c.x - cx) * (c.x - cx) + (c.y - cy) * (c.y - cy) +
+                                      (c.z - cz) * (c.z - cz));
+                const float rmin = fmaxf(d - rI - c.w, 0.f);
+                const float rmax = d + rI + c.w;
+                const float gI = rmin * rmin * wminI + kminI;
+                const float gJ = rmin * rmin * st.x + st.y;
+                const float bF = st.z / (gI * gJ * (gI + gJ));
+                const float bU = 2.f * bF / gI;
+                const float bW = bU * rmax * rmax;
+                if (fmaxf(bF, fmaxf(bU, bW)) < tol) continue;       // uniform
+            }
+            const int jend = min(64, n - (j0 + sub * 64));
+            // FP32 partial sums over groups of VG inner points, FP64 across
+            for (int jg = 0; jg < jend; jg += VG) {
+                real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0;
+                const int ge = min(jend, jg + VG);
+                for (int jj = jg; jj < ge; ++jj) {
+                    const VP4<real> p = s_pt[sub * 64 + jj];
+                    const VP2<real> k2 = s_pk[sub * 64 + jj];
+                    {
+                        const real dx = a.x - p.x, dy = a.y - p.y, dz = a.z - p.z;
+                        const real R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        const real g = fma(R2, a.w, ka.k);
+                        const real gp = fma(R2, p.w, k2.k);
+                        const real gt = g + gp;
+                        const real inv = vv_rcp<real>(g * gp * gt);
+                        const real T = k2.q * inv;
+                        const real Tu = T * (gp * (g + gt)) * inv;
+                        f0 += T; u0 += Tu; w0 = fma(Tu, R2, w0);
+                    }
+                    {
+                        const real dx = b.x - p.x, dy = b.y - p.y, dz = b.z - p.z;
+                        const real R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        const real g = fma(R2, b.w, kb.k);
+                        const real gp = fma(R2, p.w, k2.k);
+                        const real gt = g + gp;
+                        const real inv = vv_rcp<real>(g * gp * gt);
+                        const real T = k2.q * inv;
+                        const real Tu = T * (gp * (g + gt)) * inv;
+                        f1 += T; u1 += Tu; w1 = fma(Tu, R2, w1);
+                    }
+                }
+                F0 += (double)f0; U0 += (double)u0; W0 += (double)w0;
+                F1 += (double)f1; U1 += (double)u1; W1 += (double)w1;
+            }
+        }
+    }
+    if (v0) { Fo[i0] = -1.5 * F0; Uo[i0] = U0; Wo[i0] = W0; }
+    if (v1) { Fo[i1] = -1.5 * F1; Uo[i1] = U1; Wo[i1] = W1; }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256)
+void vv10_kernel(int n, const VP4<real>* __restrict__ pt, const VP2<real>* __restrict__ pk,
+                 const float4* __restrict__ ts, int ntiles, float tol,
+                 double* __restrict__ Fo, double* __restrict__ Uo, double* __restrict__ Wo)
+{
+    vv10_body<real>(n, pt, pk, ts, ntiles, tol, Fo, Uo, Wo);
+}
+
+// Same sum with the point count read from device memory (compacted sets).
+template <typename real>
+__global__ __launch_bounds__(256)
+void vv10_kernel_n(const int* __restrict__ nptr, const VP4<real>* __restrict__ pt,
+                   const VP2<real>* __restrict__ pk, const float4* __restrict__ ts, float tol,
+                   double* __restrict__ Fo, double* __restrict__ Uo, double* __restrict__ Wo)
+{
+    const int n = *nptr;
+    if (blockIdx.x * 512 >= n) return;
+    vv10_body<real>(n, pt, pk, ts, (n + 63) / 64, tol, Fo, Uo, Wo);
+}
+
+void launch_vv10_sum(hipStream_t s, int n, const void* pts, const void* pk, const float* tiles,
+                     int ntiles, float tol, bool f64, double* F, double* U, double* W)
+{
+    if (n == 0) return;
+    const int blocks = (n + 511) / 512;
+    if (f64)
+        hipLaunchKernelGGL(vv10_kernel<double>, dim3(blocks), dim3(256), 0, s, n,
+                           (const VP4<double>*)pts, (const VP2<double>*)pk,
+                           (const float4*)tiles, ntiles, tol, F, U, W);
+    else
+        hipLaunchKernelGGL(vv10_kernel<float>, dim3(blocks), dim3(256), 0, s, n,
+                           (const VP4<float>*)pts, (const VP2<float>*)pk,
+                           (const float4*)tiles, ntiles, tol, F, U, W);
+    WB_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------ VV10 flow helpers -----
+// Per-point quantities of numint._vv10nlc (gpu/vv10.py:243-265), FP64,
+// for points with rho >= thresh and w > 0; the kept points are compacted in
+// grid order (one wave per 64-point sub-block; offsets from a scan).
+__global__ __launch_bounds__(256)
+void vv_count_kernel(int nsub, int Ppad, const double* __restrict__ rho,
+                     const double* __restrict__ w, double thresh, int* __restrict__ cnt)
+{
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= nsub) return;
+    const int g = s * SUB + (threadIdx.x & 63);
+    const bool keep = w[g] > 0.0 && rho[g] >= thresh;
+    const unsigned long long b = __ballot(keep);
+    if ((threadIdx.x & 63) == 0) cnt[s] = __popcll(b);
+}
+
+// exclusive scan of cnt[0..n) into off[0..n], off[n] = total (one block)
+__global__ __launch_bounds__(1024)
+void scan_kernel(int n, const int* __restrict__ cnt, int* __restrict__ off)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += cnt[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int i = b; i < e; ++i) { off[i] = run; run += cnt[i]; }
+    if (t == 1023) off[n] = part[1023];
+}
+
+template <typename real>
+__global__ __launch_bounds__(256)
+void vv_point_kernel(int nsub, int Ppad, const double* __restrict__ xyz,
+                     const double* __restrict__ rho, const double* __restrict__ w,
+                     double thresh, double b, double C, double ox, double oy, double oz,
+                     const int* __restrict__ off, VP4<real>* __restrict__ pt,
+                     VP2<real>* __restrict__ pk, int* __restrict__ idx,
+                     double* __restrict__ aux /* [4][Ppad]: dW0dR, dW0dG, dKdR, - */)
+{
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= nsub) return;
+    const int lane = threadIdx.x & 63;
+    const int g = s * SUB + lane;
+    const double R = rho[g];
+    const bool keep = w[g] > 0.0 && R >= thresh;
+    const unsigned long long bal = __ballot(keep);
+    if (!keep) return;
+    const int pos = off[s] + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+    const double gx = rho[Ppad + g], gy = rho[2 * Ppad + g], gz = rho[3 * Ppad + g];
+    const double G = gx * gx + gy * gy + gz * gz;
+    const double pi43 = 4.0 * M_PI / 3.0;
+    const double kvv = b * 1.5 * M_PI * pow(9.0 * M_PI, -1.0 / 6.0);
+    const double t = G / (R * R);
+    const double w0tmp = C * t * t;
+    const double W0 = sqrt(w0tmp + pi43 * R);
+    const double kap = kvv * pow(R, 1.0 / 6.0);
+    pt[pos] = VP4<real>{(real)(xyz[g] - ox), (real)(xyz[Ppad + g] - oy),
+                        (real)(xyz[2 * Ppad + g] - oz), (real)W0};
+    pk[pos] = VP2<real>{(real)kap, (real)(R * w[g])};
+    idx[pos] = g;
+    aux[pos] = (0.5 * pi43 * R - 2.0 * w0tmp) / W0;
+    aux[Ppad + pos] = w0tmp * R / (G * W0);
+    aux[2 * Ppad + pos] = kap / 6.0;
+}
+
+// per 64-point tile of the compacted set: centre, radius, min W0, min kappa, sum q
+template <typename real>
+__global__ __launch_bounds__(256)
+void vv_tile_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__ pt,
+                    const VP2<real>* __restrict__ pk, int maxtiles, float4* __restrict__ ts)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int n = *nptr;
+    const int ntiles = (n + 63) / 64;
+    if (t >= ntiles || t >= maxtiles) return;
+    const int lane = threadIdx.x & 63;
+    const int i = t * 64 + lane;
+    const bool v = i < n;
+    const VP4<real> p = v ? pt[i] : pt[t * 64];
+    const VP2<real> k = v ? pk[i] : pk[t * 64];
+    const float cnt = (float)__popcll(__ballot(v));
+    float sx = v ? (float)p.x : 0.f, sy = v ? (float)p.y : 0.f, sz = v ? (float)p.z : 0.f;
+    double qs = v ? (double)k.q : 0.0;
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+        sz += __shfl_xor(sz, o, 64);
+        qs += __shfl_xor(qs, o, 64);
+    }
+    const float cx = sx / cnt, cy = sy / cnt, cz = sz / cnt;
+    const float ex = (float)p.x - cx, ey = (float)p.y - cy, ez = (float)p.z - cz;
+    float r = sqrtf(ex * ex + ey * ey + ez * ez);
+    float wm = (float)p.w, km = (float)k.k;
+    for (int o = 32; o > 0; o >>= 1) {
+        r = fmaxf(r, __shfl_xor(r, o, 64));
+        wm = fminf(wm, __shfl_xor(wm, o, 64));
+        km = fminf(km, __shfl_xor(km, o, 64));
+    }
+    if (lane == 0) {
+        ts[2 * t] = make_float4(cx, cy, cz, r * 1.0001f + 1e-5f);
+        ts[2 * t + 1] = make_float4(wm * 0.9999f, km * 0.9999f, (float)(qs * 1.0001), 0.f);
+    }
+}
+
+// exc/v0/v1 on the kept points -> weighted GGA potential wv[4][Ppad] and the
+// energy density e[g] = rho w exc (zero elsewhere)
+__global__ __launch_bounds__(256)
+void vv_pot_kernel(const int* __restrict__ nptr, const int* __restrict__ idx, int Ppad,
+                   const double* __restrict__ F, const double* __restrict__ U,
+                   const double* __restrict__ W, const double* __restrict__ aux,
+                   const double* __restrict__ rho, const double* __restrict__ w, double beta,
+                   double* __restrict__ wv, double* __restrict__ e)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= *nptr) return;
+    const int g = idx[i];
+    const double exc = beta + 0.5 * F[i];
+    const double v0 = beta + F[i] + 1.5 * (U[i] * aux[2 * Ppad + i] + W[i] * aux[i]);
+    const double v1 = 1.5 * W[i] * aux[Ppad + i];
+    const double wg = w[g];
+    wv[g] = 0.5 * v0 * wg;
+    wv[Ppad + g] = 2.0 * v1 * rho[Ppad + g] * wg;
+    wv[2 * Ppad + g] = 2.0 * v1 * rho[2 * Ppad + g] * wg;
+    wv[3 * Ppad + g] = 2.0 * v1 * rho[3 * Ppad + g] * wg;
+    e[g] = rho[g] * wg * exc;
+}
+
+// B[nu][g] = sum_c wv[c][g] X[c][nu][g]
+__global__ __launch_bounds__(256)
+void vv_b_kernel(int nao, int Ppad, const double* __restrict__ wv, const double* __restrict__ X,
+                 double* __restrict__ B)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (g >= Ppad) return;
+    const size_t plane = (size_t)nao * Ppad, o = (size_t)mu * Ppad + g;
+    B[o] = wv[g] * X[o] + wv[Ppad + g] * X[plane + o] + wv[2 * Ppad + g] * X[2 * plane + o] +
+           wv[3 * Ppad + g] * X[3 * plane + o];
+}
+
+// out[0] = sum rho w, out[1] = sum e  (one block, fixed order)
+__global__ __launch_bounds__(1024)
+void vv_sums_kernel(int Ppad, const double* __restrict__ rho, const double* __restrict__ w,
+                    const double* __restrict__ e, double* __restrict__ out)
+{
+    __shared__ double s0[1024], s1[1024];
+    const int t = threadIdx.x;
+    double a = 0, c = 0;
+    for (int g = t; g < Ppad; g += 1024) {
+        a += rho[g] * w[g];
+        c += e[g];
+    }
+    s0[t] = a;
+    s1[t] = c;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) { s0[t] += s0[t + o]; s1[t] += s1[t + o]; }
+        __syncthreads();
+    }
+    if (t == 0) { out[0] = s0[0]; out[1] = s1[0]; }
+}
+
+__global__ void fill_kernel(size_t n, double* __restrict__ p, double v)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void fill32_kernel(size_t n, unsigned* __restrict__ p)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void transpose_kernel(int n, const double* __restrict__ a, double* __restrict__ b)
+{
+    const int i = blockIdx.y, j = blockIdx.x * 256 + threadIdx.x;
+    if (j < n) b[(size_t)j * n + i] = a[(size_t)i * n + j];
+}
+// C = A + A^T (n x n)
+__global__ void symadd_kernel(int n, const double* __restrict__ a, double* __restrict__ c)
+{
+    const int i = blockIdx.y, j = blockIdx.x * 256 + threadIdx.x;
+    if (j < n) c[(size_t)i * n + j] = a[(size_t)i * n + j] + a[(size_t)j * n + i];
+}
+
+void zero_bytes(hipStream_t s, void* p, size_t bytes)
+{
+    const size_t n = bytes / 4;
+    if (n == 0) return;
+    hipLaunchKernelGGL(fill32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                       (unsigned*)p);
+    WB_HIP(hipGetLastError());
+}
+void transpose(hipStream_t s, int n, const double* a, double* b)
+{
+    hipLaunchKernelGGL(transpose_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, a, b);
+    WB_HIP(hipGetLastError());
+}
+void symadd(hipStream_t s, int n, const double* a, double* c)
+{
+    hipLaunchKernelGGL(symadd_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, a, c);
+    WB_HIP(hipGetLastError());
+}
+
+void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, const double* w,
+               const double* origin, const double* X4, const double* rho, double b, double C,
+               double tol, bool f64, DevBuf& ws, double* wv, double* e, double* Bout,
+               double* sums, Ctx* prof)
+{
+    // workspace layout
+    const size_t rs = f64 ? 8 : 4;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
+    const size_t o_cnt = take(sizeof(int) * (nsub + 1)), o_off = take(sizeof(int) * (nsub + 1));
+    const size_t o_pt = take(4 * rs * Ppad), o_pk = take(2 * rs * Ppad);
+    const size_t o_idx = take(sizeof(int) * Ppad), o_aux = take(sizeof(double) * 3 * Ppad);
+    const size_t ntile_max = (Ppad + 63) / 64;
+    const size_t o_ts = take(sizeof(float4) * 2 * ntile_max);
+    const size_t o_F = take(sizeof(double) * Ppad), o_U = take(sizeof(double) * Ppad),
+                 o_W = take(sizeof(double) * Ppad);
+    char* base = (char*)ws.ensure(o);
+    int* cnt = (int*)(base + o_cnt);
+    int* off = (int*)(base + o_off);
+    int* idx = (int*)(base + o_idx);
+    double* aux = (double*)(base + o_aux);
+    float4* ts = (float4*)(base + o_ts);
+    double *F = (double*)(base + o_F), *U = (double*)(base + o_U), *W = (double*)(base + o_W);
+    const double thresh = 1e-8;
+    const int wb = (nsub + 3) / 4;
+    hipLaunchKernelGGL(vv_count_kernel, dim3(wb), dim3(256), 0, s, nsub, Ppad, rho, w, thresh, cnt);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, nsub, cnt, off);
+    const int* nptr = off + nsub;
+    const double pi = M_PI;
+    const double beta = pow(3.0 / (b * b), 0.75) / 32.0;
+    if (f64) {
+        auto* pt = (VP4<double>*)(base + o_pt);
+        auto* pk = (VP2<double>*)(base + o_pk);
+        hipLaunchKernelGGL(vv_point_kernel<double>, dim3(wb), dim3(256), 0, s, nsub, Ppad, xyz,
+                           rho, w, thresh, b, C, origin[0], origin[1], origin[2], off, pt, pk,
+                           idx, aux);
+        hipLaunchKernelGGL(vv_tile_kernel<double>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
+                           nptr, pt, pk, (int)ntile_max, ts);
+        hipLaunchKernelGGL(vv10_kernel_n<double>, dim3((Ppad + 511) / 512), dim3(256), 0, s,
+                           nptr, pt, pk, ts, (float)tol, F, U, W);
+    } else {
+        auto* pt = (VP4<float>*)(base + o_pt);
+        auto* pk = (VP2<float>*)(base + o_pk);
+        hipLaunchKernelGGL(vv_point_kernel<float>, dim3(wb), dim3(256), 0, s, nsub, Ppad, xyz,
+                           rho, w, thresh, b, C, origin[0], origin[1], origin[2], off, pt, pk,
+                           idx, aux);
+        hipLaunchKernelGGL(vv_tile_kernel<float>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
+                           nptr, pt, pk, (int)ntile_max, ts);
+        hipLaunchKernelGGL(vv10_kernel_n<float>, dim3((Ppad + 511) / 512), dim3(256), 0, s,
+                           nptr, pt, pk, ts, (float)tol, F, U, W);
+    }
+    (void)pi;
+    if (prof) prof->mark(PH_VV10);
+    zero_bytes(s, wv, sizeof(double) * 4 * Ppad);
+    zero_bytes(s, e, sizeof(double) * Ppad);
+    hipLaunchKernelGGL(vv_pot_kernel, dim3((Ppad + 255) / 256), dim3(256), 0, s, nptr, idx, Ppad,
+                       F, U, W, aux, rho, w, beta, wv, e);
+    hipLaunchKernelGGL(vv_b_kernel, dim3((Ppad + 255) / 256, nao), dim3(256), 0, s, nao, Ppad, wv,
+                       X4, Bout);
+    hipLaunchKernelGGL(vv_sums_kernel, dim3(1), dim3(1024), 0, s, Ppad, rho, w, e, sums);
+    WB_HIP(hipGetLastError());
+}
+
+}  // namespace wb

@@ -1,0 +1,174 @@
+"""ctypes binding of libwb97mv_amd.so (the gfx950 C ABI, include/wb97mv_amd.h).
+
+The library must have been built in-tree (``__graft_entry__.build()``);
+there is no CPU fallback behind this module: a missing library or a missing
+GPU raises, and the hook layer (gpu/install.py) turns that into PySCF's own
+CPU path with a warning, exactly as the reference does on Unsupported
+(gpu/install.py:264-287).
+"""
+import ctypes
+import os
+import threading
+
+import numpy
+
+from .errors import GpuError, GpuOOM, Unsupported
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libwb97mv_amd.so')
+
+WB_OK, WB_UNSUPPORTED, WB_OOM, WB_DEVICE_ERROR, WB_INVALID = range(5)
+PREC = {'mixed': 0, 'fp32': 0, 'fp64': 1}
+
+_lib = None
+_lock = threading.Lock()
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_SIGS = {
+    'wb_last_error': (ctypes.c_char_p, []),
+    'wb_version': (_i, []),
+    'wb_ctx_create': (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    'wb_ctx_destroy': (_i, [_vp]),
+    'wb_ctx_set_copies': (_i, [_vp, _i]),
+    'wb_ctx_set_g64': (_i, [_vp, _i]),
+    'wb_sync': (_i, [_vp]),
+    'wb_last_ms': (ctypes.c_float, [_vp]),
+    'wb_ctx_set_profile': (_i, [_vp, _i]),
+    'wb_profile_read': (_i, [_vp, _vp, _i, _i]),
+    'wb_k_work': (_i, [_vp, _vp]),
+    'wb_basis_create': (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _d, ctypes.POINTER(_vp)]),
+    'wb_basis_info': (_i, [_vp, _vp]),
+    'wb_basis_destroy': (_i, [_vp]),
+    'wb_grid_create': (_i, [_vp, _i, _vp, _vp, ctypes.POINTER(_vp)]),
+    'wb_grid_info': (_i, [_vp, _vp]),
+    'wb_grid_destroy': (_i, [_vp]),
+    'wb_k_build': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _i, _vp]),
+    'wb_k_build_dev': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _i, _vp]),
+    'wb_k_stats': (_i, [_vp, _vp, _vp]),
+    'wb_nlc_vxc': (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
+    'wb_nlc_vxc_dev': (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
+    'wb_vv10_sum': (_i, [_vp, _i, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
+    'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
+    'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+}
+EXPORTS = tuple(_SIGS)
+
+
+def load(path=LIB_PATH):
+    """The loaded library (ctypes releases the GIL around every call)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise Unsupported('libwb97mv_amd.so is not built (%s); run '
+                                  '__graft_entry__.build()' % path)
+            lib = ctypes.CDLL(path)
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status):
+    if status == WB_OK:
+        return
+    msg = load().wb_last_error().decode(errors='replace')
+    if status == WB_UNSUPPORTED:
+        raise Unsupported(msg)
+    if status == WB_OOM:
+        raise GpuOOM(msg)
+    raise GpuError('wb97mv_amd status %d: %s' % (status, msg))
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def f64(a):
+    return numpy.ascontiguousarray(a, dtype=numpy.float64)
+
+
+class Context:
+    """One device context (a private HIP stream) of the library."""
+
+    def __init__(self, device=0, ncopy=None):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.wb_ctx_create(int(device), None, ctypes.byref(h)))
+        self.h = h
+        self.device = int(device)
+        if ncopy is not None:
+            check(lib.wb_ctx_set_copies(self.h, int(ncopy)))
+
+    def close(self):
+        if getattr(self, 'h', None):
+            load().wb_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:      # noqa: BLE001 -- interpreter shutdown
+            pass
+
+    def sync(self):
+        check(load().wb_sync(self.h))
+
+    @property
+    def last_ms(self):
+        return float(load().wb_last_ms(self.h))
+
+
+class DeviceBasis:
+    """Basis handle; shells as arrays from basis.pack_mol."""
+
+    def __init__(self, ctx, pack, pair_cut=1e-14):
+        lib = load()
+        self.ctx = ctx
+        self.pack = pack
+        l = numpy.ascontiguousarray(pack.l, dtype=numpy.int32)
+        nprim = numpy.ascontiguousarray(pack.nprim, dtype=numpy.int32)
+        cen, ex, co = f64(pack.center), f64(pack.exps), f64(pack.coefs)
+        h = ctypes.c_void_p()
+        check(lib.wb_basis_create(ctx.h, len(l), ptr(l), ptr(nprim), ptr(cen), ptr(ex), ptr(co),
+                                  float(pair_cut), ctypes.byref(h)))
+        self.h = h
+        info = numpy.zeros(4, dtype=numpy.int64)
+        check(lib.wb_basis_info(self.h, ptr(info)))
+        self.nao, self.nshell, self.npairs, self.nprims = (int(v) for v in info)
+
+    def __del__(self):
+        try:
+            if getattr(self, 'h', None):
+                load().wb_basis_destroy(self.h)
+        except Exception:      # noqa: BLE001
+            pass
+
+
+class DeviceGrid:
+    """Grid handle (coords bohr (n, 3), weights (n,)), resident on the device."""
+
+    def __init__(self, ctx, coords, weights):
+        lib = load()
+        self.ctx = ctx
+        coords, weights = f64(coords).reshape(-1, 3), f64(weights).ravel()
+        if len(coords) != len(weights):
+            raise ValueError('coords / weights size mismatch')
+        h = ctypes.c_void_p()
+        check(lib.wb_grid_create(ctx.h, len(weights), ptr(coords), ptr(weights),
+                                 ctypes.byref(h)))
+        self.h = h
+        info = numpy.zeros(3, dtype=numpy.int64)
+        check(lib.wb_grid_info(self.h, ptr(info)))
+        self.npts, self.ppad, self.nsub = (int(v) for v in info)
+
+    def __del__(self):
+        try:
+            if getattr(self, 'h', None):
+                load().wb_grid_destroy(self.h)
+        except Exception:      # noqa: BLE001
+            pass

@@ -1,0 +1,165 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances: the FP64 form is checked at 1e-10 relative; the mixed form (FP32
+integrals / pair sums) follows the reference's gates (tests/gpu/
+test_sgx_k.py:22-44: max|dK| < 1e-6 max|K|; tests/gpu/test_xc_vv10.py:
+274-292: F/U/W relative < 1e-6)."""
+import numpy
+import pytest
+
+import oracle
+from pyscf_wb97mv_fast_amd import lib
+from pyscf_wb97mv_fast_amd.gpu import sgx_k, vv10
+from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole, sgx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def ctx():
+    c = lib.Context(0)
+    yield c
+    c.close()
+
+
+def _system(basis='def2-svp', level=0, npts=2500, seed=0, atoms=None):
+    mol = mole.Mole(atoms or geometry.WATER_DIMER, basis)
+    g = grids.Grids(mol, level=level).build()
+    if npts and g.size > npts:
+        rng = numpy.random.default_rng(seed)
+        sel = numpy.sort(rng.choice(g.size, npts, replace=False))
+        g.coords, g.weights = g.coords[sel], g.weights[sel]
+    return mol, g, guess.huckel_density(mol)
+
+
+@pytest.mark.parametrize('kind', [0, 1, 2, 3])
+@pytest.mark.parametrize('shape', [(16, 64, 4), (48, 1536, 48), (100, 70, 333), (648, 640, 648)])
+def test_fp64_mfma_gemm(ctx, kind, shape):
+    M, N, K = shape
+    rng = numpy.random.default_rng(M + N + K)
+    A = rng.normal(size=(M, K))
+    B = rng.normal(size=(K, N)) if kind % 2 == 0 else rng.normal(size=(N, K))
+    w = None if kind % 2 == 0 else rng.uniform(0.5, 1.5, K)
+    C = numpy.empty((M, N))
+    wp = lib.ptr(w) if w is not None else None
+    lib.check(lib.load().wb_test_gemm(ctx.h, kind, M, N, K, lib.ptr(A), lib.ptr(B), wp, lib.ptr(C)))
+    ref = A @ B if kind % 2 == 0 else (A * w) @ B.T
+    assert numpy.allclose(C, ref, rtol=1e-12, atol=1e-12 * numpy.abs(ref).max())
+
+
+def test_ao_eval_matches_oracle(ctx):
+    mol, g, _ = _system('def2-tzvp', npts=700)
+    b = lib.DeviceBasis(ctx, __import__('pyscf_wb97mv_fast_amd.basis', fromlist=['x']).pack_mol(mol))
+    out = numpy.empty((4, g.size, b.nao))
+    lib.check(lib.load().wb_ao_eval(ctx.h, b.h, g.size, lib.ptr(lib.f64(g.coords)), 1,
+                                    lib.ptr(out)))
+    ref = oracle.ao_eval(oracle.OracleBasis.from_mol(mol), g.coords, deriv=1)
+    assert numpy.allclose(out, ref, rtol=0, atol=1e-12 * numpy.abs(ref).max())
+
+
+@pytest.mark.parametrize('basis', ['def2-svp', 'def2-tzvp'])
+@pytest.mark.parametrize('omega', [0.0, 0.3])
+def test_k_fp64_matches_oracle(ctx, basis, omega):
+    mol, g, dm = _system(basis, npts=1500)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
+    K = kb.k_raw(g, dm, [omega])[0]
+    ref = oracle.k_build(oracle.OracleBasis.from_mol(mol), g.coords, g.weights, dm, omega,
+                         pair_cut=1e-14)
+    print('%s omega=%.1f fp64 max|dK| %.3e of %.3e' % (basis, omega, numpy.abs(K - ref).max(),
+                                                      numpy.abs(ref).max()))
+    assert numpy.allclose(K, ref, rtol=0, atol=1e-10 * numpy.abs(ref).max())
+
+
+@pytest.mark.parametrize('basis', ['def2-svp', 'def2-tzvp'])
+@pytest.mark.parametrize('omega', [0.0, 0.3])
+def test_k_mixed_matches_oracle(ctx, basis, omega):
+    mol, g, dm = _system(basis, npts=1500)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='mixed')
+    K = kb.k_raw(g, dm, [omega])[0]
+    ref = oracle.k_build(oracle.OracleBasis.from_mol(mol), g.coords, g.weights, dm, omega,
+                         pair_cut=1e-14)
+    dE = 0.25 * numpy.einsum('ij,ji->', dm, K - ref)
+    print('%s omega=%.1f mixed max|dK| %.3e of %.3e, dE_K %.3e' % (
+        basis, omega, numpy.abs(K - ref).max(), numpy.abs(ref).max(), dE))
+    assert numpy.abs(K - ref).max() < 1e-6 * numpy.abs(ref).max()
+    assert abs(dE) < 3e-7
+
+
+def test_k_two_operators_one_call_and_repeatable(ctx):
+    mol, g, dm = _system(npts=1200)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='mixed')
+    both = kb.k_raw(g, dm, [0.0, 0.3])
+    again = kb.k_raw(g, dm, [0.0, 0.3])
+    assert numpy.array_equal(both, again)                     # bit-reproducible
+    assert numpy.array_equal(both[1], kb.k_raw(g, dm, [0.3])[0])
+
+
+def test_k_screening(ctx):
+    mol, g, dm = _system(atoms=geometry.water_cluster(8, seed=1), level=0, npts=6000)
+    kb0 = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
+    kb1 = sgx_k.GpuKBuilder(mol, ctx, tile_tol=1e-11, precision='fp64')
+    K0 = kb0.k_raw(g, dm, [0.0])[0]
+    K1 = kb1.k_raw(g, dm, [0.0])[0]
+    st = kb1.last_stats
+    print('screening kept %d of %d, max|dK| %.2e' % (st['pairs_kept'], st['pairs_total'],
+                                                      numpy.abs(K1 - K0).max()))
+    assert numpy.abs(K1 - K0).max() < 1e-8
+    assert st['pairs_kept'] < st['pairs_total']
+
+
+def test_get_k_only_contract(ctx):
+    mol, g, dm = _system(npts=1000)
+    rng = numpy.random.default_rng(3)
+    Q = numpy.eye(mol.nao) + 0.01 * rng.normal(size=(mol.nao, mol.nao))
+    s = sgx.SyntheticSGX(mol, g, Q=Q, sym_ovlp=True)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
+    with mol.with_range_coulomb(0.3):
+        K = sgx_k.get_k_only_gpu(kb, s, dm)
+    ref = oracle.get_k_only(oracle.OracleBasis.from_mol(mol), g.coords, g.weights, dm, Q=Q,
+                            sym_ovlp=True, omega=0.3)
+    assert numpy.allclose(K, ref, atol=1e-10 * numpy.abs(ref).max())
+    assert numpy.array_equal(K, K.T)
+
+
+@pytest.mark.parametrize('precision,rtol', [('fp64', 1e-11), ('mixed', 1e-6)])
+def test_vv10_sum_matches_oracle(ctx, precision, rtol):
+    rng = numpy.random.default_rng(4)
+    n = 1100
+    coords = rng.normal(size=(n, 3)) * 4
+    q = rng.uniform(1e-4, 0.1, n)
+    W0 = rng.uniform(0.05, 2.0, n)
+    kappa = rng.uniform(0.05, 1.0, n)
+    k = vv10.Vv10Kernel(ctx, precision=precision)
+    got = k.kernel_sum(coords, q, W0, kappa)
+    ref = oracle.vv10_sum(coords, q, W0, kappa)
+    for a, b in zip(got, ref):
+        assert numpy.allclose(a, b, rtol=rtol, atol=0)
+
+
+def test_vv10_pruning_two_far_clusters(ctx):
+    """inner/outer tile indexing with two clusters 1000 bohr apart
+    (tests/gpu/test_xc_vv10.py:320-343)"""
+    rng = numpy.random.default_rng(5)
+    n = 600
+    coords = rng.normal(size=(n, 3))
+    coords[n // 2:] += 1000.0
+    q, W0, kappa = rng.uniform(0.01, 0.1, n), rng.uniform(0.5, 1.0, n), rng.uniform(0.3, 0.6, n)
+    ref = oracle.vv10_sum(coords, q, W0, kappa)
+    got = vv10.Vv10Kernel(ctx, tile_tol=1e-14, precision='fp64').kernel_sum(coords, q, W0, kappa)
+    for a, b in zip(got, ref):
+        assert numpy.allclose(a, b, rtol=1e-9)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_nlc_vxc_matches_oracle(ctx, precision):
+    mol, g, dm = _system(npts=3000)
+    st = vv10.NlcState(ctx, mol)
+    n_g, e_g, v_g = vv10.gpu_nr_nlc_vxc(st, sgx.SyntheticNumInt(), mol, g, 'wb97m-v', dm,
+                                        precision=precision)
+    n_o, e_o, v_o = oracle.nr_nlc_vxc(oracle.OracleBasis.from_mol(mol), g.coords, g.weights, dm)
+    print('%s nelec %.12f vs %.12f, exc %.3e, max|dV| %.2e' % (precision, n_g, n_o, e_g - e_o,
+                                                              numpy.abs(v_g - v_o).max()))
+    assert abs(n_g - n_o) < 1e-10
+    tol = 1e-10 if precision == 'fp64' else 1e-7
+    assert abs(e_g - e_o) < tol
+    assert numpy.abs(v_g - v_o).max() < (1e-9 if precision == 'fp64' else 1e-6)
