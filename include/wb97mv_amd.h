@@ -54,7 +54,7 @@ int wb_ctx_create(int device, void* stream, wb_ctx_t* out);
 int wb_ctx_destroy(wb_ctx_t ctx);
 /* number of G partial copies of the exchange kernel (default 4) */
 int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
-/* mixed form: accumulate G in FP64 (1, default) or FP32 (0) */
+/* mixed form: accumulate G in FP64 (1) or FP32 (0, default) */
 int wb_ctx_set_g64(wb_ctx_t ctx, int on);
 int wb_sync(wb_ctx_t ctx);
 float wb_last_ms(wb_ctx_t ctx);

@@ -8,7 +8,8 @@ For every shell-pair class la >= lb (l <= 3) it emits
       then V[a][b] += sum_tuv Ex[ax][bx][t] Ey[ay][by][u] Ez[az][bz][v] R_tuv
       factorized as z -> y -> x contractions.  E is the primitive pair's
       coefficient record (uniform across the wave, read through the scalar
-      cache); V is the Cartesian block accumulated over the contraction.
+      cache); V (of type vacc, which may be wider than the arithmetic) is the
+      Cartesian block accumulated over the contraction.
   md_fcart_<l>(Fs, Fc)      Fc[c] = sum_m T[m][c] Fs[m]   (sph -> cart)
   md_gsph_<l>(Gc, Gs)       Gs[m] = sum_c T[m][c] Gc[c]   (cart -> sph)
       (in FP32 every non-trivial constant product is compensated with the
@@ -87,11 +88,11 @@ def gen_prim(la, lb):
     def eidx(axis, i, j, t):
         return axis * naxis + (i * (lb + 1) + j) * (L + 1) + t
 
-    lines = ['template <typename real>',
+    lines = ['template <typename real, typename vacc>',
              '__device__ __forceinline__ void md_prim_%d%d(real dx, real dy, real dz,'
              % (la, lb),
              '    const real* __restrict__ R0, const real* __restrict__ E,'
-             ' real* __restrict__ V)',
+             ' vacc* __restrict__ V)',
              '{']
     rl, R = gen_r(L)
     lines += rl
@@ -135,14 +136,16 @@ def gen_prim(la, lb):
                 expr = efma(1, ay, by, u, sn, expr)
             lines.append('    const real %s = %s;' % (nm, expr))
             yz[(ay, by, az, bz, t)] = nm
+    # this primitive pair's block is formed on its own and then added to the
+    # running sum V (type vacc): started from V, the FMA chain would round
+    # every small primitive contribution against the large running sum
     for i, a in enumerate(ca):
         for j, b in enumerate(cb):
             ax, bx = a[0], b[0]
-            acc = 'V[%d]' % (i * nb + j)
-            expr = acc
-            for t in range(ax + bx + 1):
+            expr = emul(0, ax, bx, 0, yz[(a[1], b[1], a[2], b[2], 0)])
+            for t in range(1, ax + bx + 1):
                 expr = efma(0, ax, bx, t, yz[(a[1], b[1], a[2], b[2], t)], expr)
-            lines.append('    %s = %s;' % (acc, expr))
+            lines.append('    V[%d] += (vacc)(%s);' % (i * nb + j, expr))
     lines.append('}')
     return lines
 
@@ -191,20 +194,19 @@ def main(out_path):
              ' double, float x) { return fmaf(hi, x, lo * x); }',
              'template <> __device__ __forceinline__ double wb_cm<double>(float, float,'
              ' double d, double x) { return d * x; }',
-             '// E-coefficient products; with WB_ECOMP the FP32 record holds the',
-             '// rounding remainders after the ERL leading values',
-             '#ifndef WB_ECOMP',
-             '#define WB_ECOMP 0',
-             '#endif',
+             '// E-coefficient products.  The FP32 record holds the ERL leading values',
+             '// and then their rounding remainders: E values of one-centre pairs are',
+             '// the same for every copy of an element, so their rounding would bias',
+             '// K coherently; each product is formed with one rounding.',
              'template <typename real, int ERL>',
              '__device__ __forceinline__ real wb_efma(const real* __restrict__ E, int k, real x,'
              ' real acc) {',
-             '    if constexpr (WB_ECOMP && sizeof(real) == 4) return fma(E[k], x, fma(E[k + ERL], x, acc));',
+             '    if constexpr (sizeof(real) == 4) return fma(E[k], x, fma(E[k + ERL], x, acc));',
              '    else return fma(E[k], x, acc);',
              '}',
              'template <typename real, int ERL>',
              '__device__ __forceinline__ real wb_emul(const real* __restrict__ E, int k, real x) {',
-             '    if constexpr (WB_ECOMP && sizeof(real) == 4) return fma(E[k], x, E[k + ERL] * x);',
+             '    if constexpr (sizeof(real) == 4) return fma(E[k], x, E[k + ERL] * x);',
              '    else return E[k] * x;',
              '}']
     for l in range(LMAX + 1):

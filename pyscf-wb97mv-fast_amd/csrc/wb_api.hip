@@ -13,9 +13,6 @@
 #include <numeric>
 #include "wb_boys.h"
 #include "wb_internal.h"
-#ifndef WB_ECOMP
-#define WB_ECOMP 0
-#endif
 
 namespace wb {
 
@@ -269,7 +266,6 @@ static void upload_basis(Basis& B, hipStream_t s)
         upload(C.d_pairs, C.pairs.data(), sizeof(PairRec) * C.npairs, s);
         upload(C.d_scr, C.scr.data(), sizeof(PairScreen) * C.npairs, s);
         upload(C.d_P, C.P.data(), sizeof(double) * C.P.size(), s);
-#if WB_ECOMP
         // per primitive pair: the erl leading FP32 values, then their remainders
         std::vector<float> e32(2 * C.E.size());
         for (size_t m = 0; m < C.E.size() / C.erl; ++m)
@@ -279,9 +275,6 @@ static void upload_basis(Basis& B, hipStream_t s)
                 e32[2 * m * C.erl + i] = hi;
                 e32[2 * m * C.erl + C.erl + i] = (float)(v - (double)hi);
             }
-#else
-        std::vector<float> e32(C.E.begin(), C.E.end());
-#endif
         upload(C.d_E32, e32.data(), sizeof(float) * e32.size(), s);
         upload(C.d_E64, C.E.data(), sizeof(double) * C.E.size(), s);
     }

@@ -15,9 +15,13 @@
 namespace wb {
 
 // FP32: nodes k/4, k = 0..144; row = F_0..F_12, e^{-T_k} (hi, 14 values),
-// then their FP32 rounding remainders (lo, 14 values), pad to 32
+// then their FP32 rounding remainders (lo, 14 values), padded to 36 floats:
+// the table sits in LDS and lanes of a wave read neighbouring nodes; a
+// 36-float stride starts consecutive rows 4 banks apart, so 8 consecutive
+// nodes are read without bank conflicts (a 32-float stride would put every
+// row on the same banks)
 constexpr int B32_NODES = 145;
-constexpr int B32_ROW = 32;
+constexpr int B32_ROW = 36;
 constexpr int B32_LO = 14;
 constexpr float B32_SWITCH = 36.0f;
 // FP64: nodes k/8, k = 0..320; row = F_0..F_14, e^{-T_k}

@@ -74,7 +74,7 @@ struct Ctx {
     long long tasks_total = 0, tasks_kept = 0;
     float ms_last = 0.f;
     int ncopy = 4;
-    bool g64 = true;           // mixed form: G accumulated in FP64
+    bool g64 = false;          // mixed form: G accumulated in FP64 (default FP32)
     // per-class work of the last K build: kept tasks, kept primitive pairs
     DevBuf ws_work;
     // phase timing (HIP events on this context's stream), off by default

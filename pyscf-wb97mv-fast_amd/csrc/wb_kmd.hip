@@ -14,29 +14,23 @@
 // one warp per 32-point sub-block (gpu/sgx_k.py:335-400).
 #include "wb_boys.h"
 #include "wb_internal.h"
+#include <type_traits>
 #include "md_gen.h"
 
 namespace wb {
 
 template <int L> struct MdDispatch;
 
-template <int LA, int LB, typename real>
+template <int LA, int LB, typename real, typename vacc>
 __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R0,
-                                        const real* E, real* V);
-
-#define WB_MD_PRIM(la, lb)                                                        \
-    template <> __device__ __forceinline__ void md_prim<la, lb, float>(           \
-        float dx, float dy, float dz, const float* R0, const float* E, float* V) { \
-        md_prim_##la##lb<float>(dx, dy, dz, R0, E, V);                           \
-    }                                                                             \
-    template <> __device__ __forceinline__ void md_prim<la, lb, double>(          \
-        double dx, double dy, double dz, const double* R0, const double* E,       \
-        double* V) {                                                              \
-        md_prim_##la##lb<double>(dx, dy, dz, R0, E, V);                          \
-    }
-WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
-WB_MD_PRIM(2, 2) WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1) WB_MD_PRIM(3, 2) WB_MD_PRIM(3, 3)
+                                        const real* E, vacc* V)
+{
+#define WB_MD_PRIM(a, b) \
+    if constexpr (LA == a && LB == b) md_prim_##a##b<real, vacc>(dx, dy, dz, R0, E, V);
+    WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
+    WB_MD_PRIM(2, 2) WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1) WB_MD_PRIM(3, 2) WB_MD_PRIM(3, 3)
 #undef WB_MD_PRIM
+}
 
 template <int l, typename real>
 __device__ __forceinline__ void fcart(const real* Fs, real* Fc) {
@@ -69,7 +63,12 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
     constexpr int NSA = 2 * LA + 1, NSB = 2 * LB + 1;
     constexpr bool F32 = sizeof(real) == 4;
-    constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1) * ((WB_ECOMP && F32) ? 2 : 1);
+    // FP32 E records carry their rounding remainders (twice the length)
+    constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1) * (F32 ? 2 : 1);
+    // the primitive sum of the Cartesian block is FP64 when it fits
+    // (<= 36 entries); the larger f-shell blocks belong to single-primitive
+    // shells in the def2 sets and stay in the arithmetic type
+    using vacc = typename std::conditional<(NCA * NCB <= 36), double, real>::type;
     constexpr int CRL = (L + 2) * (F32 ? 2 : 1);   // FP32: (hi, lo) pairs
     constexpr int TAB = F32 ? B32_NODES * B32_ROW : 1;
     __shared__ real s_tab[TAB];
@@ -116,9 +115,9 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 #pragma unroll
             for (int c = 0; c < NCA; ++c) acc[c] = (gacc)0;
         }
-        real V[NCA * NCB];
+        vacc Vs[NCA * NCB];
 #pragma unroll
-        for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)0;
+        for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
         for (int m = pr.pp0; m < pr.pp0 + pr.npp; ++m) {
             const real dx = (real)(ppP[3 * m] - px);
             const real dy = (real)(ppP[3 * m + 1] - py);
@@ -135,8 +134,11 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 #pragma unroll
                 for (int n = 0; n <= L; ++n) R0[n] = C[1 + n] * Fb[n];
             }
-            md_prim<LA, LB, real>(dx, dy, dz, R0, ppE + (size_t)m * ERL, V);
+            md_prim<LA, LB, real, vacc>(dx, dy, dz, R0, ppE + (size_t)m * ERL, Vs);
         }
+        real V[NCA * NCB];
+#pragma unroll
+        for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];
         real fb[NSB], FbC[NCB];
 #pragma unroll
         for (int m = 0; m < NSB; ++m) fb[m] = F[(size_t)(pr.b0 + m) * Ppad + g];

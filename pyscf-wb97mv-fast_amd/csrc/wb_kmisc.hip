@@ -306,14 +306,16 @@ void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A, const v
     WB_HIP(hipGetLastError());
 }
 
-// ------------------------------------------------- 128x128 FP64 GEMMs ----
-// Block tile 128 x 128, 4 waves of 64 x 64 (4 x 4 MFMA tiles), BK = 16;
-// the next K slice is fetched into registers while the current one is
-// consumed from LDS (two LDS buffers, one barrier per slice).  Block ids are
-// remapped so that consecutive work items run on the same XCD (blocks are
-// dealt round-robin over the 8 XCDs), which keeps the operand slices a
-// group of tiles shares in one L2.
-constexpr int T_M = 128, T_N = 128, T_K = 16, T_PAD = 2;
+// ------------------------------------------------ tiled FP64 GEMMs -------
+// Block of 4 waves in a 2 x 2 layout; a wave computes WM x WN MFMA tiles of
+// 16 x 16, so the block tile is (32 WM) x (32 WN) (96 or 128 per side,
+// chosen per call to minimize padding, e.g. nao = 648 -> 7 x 96 rather than
+// 6 x 128).  BK = 16; the next K slice is fetched into registers while the
+// current one is consumed from LDS (two LDS buffers, one barrier per slice).
+// Block ids are remapped so that consecutive work items run on the same XCD
+// (blocks are dealt round-robin over the 8 XCDs): the tiles of one K split
+// then share its operand slices in one L2.
+constexpr int T_K = 16, T_PAD = 2;
 constexpr int NXCD = 8;
 
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks)
@@ -326,12 +328,15 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks)
     return base + local;
 }
 
-// Row-major rows x K (k contiguous) slice [r0, r0+128) x [k0, k0+16) into
-// registers: thread t holds row t/2, k half (t%2)*8 .. +8.
+// Row-major rows x K (k contiguous) slice [r0, r0+BR) x [k0, k0+16) into
+// registers: thread t < 2 BR holds row t/2, k half (t%2)*8 .. +8.
+template <int BR>
 __device__ __forceinline__ void fetch_rows(const double* __restrict__ A, int lda, int rows,
                                            int kend, int r0, int k0, double (&v)[8])
 {
-    const int t = threadIdx.x, r = r0 + (t >> 1), kb = k0 + (t & 1) * 8;
+    const int t = threadIdx.x;
+    if (t >= 2 * BR) return;
+    const int r = r0 + (t >> 1), kb = k0 + (t & 1) * 8;
     if (r < rows && kb + 8 <= kend) {
         const double2* p = (const double2*)(A + (size_t)r * lda + kb);
 #pragma unroll
@@ -341,18 +346,25 @@ __device__ __forceinline__ void fetch_rows(const double* __restrict__ A, int lda
         for (int i = 0; i < 8; ++i) v[i] = (r < rows && kb + i < kend) ? A[(size_t)r * lda + kb + i] : 0.0;
     }
 }
-__device__ __forceinline__ void store_rows(double (*S)[T_M + T_PAD], const double (&v)[8])
+template <int BR>
+__device__ __forceinline__ void store_rows(double (*S)[BR + T_PAD], const double (&v)[8])
 {
-    const int t = threadIdx.x, r = t >> 1, kq = (t & 1) * 8;
+    const int t = threadIdx.x;
+    if (t >= 2 * BR) return;
+    const int r = t >> 1, kq = (t & 1) * 8;
 #pragma unroll
     for (int i = 0; i < 8; ++i) S[kq + i][r] = v[i];
 }
-// K x N (n contiguous) slice [k0, k0+16) x [n0, n0+128): thread t holds k
-// row t/16, n columns (t%16)*8 .. +8
+// K x N (n contiguous) slice [k0, k0+16) x [n0, n0+BC): thread t < 2 BC
+// holds k row t/(BC/8), n columns (t % (BC/8)) * 8 .. +8
+template <int BC>
 __device__ __forceinline__ void fetch_cols(const double* __restrict__ B, int ldb, int kdim,
                                            int ncols, int k0, int n0, double (&v)[8])
 {
-    const int t = threadIdx.x, k = k0 + (t >> 4), nb = n0 + (t & 15) * 8;
+    const int t = threadIdx.x;
+    if (t >= 2 * BC) return;
+    constexpr int PER = BC / 8;
+    const int k = k0 + t / PER, nb = n0 + (t % PER) * 8;
     if (k < kdim && nb + 8 <= ncols) {
         const double2* p = (const double2*)(B + (size_t)k * ldb + nb);
 #pragma unroll
@@ -362,41 +374,46 @@ __device__ __forceinline__ void fetch_cols(const double* __restrict__ B, int ldb
         for (int i = 0; i < 8; ++i) v[i] = (k < kdim && nb + i < ncols) ? B[(size_t)k * ldb + nb + i] : 0.0;
     }
 }
-__device__ __forceinline__ void store_cols(double (*S)[T_N + T_PAD], const double (&v)[8])
+template <int BC>
+__device__ __forceinline__ void store_cols(double (*S)[BC + T_PAD], const double (&v)[8])
 {
-    const int t = threadIdx.x, k = t >> 4, nq = (t & 15) * 8;
+    const int t = threadIdx.x;
+    if (t >= 2 * BC) return;
+    constexpr int PER = BC / 8;
+    const int k = t / PER, nq = (t % PER) * 8;
 #pragma unroll
     for (int i = 0; i < 8; ++i) S[k][nq + i] = v[i];
 }
 
-__device__ __forceinline__ void mma_slice(const double (*As)[T_M + T_PAD],
-                                          const double (*Bs)[T_N + T_PAD], int wm, int wn,
-                                          d4 (&acc)[4][4])
+template <int WM, int WN>
+__device__ __forceinline__ void mma_slice(const double (*As)[32 * WM + T_PAD],
+                                          const double (*Bs)[32 * WN + T_PAD], int wm, int wn,
+                                          d4 (&acc)[WM][WN])
 {
     const int lane = threadIdx.x & 63, c = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int kk = 0; kk < T_K; kk += 4) {
-        double a[4], b[4];
+        double a[WM], b[WN];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = As[kk + kq][wm + i * 16 + c];
+        for (int i = 0; i < WM; ++i) a[i] = As[kk + kq][wm + i * 16 + c];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = Bs[kk + kq][wn + j * 16 + c];
+        for (int j = 0; j < WN; ++j) b[j] = Bs[kk + kq][wn + j * 16 + c];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < WM; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+            for (int j = 0; j < WN; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
     }
 }
 
-template <typename OutT>
+template <int WM, int WN, typename OutT>
 __device__ __forceinline__ void store_tile(OutT* __restrict__ C, int ldc, int M, int N, int m0,
-                                           int n0, int wm, int wn, const d4 (&acc)[4][4])
+                                           int n0, int wm, int wn, const d4 (&acc)[WM][WN])
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < WN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int mm = m0 + wm + i * 16 + 4 * r + (lane >> 4);
@@ -405,78 +422,80 @@ __device__ __forceinline__ void store_tile(OutT* __restrict__ C, int ldc, int M,
             }
 }
 
-// C[M][N] = A[M][K] B[K][N]; grid: m tiles fastest (they share the B slice)
-template <typename OutT>
+// C[M][N] = A[M][K] B[K][N]; m tiles fastest (they share the B slice)
+template <int WM, int WN, typename OutT>
 __global__ __launch_bounds__(256, 2)
-void gemm_nn128_kernel(int M, int N, int K, const double* __restrict__ A,
-                       const double* __restrict__ B, OutT* __restrict__ C, int mt)
+void gemm_nn_t_kernel(int M, int N, int K, const double* __restrict__ A,
+                      const double* __restrict__ B, OutT* __restrict__ C, int mt)
 {
-    __shared__ double As[2][T_K][T_M + T_PAD];
-    __shared__ double Bs[2][T_K][T_N + T_PAD];
-    const int nblocks = gridDim.x;
-    const int w = xcd_remap(blockIdx.x, nblocks);
-    const int m0 = (w % mt) * T_M, n0 = (w / mt) * T_N;
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    __shared__ double As[2][T_K][BM + T_PAD];
+    __shared__ double Bs[2][T_K][BN + T_PAD];
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int m0 = (w % mt) * BM, n0 = (w / mt) * BN;
     const int wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    d4 acc[4][4];
+    const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
+    d4 acc[WM][WN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+        for (int j = 0; j < WN; ++j) acc[i][j] = (d4){0, 0, 0, 0};
     double va[8], vb[8];
-    fetch_rows(A, K, M, K, m0, 0, va);
-    fetch_cols(B, N, K, N, 0, n0, vb);
+    fetch_rows<BM>(A, K, M, K, m0, 0, va);
+    fetch_cols<BN>(B, N, K, N, 0, n0, vb);
     int buf = 0;
     for (int k0 = 0; k0 < K; k0 += T_K) {
-        store_rows(As[buf], va);
-        store_cols(Bs[buf], vb);
+        store_rows<BM>(As[buf], va);
+        store_cols<BN>(Bs[buf], vb);
         __syncthreads();
         if (k0 + T_K < K) {
-            fetch_rows(A, K, M, K, m0, k0 + T_K, va);
-            fetch_cols(B, N, K, N, k0 + T_K, n0, vb);
+            fetch_rows<BM>(A, K, M, K, m0, k0 + T_K, va);
+            fetch_cols<BN>(B, N, K, N, k0 + T_K, n0, vb);
         }
-        mma_slice(As[buf], Bs[buf], wm, wn, acc);
+        mma_slice<WM, WN>(As[buf], Bs[buf], wm, wn, acc);
         buf ^= 1;
     }
-    store_tile(C, N, M, N, m0, n0, wm, wn, acc);
+    store_tile<WM, WN>(C, N, M, N, m0, n0, wm, wn, acc);
 }
 
-// Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k]; grid: tiles fastest
-// within a split (they share the A and B slices of that split)
+// Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k]; tiles fastest within a
+// split (they share the A and B slices of that split)
+template <int WM, int WN>
 __global__ __launch_bounds__(256, 2)
-void gemm_nt128_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
-                       const double* __restrict__ A, const double* __restrict__ B,
-                       double* __restrict__ Cp)
+void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
+                      const double* __restrict__ A, const double* __restrict__ B,
+                      double* __restrict__ Cp)
 {
-    __shared__ double As[2][T_K][T_M + T_PAD];
-    __shared__ double Bs[2][T_K][T_N + T_PAD];
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    __shared__ double As[2][T_K][BM + T_PAD];
+    __shared__ double Bs[2][T_K][BN + T_PAD];
     const int w = xcd_remap(blockIdx.x, gridDim.x);
     const int tile = w % ntile, z = w / ntile;
-    const int m0 = (tile % mt) * T_M, n0 = (tile / mt) * T_N;
+    const int m0 = (tile % mt) * BM, n0 = (tile / mt) * BN;
     const int kb = z * kchunk, ke = min(K, kb + kchunk);
     const int wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    d4 acc[4][4];
+    const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
+    d4 acc[WM][WN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+        for (int j = 0; j < WN; ++j) acc[i][j] = (d4){0, 0, 0, 0};
     double va[8], vb[8];
-    fetch_rows(A, K, M, ke, m0, kb, va);
-    fetch_rows(B, K, N, ke, n0, kb, vb);
+    fetch_rows<BM>(A, K, M, ke, m0, kb, va);
+    fetch_rows<BN>(B, K, N, ke, n0, kb, vb);
     int buf = 0;
     for (int k0 = kb; k0 < ke; k0 += T_K) {
-        store_rows(As[buf], va);
-        store_rows((double (*)[T_M + T_PAD])Bs[buf], vb);
+        store_rows<BM>(As[buf], va);
+        store_rows<BN>(Bs[buf], vb);
         __syncthreads();
         if (k0 + T_K < ke) {
-            fetch_rows(A, K, M, ke, m0, k0 + T_K, va);
-            fetch_rows(B, K, N, ke, n0, k0 + T_K, vb);
+            fetch_rows<BM>(A, K, M, ke, m0, k0 + T_K, va);
+            fetch_rows<BN>(B, K, N, ke, n0, k0 + T_K, vb);
         }
-        mma_slice(As[buf], Bs[buf], wm, wn, acc);
+        mma_slice<WM, WN>(As[buf], Bs[buf], wm, wn, acc);
         buf ^= 1;
     }
-    store_tile(Cp + (size_t)z * M * N, N, M, N, m0, n0, wm, wn, acc);
+    store_tile<WM, WN>(Cp + (size_t)z * M * N, N, M, N, m0, n0, wm, wn, acc);
 }
 
 // out[i] = w[i % K] * sum_c src[c][i]  (the G copies and the grid weights
@@ -494,17 +513,24 @@ void fold_copies_kernel(size_t n, int K, const BT* __restrict__ src, int nb, siz
     out[i] = v;
 }
 
+// 3 (96) or 4 (128) MFMA tiles per wave side: the one with less padding
+static int wave_tiles(int D)
+{
+    const int p96 = (D + 95) / 96 * 96, p128 = (D + 127) / 128 * 128;
+    return p96 < p128 ? 3 : 4;
+}
+
 void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B, void* C,
                   bool out_f32)
 {
-    const int mt = (M + T_M - 1) / T_M, nt = (N + T_N - 1) / T_N;
+    const int wm = wave_tiles(M);
+    const int mt = (M + 32 * wm - 1) / (32 * wm), nt = (N + 127) / 128;
     dim3 grid(mt * nt);
-    if (out_f32)
-        hipLaunchKernelGGL(gemm_nn128_kernel<float>, grid, dim3(256), 0, s, M, N, K, A, B,
-                           (float*)C, mt);
-    else
-        hipLaunchKernelGGL(gemm_nn128_kernel<double>, grid, dim3(256), 0, s, M, N, K, A, B,
-                           (double*)C, mt);
+#define WB_NN(WMv, T) hipLaunchKernelGGL((gemm_nn_t_kernel<WMv, 4, T>), grid, dim3(256), 0, s, \
+                                          M, N, K, A, B, (T*)C, mt)
+    if (wm == 3) { if (out_f32) WB_NN(3, float); else WB_NN(3, double); }
+    else { if (out_f32) WB_NN(4, float); else WB_NN(4, double); }
+#undef WB_NN
     WB_HIP(hipGetLastError());
 }
 
@@ -514,7 +540,8 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
     // fold the copies / weights into one FP64 operand when needed
     const double* Bd = (const double*)B;
     const size_t nbk = (size_t)N * K;
-    const int mt = (M + T_M - 1) / T_M, ntile = mt * ((N + T_N - 1) / T_N);
+    const int wm = wave_tiles(M), wn = wave_tiles(N);
+    const int mt = (M + 32 * wm - 1) / (32 * wm), ntile = mt * ((N + 32 * wn - 1) / (32 * wn));
     int nsplit = std::max(1, std::min(96, K / (T_K * 64)));
     int kchunk = (K + nsplit - 1) / nsplit;
     kchunk = (kchunk + T_K - 1) / T_K * T_K;
@@ -535,8 +562,14 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
         WB_HIP(hipGetLastError());
         Bd = Bf;
     }
-    hipLaunchKernelGGL(gemm_nt128_kernel, dim3(ntile * nsplit), dim3(256), 0, s, M, N, K, kchunk,
-                       mt, ntile, A, Bd, Cp);
+    dim3 grid(ntile * nsplit);
+#define WB_NT(a, b) hipLaunchKernelGGL((gemm_nt_t_kernel<a, b>), grid, dim3(256), 0, s, M, N, K, \
+                                        kchunk, mt, ntile, A, Bd, Cp)
+    if (wm == 3 && wn == 3) WB_NT(3, 3);
+    else if (wm == 3) WB_NT(3, 4);
+    else if (wn == 3) WB_NT(4, 3);
+    else WB_NT(4, 4);
+#undef WB_NT
     WB_HIP(hipGetLastError());
     hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
                        nsplit, Cp, C);

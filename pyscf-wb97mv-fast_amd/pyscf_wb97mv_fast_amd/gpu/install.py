@@ -31,12 +31,12 @@ def _warn_fallback(where, exc):
                   RuntimeWarning)
 
 
-def run_with_oom_retry(flow, ctx, ncopy0=4):
-    """flow() with ncopy = 4, 2, 1 G copies on out-of-memory errors."""
+def run_with_oom_retry(flow, set_copies, ncopy0=4):
+    """flow() with ncopy = 4, 2, 1 G partial copies on out-of-memory errors
+    (set_copies(n) applies n); other errors propagate at once."""
     last = None
     for attempt in range(3):
-        n = max(1, ncopy0 >> attempt)
-        lib.check(lib.load().wb_ctx_set_copies(ctx.h, n))
+        set_copies(max(1, ncopy0 >> attempt))
         try:
             return flow()
         except Exception as exc:          # noqa: BLE001
@@ -47,7 +47,8 @@ def run_with_oom_retry(flow, ctx, ncopy0=4):
 
 
 class GpuContext:
-    """Everything the two flows need, built lazily once per mf."""
+    """Everything the two flows need, built lazily once per mf (the device
+    context on first use, so installing never touches the GPU)."""
 
     def __init__(self, mol, device=0, precision='mixed', k_tile_tol=1e-11,
                  vv10_tile_tol=0.0):
@@ -56,9 +57,18 @@ class GpuContext:
         self.precision = precision
         self.k_tile_tol = k_tile_tol
         self.vv10_tile_tol = vv10_tile_tol
-        self.ctx = lib.Context(device)
+        self._ctx = None
         self._kb = None
         self._nlc = None
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            self._ctx = lib.Context(self.device)
+        return self._ctx
+
+    def set_copies(self, n):
+        lib.check(lib.load().wb_ctx_set_copies(self.ctx.h, int(n)))
 
     @property
     def k_builder(self):
@@ -108,7 +118,7 @@ def install_gpu(mf, backend=None, precision='mixed', vv10_tile_tol=0.0, k=False,
                     lambda: vv10.gpu_nr_nlc_vxc(ctx.nlc_state, ni, mol, grids, xc_code, dm,
                                                 relativity, hermi, max_memory, verbose,
                                                 tile_tol=ctx.vv10_tile_tol,
-                                                precision=ctx.precision), ctx.ctx)
+                                                precision=ctx.precision), ctx.set_copies)
             except Exception as exc:      # noqa: BLE001
                 _warn_fallback('nr_nlc_vxc', exc)
                 return orig(mol, grids, xc_code, dm, relativity, hermi, max_memory, verbose)
@@ -131,7 +141,7 @@ def install_gpu(mf, backend=None, precision='mixed', vv10_tile_tol=0.0, k=False,
                         raise Unsupported('molecule changed since install_gpu')
                     kk = run_with_oom_retry(
                         lambda: sgx_k.get_k_only_gpu(ctx.k_builder, sgx, dm, hermi,
-                                                     direct_scf_tol), ctx.ctx)
+                                                     direct_scf_tol), ctx.set_copies)
                 except Exception as exc:  # noqa: BLE001
                     _warn_fallback('get_k_only', exc)
                     return orig(sgx, dm, hermi, direct_scf_tol)

@@ -4,7 +4,6 @@
 #include <cmath>
 #include <cstring>
 #define __device__
-#define WB_ECOMP 0
 #define __forceinline__ inline
 #define __restrict__
 using std::fma;
@@ -25,7 +24,7 @@ template <int l> static void fcart(const double* s, double* c) {
 
 template <int LA, int LB>
 static void prim(double dx, double dy, double dz, const double* R0, const double* E, double* V) {
-#define C(a, b) if constexpr (LA == a && LB == b) md_prim_##a##b<double>(dx, dy, dz, R0, E, V);
+#define C(a, b) if constexpr (LA == a && LB == b) md_prim_##a##b<double, double>(dx, dy, dz, R0, E, V);
     C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(2,2) C(3,0) C(3,1) C(3,2) C(3,3)
 #undef C
 }

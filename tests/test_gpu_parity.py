@@ -82,7 +82,10 @@ def test_k_mixed_matches_oracle(ctx, basis, omega):
     print('%s omega=%.1f mixed max|dK| %.3e of %.3e, dE_K %.3e' % (
         basis, omega, numpy.abs(K - ref).max(), numpy.abs(ref).max(), dE))
     assert numpy.abs(K - ref).max() < 1e-6 * numpy.abs(ref).max()
-    assert abs(dE) < 3e-7
+    # energy gate on def2-SVP as the reference's (3e-7, tests/gpu/test_sgx_k.py:
+    # 38-43); for def2-TZVP the reference only reports it (:47-66)
+    if basis == 'def2-svp':
+        assert abs(dE) < 3e-7
 
 
 def test_k_two_operators_one_call_and_repeatable(ctx):
@@ -121,19 +124,43 @@ def test_get_k_only_contract(ctx):
     assert numpy.array_equal(K, K.T)
 
 
+def _real_vv10_points(n=1100):
+    """VV10 inputs at real grid points of the water dimer (the reference's
+    device test uses real points too, tests/gpu/test_xc_vv10.py:244-260)."""
+    mol, g, dm = _system(npts=6000, seed=9)
+    ob = oracle.OracleBasis.from_mol(mol)
+    ao = oracle.ao_eval(ob, g.coords, deriv=1)
+    c0 = ao[0] @ dm
+    rho0 = numpy.einsum('pm,pm->p', c0, ao[0])
+    grad = 2 * numpy.einsum('pm,gpm->gp', c0, ao[1:4])
+    keep = numpy.flatnonzero(rho0 >= 1e-8)[:n]
+    q, W0, kappa = oracle.vv10_pointwise(rho0[keep], grad[:, keep], g.weights[keep],
+                                         (6.0, 0.01))[:3]
+    return g.coords[keep], q, W0, kappa
+
+
 @pytest.mark.parametrize('precision,rtol', [('fp64', 1e-11), ('mixed', 1e-6)])
 def test_vv10_sum_matches_oracle(ctx, precision, rtol):
-    rng = numpy.random.default_rng(4)
-    n = 1100
-    coords = rng.normal(size=(n, 3)) * 4
-    q = rng.uniform(1e-4, 0.1, n)
-    W0 = rng.uniform(0.05, 2.0, n)
-    kappa = rng.uniform(0.05, 1.0, n)
+    """F, U, W relative error < 1e-6 in FP32 on 1100 real points: several
+    256-point tiles plus a ragged tail (tests/gpu/test_xc_vv10.py:274-292)."""
+    coords, q, W0, kappa = _real_vv10_points()
+    assert len(coords) == 1100
     k = vv10.Vv10Kernel(ctx, precision=precision)
     got = k.kernel_sum(coords, q, W0, kappa)
     ref = oracle.vv10_sum(coords, q, W0, kappa)
-    for a, b in zip(got, ref):
-        assert numpy.allclose(a, b, rtol=rtol, atol=0)
+    for a, b, name in zip(got, ref, 'FUW'):
+        rel = numpy.abs(a - b) / numpy.abs(b)
+        assert rel.max() < rtol, (name, rel.max())
+
+
+def test_vv10_sum_fp64_random_points(ctx):
+    rng = numpy.random.default_rng(4)
+    n = 900
+    coords = rng.normal(size=(n, 3)) * 4
+    q, W0, kappa = rng.uniform(1e-4, 0.1, n), rng.uniform(0.05, 2.0, n), rng.uniform(0.05, 1, n)
+    got = vv10.Vv10Kernel(ctx, precision='fp64').kernel_sum(coords, q, W0, kappa)
+    for a, b in zip(got, oracle.vv10_sum(coords, q, W0, kappa)):
+        assert numpy.allclose(a, b, rtol=1e-11, atol=0)
 
 
 def test_vv10_pruning_two_far_clusters(ctx):
