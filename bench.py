@@ -62,6 +62,11 @@ def main():
     ap.add_argument('--fp64-steps', type=int, default=2)
     ap.add_argument('--vv10-tol', type=float, default=0.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--mixed-only', action='store_true',
+                    help='time the mixed form only (for profiler runs)')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='only run a device copy of MIB MiB (5 times): the known-size '
+                         'kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -70,6 +75,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        n = args.copy_probe << 18                  # float32 elements
+        a = torch.ones(n, dtype=torch.float32, device='cuda:0')
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_probe_bytes_read': 4 * n, 'copy_probe_bytes_written': 4 * n,
+                          'launches': 5}), flush=True)
+        return
     from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.basis import pack_mol
 
@@ -134,6 +149,11 @@ def main():
     md_ms = ph[3] / 2.0
     achieved = flops / (md_ms * 1e-3) if md_ms > 0 else 0.0
 
+    if args.mixed_only:
+        print(json.dumps({'mixed_ms_per_step': round(1e3 * total / args.steps, 3),
+                          'phases_ms': dict(zip(PHASES, [round(float(v), 4) for v in ph])),
+                          'md_algorithmic_flops_per_operator': flops}), flush=True)
+        return
     # all-FP64 form of the same kernels
     total64, times64, ph64, res64 = run(lib.PREC['fp64'], args.fp64_steps, 1, profile=True)
     value64 = args.fp64_steps / total64

@@ -24,8 +24,8 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
 
 
 def build(verbose=False, out=OUT, defines=()):
-    """defines: extra -D macros (e.g. WB_ECOMP=1, FP32 E-coefficient
-    compensation) for variant builds written to another ``out``."""
+    """defines: extra -D macros for variant builds written to another
+    ``out`` (objects go to their own build folder)."""
     tag = '_'.join(d.replace('=', '') for d in defines)
     obj_dir = os.path.join(BUILD, 'obj' + ('_' + tag if tag else ''))
     os.makedirs(os.path.join(BUILD, 'gen'), exist_ok=True)

@@ -2,14 +2,16 @@
 
 For every shell-pair class la >= lb (l <= 3) it emits
 
-  md_prim_<la><lb>(dx, dy, dz, R0, E, V)
+  md_prim_<la><lb>_oc / _tc(dx, dy, dz, R0, E, V)
       one primitive pair at one grid point: the pruned Hermite Coulomb
       recursion R^0_tuv (t+u+v <= L, L = la+lb) from R0[n] = c_n F_n(T),
       then V[a][b] += sum_tuv Ex[ax][bx][t] Ey[ay][by][u] Ez[az][bz][v] R_tuv
       factorized as z -> y -> x contractions.  E is the primitive pair's
       coefficient record (uniform across the wave, read through the scalar
       cache); V (of type vacc, which may be wider than the arithmetic) is the
-      Cartesian block accumulated over the contraction.
+      Cartesian block accumulated over the contraction.  _oc is the
+      one-centre variant (A = B = P), _tc the two-centre one; they differ in
+      which E products are compensated in FP32 and in the one-centre zeros.
   md_fcart_<l>(Fs, Fc)      Fc[c] = sum_m T[m][c] Fs[m]   (sph -> cart)
   md_gsph_<l>(Gc, Gs)       Gs[m] = sum_c T[m][c] Gc[c]   (cart -> sph)
       (in FP32 every non-trivial constant product is compensated with the
@@ -79,49 +81,65 @@ def gen_r(L):
     return lines, out
 
 
-def gen_prim(la, lb):
+def gen_prim(la, lb, oc):
+    """oc: one-centre variant (P = A = B): E^{ij}_t vanishes unless i+j-t is
+    even, and every remaining E is a function of p alone -- the same for
+    every copy of an element -- so each FP32 product carries the E rounding
+    remainder.  Two-centre variant: only E^{ij}_{i+j} = (2p)^-(i+j) is
+    geometry independent; those products are compensated, the others (which
+    depend on the pair's orientation) are plain."""
     L = la + lb
     ca, cb = cart_components(la), cart_components(lb)
     nb = len(cb)
     naxis = (la + 1) * (lb + 1) * (L + 1)
+    erl = 3 * naxis
 
     def eidx(axis, i, j, t):
         return axis * naxis + (i * (lb + 1) + j) * (L + 1) + t
 
     lines = ['template <typename real, typename vacc>',
-             '__device__ __forceinline__ void md_prim_%d%d(real dx, real dy, real dz,'
-             % (la, lb),
+             '__device__ __forceinline__ void md_prim_%d%d_%s(real dx, real dy, real dz,'
+             % (la, lb, 'oc' if oc else 'tc'),
              '    const real* __restrict__ R0, const real* __restrict__ E,'
              ' vacc* __restrict__ V)',
              '{']
     rl, R = gen_r(L)
     lines += rl
 
-    erl = 3 * naxis
+    def zero(i, j, t):
+        return oc and (i + j - t) % 2 == 1
+
+    def comp(i, j, t):
+        return oc or t == i + j
 
     def emul(axis, i, j, t, x):
         if i == 0 and j == 0 and t == 0:
             return x                      # E^{00}_0 = 1
-        return 'wb_emul<real, %d>(E, %d, %s)' % (erl, eidx(axis, i, j, t), x)
+        fn = 'wb_emulc' if comp(i, j, t) else 'wb_emul'
+        return '%s<real, %d>(E, %d, %s)' % (fn, erl, eidx(axis, i, j, t), x)
 
     def efma(axis, i, j, t, x, acc):
         if i == 0 and j == 0 and t == 0:
             return '%s + %s' % (acc, x)
-        return 'wb_efma<real, %d>(E, %d, %s, %s)' % (erl, eidx(axis, i, j, t), x, acc)
+        fn = 'wb_efmac' if comp(i, j, t) else 'wb_efma'
+        return '%s<real, %d>(E, %d, %s, %s)' % (fn, erl, eidx(axis, i, j, t), x, acc)
 
-    # which (az,bz) / (ay,by,az,bz) combinations are used
+    def chain(axis, i, j, terms):
+        """sum over (index, value) of E^{ij}_index * value, zeros skipped"""
+        terms = [(t, x) for t, x in terms if not zero(i, j, t)]
+        expr = emul(axis, i, j, terms[0][0], terms[0][1])
+        for t, x in terms[1:]:
+            expr = efma(axis, i, j, t, x, expr)
+        return expr
+
     zpairs = sorted({(a[2], b[2]) for a in ca for b in cb})
     sz = {}
     for (az, bz) in zpairs:
-        # t + u <= la + lb - az - bz
         rem = L - az - bz
         for t in range(rem + 1):
             for u in range(rem + 1 - t):
-                terms = [(v, R[(t, u, v)]) for v in range(az + bz + 1)]
                 nm = 'sz_%d%d_%d_%d' % (az, bz, t, u)
-                expr = emul(2, az, bz, terms[0][0], terms[0][1])
-                for v, rn in terms[1:]:
-                    expr = efma(2, az, bz, v, rn, expr)
+                expr = chain(2, az, bz, [(v, R[(t, u, v)]) for v in range(az + bz + 1)])
                 lines.append('    const real %s = %s;' % (nm, expr))
                 sz[(az, bz, t, u)] = nm
     yz = {}
@@ -129,11 +147,8 @@ def gen_prim(la, lb):
     for (ay, by, az, bz) in yzpairs:
         tmax = L - ay - by - az - bz
         for t in range(tmax + 1):
-            terms = [(u, sz[(az, bz, t, u)]) for u in range(ay + by + 1)]
             nm = 'syz_%d%d%d%d_%d' % (ay, by, az, bz, t)
-            expr = emul(1, ay, by, terms[0][0], terms[0][1])
-            for u, sn in terms[1:]:
-                expr = efma(1, ay, by, u, sn, expr)
+            expr = chain(1, ay, by, [(u, sz[(az, bz, t, u)]) for u in range(ay + by + 1)])
             lines.append('    const real %s = %s;' % (nm, expr))
             yz[(ay, by, az, bz, t)] = nm
     # this primitive pair's block is formed on its own and then added to the
@@ -142,9 +157,8 @@ def gen_prim(la, lb):
     for i, a in enumerate(ca):
         for j, b in enumerate(cb):
             ax, bx = a[0], b[0]
-            expr = emul(0, ax, bx, 0, yz[(a[1], b[1], a[2], b[2], 0)])
-            for t in range(1, ax + bx + 1):
-                expr = efma(0, ax, bx, t, yz[(a[1], b[1], a[2], b[2], t)], expr)
+            expr = chain(0, ax, bx, [(t, yz[(a[1], b[1], a[2], b[2], t)])
+                                     for t in range(ax + bx + 1)])
             lines.append('    V[%d] += (vacc)(%s);' % (i * nb + j, expr))
     lines.append('}')
     return lines
@@ -199,21 +213,28 @@ def main(out_path):
              '// the same for every copy of an element, so their rounding would bias',
              '// K coherently; each product is formed with one rounding.',
              'template <typename real, int ERL>',
-             '__device__ __forceinline__ real wb_efma(const real* __restrict__ E, int k, real x,'
+             '__device__ __forceinline__ real wb_efmac(const real* __restrict__ E, int k, real x,'
              ' real acc) {',
              '    if constexpr (sizeof(real) == 4) return fma(E[k], x, fma(E[k + ERL], x, acc));',
              '    else return fma(E[k], x, acc);',
              '}',
              'template <typename real, int ERL>',
-             '__device__ __forceinline__ real wb_emul(const real* __restrict__ E, int k, real x) {',
+             '__device__ __forceinline__ real wb_emulc(const real* __restrict__ E, int k, real x) {',
              '    if constexpr (sizeof(real) == 4) return fma(E[k], x, E[k + ERL] * x);',
              '    else return E[k] * x;',
-             '}']
+             '}',
+             'template <typename real, int ERL>',
+             '__device__ __forceinline__ real wb_efma(const real* __restrict__ E, int k, real x,'
+             ' real acc) { return fma(E[k], x, acc); }',
+             'template <typename real, int ERL>',
+             '__device__ __forceinline__ real wb_emul(const real* __restrict__ E, int k, real x)'
+             ' { return E[k] * x; }']
     for l in range(LMAX + 1):
         lines += gen_transforms(l)
     for la in range(LMAX + 1):
         for lb in range(la + 1):
-            lines += gen_prim(la, lb)
+            lines += gen_prim(la, lb, True)
+            lines += gen_prim(la, lb, False)
     text = '\n'.join(lines) + '\n'
     os.makedirs(os.path.dirname(out_path), exist_ok=True)
     old = open(out_path).read() if os.path.exists(out_path) else None

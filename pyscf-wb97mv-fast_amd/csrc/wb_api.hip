@@ -179,6 +179,7 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         pr.pp0 = C.nprims;
         pr.shA = a; pr.shB = b;
         pr.diag = (a == b);
+        pr.onec = (ab2 == 0.0);
         cart_idx_list(la, comps_a);
         cart_idx_list(lb, comps_b);
         PairScreen ps{};

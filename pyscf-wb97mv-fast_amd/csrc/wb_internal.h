@@ -16,7 +16,7 @@ struct PairRec {              // one shell pair of a class, la >= lb
     int a0, b0;               // AO offsets of shells A and B
     int pp0, npp;             // primitive pairs [pp0, pp0 + npp) in the class arrays
     int shA, shB;             // shell indices (screening)
-    int diag, pad;            // diag: A == B
+    int diag, onec;           // diag: A == B; onec: A and B on one centre
 };
 
 struct PairScreen {           // screening data per pair (pair order of its class)
@@ -73,7 +73,7 @@ struct Ctx {
     // statistics of the last K build
     long long tasks_total = 0, tasks_kept = 0;
     float ms_last = 0.f;
-    int ncopy = 4;
+    int ncopy = 8;
     bool g64 = false;          // mixed form: G accumulated in FP64 (default FP32)
     // per-class work of the last K build: kept tasks, kept primitive pairs
     DevBuf ws_work;

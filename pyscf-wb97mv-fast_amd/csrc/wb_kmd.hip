@@ -21,15 +21,52 @@ namespace wb {
 
 template <int L> struct MdDispatch;
 
-template <int LA, int LB, typename real, typename vacc>
+// ONEC: the one-centre variant (A, B and P coincide)
+template <int LA, int LB, bool ONEC, typename real, typename vacc>
 __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R0,
                                         const real* E, vacc* V)
 {
-#define WB_MD_PRIM(a, b) \
-    if constexpr (LA == a && LB == b) md_prim_##a##b<real, vacc>(dx, dy, dz, R0, E, V);
+#define WB_MD_PRIM(a, b)                                                              \
+    if constexpr (LA == a && LB == b) {                                               \
+        if constexpr (ONEC) md_prim_##a##b##_oc<real, vacc>(dx, dy, dz, R0, E, V);    \
+        else md_prim_##a##b##_tc<real, vacc>(dx, dy, dz, R0, E, V);                   \
+    }
     WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
     WB_MD_PRIM(2, 2) WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1) WB_MD_PRIM(3, 2) WB_MD_PRIM(3, 3)
 #undef WB_MD_PRIM
+}
+
+// the Cartesian block of one shell pair at this lane's grid point, summed
+// over the pair's primitive pairs
+template <int LA, int LB, bool ONEC, typename real, typename vacc>
+__device__ __forceinline__ void md_block(const PairRec& pr, double px, double py, double pz,
+                                         const double* __restrict__ ppP,
+                                         const real* __restrict__ ppE,
+                                         const real* __restrict__ ppC,
+                                         const real* __restrict__ tab, vacc* Vs)
+{
+    constexpr int L = LA + LB;
+    constexpr bool F32 = sizeof(real) == 4;
+    constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1) * (F32 ? 2 : 1);
+    constexpr int CRL = (L + 2) * (F32 ? 2 : 1);   // FP32: (hi, lo) pairs
+    for (int m = pr.pp0; m < pr.pp0 + pr.npp; ++m) {
+        const real dx = (real)(ppP[3 * m] - px);
+        const real dy = (real)(ppP[3 * m + 1] - py);
+        const real dz = (real)(ppP[3 * m + 2] - pz);
+        const real* C = ppC + (size_t)m * CRL;
+        const real r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        real Fb[L + 1], R0[L + 1];
+        if constexpr (F32) {
+            Boys<real>::template eval<L>(fmaf(C[0], r2, C[1] * r2), tab, Fb);
+#pragma unroll
+            for (int n = 0; n <= L; ++n) R0[n] = fmaf(C[2 + 2 * n], Fb[n], C[3 + 2 * n] * Fb[n]);
+        } else {
+            Boys<real>::template eval<L>(C[0] * r2, tab, Fb);
+#pragma unroll
+            for (int n = 0; n <= L; ++n) R0[n] = C[1 + n] * Fb[n];
+        }
+        md_prim<LA, LB, ONEC, real, vacc>(dx, dy, dz, R0, ppE + (size_t)m * ERL, Vs);
+    }
 }
 
 template <int l, typename real>
@@ -63,13 +100,10 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
     constexpr int NSA = 2 * LA + 1, NSB = 2 * LB + 1;
     constexpr bool F32 = sizeof(real) == 4;
-    // FP32 E records carry their rounding remainders (twice the length)
-    constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1) * (F32 ? 2 : 1);
     // the primitive sum of the Cartesian block is FP64 when it fits
     // (<= 36 entries); the larger f-shell blocks belong to single-primitive
     // shells in the def2 sets and stay in the arithmetic type
     using vacc = typename std::conditional<(NCA * NCB <= 36), double, real>::type;
-    constexpr int CRL = (L + 2) * (F32 ? 2 : 1);   // FP32: (hi, lo) pairs
     constexpr int TAB = F32 ? B32_NODES * B32_ROW : 1;
     __shared__ real s_tab[TAB];
     const real* tab = boys_g;
@@ -118,24 +152,10 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
         vacc Vs[NCA * NCB];
 #pragma unroll
         for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
-        for (int m = pr.pp0; m < pr.pp0 + pr.npp; ++m) {
-            const real dx = (real)(ppP[3 * m] - px);
-            const real dy = (real)(ppP[3 * m + 1] - py);
-            const real dz = (real)(ppP[3 * m + 2] - pz);
-            const real* C = ppC + (size_t)m * CRL;
-            const real r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            real Fb[L + 1], R0[L + 1];
-            if constexpr (F32) {
-                Boys<real>::template eval<L>(fmaf(C[0], r2, C[1] * r2), tab, Fb);
-#pragma unroll
-                for (int n = 0; n <= L; ++n) R0[n] = fmaf(C[2 + 2 * n], Fb[n], C[3 + 2 * n] * Fb[n]);
-            } else {
-                Boys<real>::template eval<L>(C[0] * r2, tab, Fb);
-#pragma unroll
-                for (int n = 0; n <= L; ++n) R0[n] = C[1 + n] * Fb[n];
-            }
-            md_prim<LA, LB, real, vacc>(dx, dy, dz, R0, ppE + (size_t)m * ERL, Vs);
-        }
+        // one-centre pairs (uniform across the wave) take the variant whose
+        // FP32 E products all carry their rounding remainders
+        if (pr.onec) md_block<LA, LB, true, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
+        else md_block<LA, LB, false, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
         real V[NCA * NCB];
 #pragma unroll
         for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];

@@ -10,7 +10,7 @@ The K hook dispatches only for ``mf.with_df`` and the long-range copies in
 keeps the original.  Any ``Unsupported`` input or GPU error emits a
 RuntimeWarning with the traceback and calls the original PySCF function
 (gpu/install.py:91-96); an out-of-memory error first retries with half the
-number of G partial copies (twice), the memory lever of this design, then
+number of G partial copies (down to one), the memory lever of this design, then
 falls back (gpu/install.py:99-109).  ``restore_all()`` on the returned
 HookSet removes every patch.  The semilocal XC stays on PySCF's CPU path.
 """
@@ -31,11 +31,11 @@ def _warn_fallback(where, exc):
                   RuntimeWarning)
 
 
-def run_with_oom_retry(flow, set_copies, ncopy0=4):
-    """flow() with ncopy = 4, 2, 1 G partial copies on out-of-memory errors
+def run_with_oom_retry(flow, set_copies, ncopy0=8):
+    """flow() with ncopy = 8, 4, 2, 1 G partial copies on out-of-memory errors
     (set_copies(n) applies n); other errors propagate at once."""
     last = None
-    for attempt in range(3):
+    for attempt in range(4):
         set_copies(max(1, ncopy0 >> attempt))
         try:
             return flow()

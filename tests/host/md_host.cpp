@@ -22,9 +22,15 @@ template <int l> static void fcart(const double* s, double* c) {
     else md_fcart_3<double>(s, c);
 }
 
+// onec: the one-centre variant (valid only when A = B)
 template <int LA, int LB>
-static void prim(double dx, double dy, double dz, const double* R0, const double* E, double* V) {
-#define C(a, b) if constexpr (LA == a && LB == b) md_prim_##a##b<double, double>(dx, dy, dz, R0, E, V);
+static void prim(bool onec, double dx, double dy, double dz, const double* R0, const double* E,
+                 double* V) {
+#define C(a, b)                                                                  \
+    if constexpr (LA == a && LB == b) {                                          \
+        if (onec) md_prim_##a##b##_oc<double, double>(dx, dy, dz, R0, E, V);     \
+        else md_prim_##a##b##_tc<double, double>(dx, dy, dz, R0, E, V);          \
+    }
     C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(2,2) C(3,0) C(3,1) C(3,2) C(3,3)
 #undef C
 }
@@ -34,13 +40,13 @@ static void prim(double dx, double dy, double dz, const double* R0, const double
 // spherical block A_sph (nsa x nsb) via sph->cart of unit vectors, i.e. the
 // same transforms the kernel applies to F and G.
 template <int LA, int LB>
-static void block(int npp, const double* d, const double* R0, const double* E, double* out) {
+static void block(int onec, int npp, const double* d, const double* R0, const double* E, double* out) {
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
     constexpr int NSA = 2 * LA + 1, NSB = 2 * LB + 1, L = LA + LB;
     constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1);
     double V[NCA * NCB] = {0};
     for (int m = 0; m < npp; ++m)
-        prim<LA, LB>(d[3 * m], d[3 * m + 1], d[3 * m + 2], R0 + m * (L + 1), E + m * ERL, V);
+        prim<LA, LB>(onec != 0, d[3 * m], d[3 * m + 1], d[3 * m + 2], R0 + m * (L + 1), E + m * ERL, V);
     // A_sph[mu][nu] = e_nu^T (T_B V^T T_A^T) e_mu, built column by column as the
     // kernel does: FaC = fcart(e_mu), gbc = V^T FaC, gbs = gsph(gbc)
     for (int mu = 0; mu < NSA; ++mu) {
@@ -57,9 +63,9 @@ static void block(int npp, const double* d, const double* R0, const double* E, d
     }
 }
 
-extern "C" int md_block(int la, int lb, int npp, const double* d, const double* R0,
+extern "C" int md_block(int la, int lb, int onec, int npp, const double* d, const double* R0,
                         const double* E, double* out) {
-#define C(a, b) if (la == a && lb == b) { block<a, b>(npp, d, R0, E, out); return 0; }
+#define C(a, b) if (la == a && lb == b) { block<a, b>(onec, npp, d, R0, E, out); return 0; }
     C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(2,2) C(3,0) C(3,1) C(3,2) C(3,3)
 #undef C
     return 1;

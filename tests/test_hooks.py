@@ -117,7 +117,7 @@ def test_oom_retry_halves_copies_then_gives_up():
         raise GpuOOM('out of memory')
     with pytest.raises(GpuOOM):
         install.run_with_oom_retry(flow, seen.append)
-    assert seen == [4, 2, 1]
+    assert seen == [8, 4, 2, 1]
     seen.clear()
     state = {'n': 0}
 
@@ -127,7 +127,7 @@ def test_oom_retry_halves_copies_then_gives_up():
             raise MemoryError('out of memory')
         return 'ok'
     assert install.run_with_oom_retry(flaky, seen.append) == 'ok'
-    assert seen == [4, 2]
+    assert seen == [8, 4]
     with pytest.raises(ValueError):
         install.run_with_oom_retry(lambda: (_ for _ in ()).throw(ValueError('x')), seen.append)
 

@@ -57,13 +57,18 @@ def _hermite(la, lb, p, xpa, xpb):
     return E
 
 
+@pytest.mark.parametrize('onec', [False, True])
 @pytest.mark.parametrize('omega', [0.0, 0.3])
-def test_generated_md_blocks(md_lib, omega):
+def test_generated_md_blocks(md_lib, omega, onec):
+    """Two-centre variant on A != B; one-centre variant (which drops the E
+    terms that vanish when A = B = P) on two shells of one centre.  Blocks
+    that vanish by symmetry (one-centre, C = A) are compared on the scale of
+    the pair's R0 values."""
     rng = numpy.random.default_rng(11)
     for la in range(4):
         for lb in range(la + 1):
             A = rng.normal(size=3)
-            B = A + rng.normal(size=3)
+            B = A.copy() if onec else A + rng.normal(size=3)
             ea, eb = rng.uniform(0.3, 4.0, 2), rng.uniform(0.3, 4.0, 2)
             ca = mole.normalize_contracted(la, ea, numpy.ones((2, 1)))[:, 0]
             cb = mole.normalize_contracted(lb, eb, numpy.ones((2, 1)))[:, 0]
@@ -88,12 +93,13 @@ def test_generated_md_blocks(md_lib, omega):
                             E += list(_hermite(la, lb, p, P[ax] - A[ax], P[ax] - B[ax]).ravel())
                 d, R0, E = (numpy.ascontiguousarray(v, dtype=numpy.float64) for v in (d, R0, E))
                 out = numpy.zeros((2 * la + 1) * (2 * lb + 1))
-                rc = md_lib.md_block(la, lb, 4, d.ctypes.data_as(ctypes.c_void_p),
+                rc = md_lib.md_block(la, lb, int(onec), 4, d.ctypes.data_as(ctypes.c_void_p),
                                      R0.ctypes.data_as(ctypes.c_void_p),
                                      E.ctypes.data_as(ctypes.c_void_p),
                                      out.ctypes.data_as(ctypes.c_void_p))
                 assert rc == 0
                 ref = oracle.esp_dense(basis, C[None], omega)[0][:2 * la + 1, 2 * la + 1:]
                 got = out.reshape(2 * la + 1, 2 * lb + 1)
-                assert numpy.allclose(got, ref, rtol=1e-11, atol=1e-13 * numpy.abs(ref).max()), \
+                assert numpy.allclose(got, ref, rtol=1e-11,
+                                      atol=1e-13 * max(numpy.abs(ref).max(), numpy.abs(R0).max())), \
                     (la, lb, omega, numpy.abs(got - ref).max())
