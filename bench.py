@@ -35,6 +35,11 @@ FLOPS_PRIM = {(0, 0): 43, (1, 0): 89, (1, 1): 223, (2, 0): 182, (2, 1): 516,
               (2, 2): 1256, (3, 0): 358, (3, 1): 1081, (3, 2): 2674, (3, 3): 5688}
 PEAK_FP32_PACKED = 157.3e12        # MI355X published FP32 vector peak
 PEAK_FP32_VALU = 78.6e12           # without packed FP32 (not used by this build)
+# HBM bytes per operator of the exchange kernels (all classes), measured with
+# rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload and scaled by the same
+# counters' reading for a plain copy of known size; the summary file holds
+# the commands and the calibration
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'md_k_kernel_water27_svp_traffic.txt')
 PHASES = ['ao', 'fgemm', 'screen', 'exchange', 'kgemm', 'nlc_ao', 'nlc_rho', 'vv10',
           'nlc_fock']
 
@@ -52,6 +57,17 @@ def build_workload(seed=0):
     log('workload: nao %d, %d grid points, setup %.1f s' % (mol.nao, g.size,
                                                             time.perf_counter() - t))
     return mol, g, dm
+
+
+def measured_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_operator:'):
+                    return int(line.split(':')[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -192,7 +208,10 @@ def main():
                      'achieved': round(achieved / 1e12, 3), 'peak': PEAK_FP32_PACKED / 1e12,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_PACKED, 4),
                      'frac_vs_nonpacked_78.6': round(achieved / PEAK_FP32_VALU, 4),
-                     'traffic': None, 'tasks_kept': int(kept[1]), 'tasks_total': int(kept[0]),
+                     'traffic': measured_traffic(),
+                     'traffic_unit': 'bytes per operator (rocprofv3 FETCH_SIZE + WRITE_SIZE, '
+                                     'copy-calibrated; profiles/md_k_kernel_water27_svp_traffic.txt)',
+                     'tasks_kept': int(kept[1]), 'tasks_total': int(kept[0]),
                      'algorithmic_flops_per_launch': flops},
         'cpu_baseline': cpu,
     }

@@ -126,6 +126,13 @@ int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, cons
 int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, int deriv,
                double* out);
 
+/* Boys function F_0..F_nmax (nmax <= 6) at n values T (host arrays), out
+ * (nmax + 1, n), by the device routines the exchange kernels use: FP32
+ * (WB_PREC_MIXED, T rounded to FP32) or FP64.  Replaces boys_eval(cp, T,
+ * nmax) (gpu/boys.py:194-205); nmax outside 0..6 is WB_INVALID as the
+ * reference's ValueError. */
+int wb_boys_eval(wb_ctx_t ctx, int n, const double* T, int nmax, int precision, double* out);
+
 /* Test hook (no reference counterpart): the FP64 MFMA GEMMs on host data.
  * kind 0: C[M][N] = A[M][K] B[K][N]; kind 1: C[M][N] = sum_k A[M][k]
  * B[N][k] w[k] (w may be NULL); kinds 2 / 3: the same with the 64 x 64

@@ -35,14 +35,29 @@ static long double boys_ld(int m, long double T)
 
 void boys_tables(std::vector<float>& t32, std::vector<double>& t64)
 {
-    t32.assign(B32_NODES * B32_ROW, 0.f);
-    for (int k = 0; k < B32_NODES; ++k) {
-        const long double T = 0.25L * k;
-        for (int m = 0; m <= 13; ++m) {
-            const long double v = m <= 12 ? boys_ld(m, T) : expl(-T);
-            const float hi = (float)v;
-            t32[k * B32_ROW + m] = hi;
-            t32[k * B32_ROW + B32_LO + m] = (float)(v - (long double)hi);
+    t32.assign(B32_TOTAL, 0.f);
+    long double fact[7] = {1, 1, 2, 6, 24, 120, 720};
+    for (int L = 0; L <= B32_LMAX; ++L) {
+        float* tab = t32.data() + b32_off(L);
+        const int rs = b32_row(L);
+        for (int k = 0; k < B32_NODES; ++k) {
+            const long double T = 0.25L * k;
+            float* row = tab + k * rs;
+            // orders with a Taylor series in this table, and where each sits
+            const int n0 = L <= 1 ? 0 : L, n1 = L <= 1 ? L : L;
+            for (int n = n0; n <= n1; ++n) {
+                float* c = row + (L == 1 ? 7 * n : 0);
+                for (int j = 0; j <= 6; ++j)
+                    c[j] = (float)(boys_ld(n + j, T) / fact[j]);
+                const long double v = boys_ld(n, T);
+                const int lo = L == 0 ? 7 : (L == 1 ? 14 + n : 7);
+                row[lo] = (float)(v - (long double)c[0]);
+            }
+            if (L >= 2) {
+                const long double e = expl(-T);
+                row[8] = (float)e;
+                row[9] = (float)(e - (long double)row[8]);
+            }
         }
     }
     t64.assign(B64_NODES * B64_ROW, 0.0);
@@ -876,6 +891,27 @@ int wb_test_gemm(wb_ctx_t ctx, int kind, int M, int N, int K, const double* A, c
             gemm_nt_reduce(c->stream, M, N, K, dA.as<double>(), dB.p, false, 1, 0,
                            w ? dw.as<double>() : nullptr, dC.as<double>(), ws);
         WB_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * (size_t)M * N, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_boys_eval(wb_ctx_t ctx, int n, const double* T, int nmax, int precision, double* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (nmax < 0 || nmax > B32_LMAX) throw Error(WB_INVALID, "nmax must be 0..6");
+        if (n < 0 || (n > 0 && (!T || !out))) throw Error(WB_INVALID, "bad Boys arguments");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "precision must be WB_PREC_MIXED or WB_PREC_FP64");
+        WB_HIP(hipSetDevice(c->device));
+        DevBuf dT, dF;
+        upload(dT, T, sizeof(double) * n, c->stream);
+        dF.ensure(sizeof(double) * (size_t)(nmax + 1) * n);
+        launch_boys_eval(c->stream, nmax, n, dT.as<double>(), precision == WB_PREC_MIXED,
+                         c->d_b32.p, c->d_b64.p, dF.as<double>());
+        if (n)
+            WB_HIP(hipMemcpyAsync(out, dF.p, sizeof(double) * (size_t)(nmax + 1) * n,
+                                  hipMemcpyDeviceToHost, c->stream));
         WB_HIP(hipStreamSynchronize(c->stream));
     })
 }

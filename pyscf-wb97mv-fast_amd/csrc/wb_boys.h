@@ -14,22 +14,32 @@
 
 namespace wb {
 
-// FP32: nodes k/4, k = 0..144; row = F_0..F_12, e^{-T_k} (hi, 14 values),
-// then their FP32 rounding remainders (lo, 14 values), padded to 36 floats:
-// the table sits in LDS and lanes of a wave read neighbouring nodes; a
-// 36-float stride starts consecutive rows 4 banks apart, so 8 consecutive
-// nodes are read without bank conflicts (a 32-float stride would put every
-// row on the same banks)
+// FP32: nodes k/4, k = 0..144, one table per highest order L = 0..6 (a
+// kernel class needs only its own).  Taylor coefficients are stored divided
+// by j!, c_{n,j} = F_{n+j}(T_k) / j!, so the series is a plain Horner chain
+// in x = T_k - T.  Row of table L:
+//   L = 0:  c_{0,0..6}, lo_0                         (8 values)
+//   L = 1:  c_{0,0..6}, c_{1,0..6}, lo_0, lo_1       (16 values)
+//   L >= 2: c_{L,0..6}, lo_L, e_hi, e_lo             (10 values)
+// lo: FP32 rounding remainder of the leading value; e = e^{-T_k}.  Rows are
+// padded to a multiple of 4 floats (16-byte loads) plus 4, so that rows of
+// neighbouring nodes, which the lanes of a wave read together, start on
+// different LDS banks.
 constexpr int B32_NODES = 145;
-constexpr int B32_ROW = 36;
-constexpr int B32_LO = 14;
 constexpr float B32_SWITCH = 36.0f;
+__host__ __device__ constexpr int b32_vals(int L) { return L == 0 ? 8 : (L == 1 ? 16 : 10); }
+__host__ __device__ constexpr int b32_row(int L) { return (b32_vals(L) + 3) / 4 * 4 + 4; }
+__host__ __device__ constexpr int b32_off(int L) {
+    return L == 0 ? 0 : b32_off(L - 1) + B32_NODES * b32_row(L - 1);
+}
+constexpr int B32_LMAX = 6;
+constexpr int B32_TOTAL = b32_off(B32_LMAX + 1);
 // FP64: nodes k/8, k = 0..320; row = F_0..F_14, e^{-T_k}
 constexpr int B64_NODES = 321;
 constexpr int B64_ROW = 16;
 constexpr double B64_SWITCH = 40.0;
 
-// host: fill the tables (FP64 values; caller rounds for FP32)
+// host: fill the tables (FP32: all orders at their offsets b32_off(L))
 void boys_tables(std::vector<float>& t32, std::vector<double>& t64);
 
 // Exact-ish x * (1/d) for a small odd integer d: one FMA correction makes the
@@ -41,38 +51,41 @@ __device__ __forceinline__ float div_small(float x, float d, float inv) {
     return fmaf(r, inv, y);
 }
 
-// The leading table values and sqrt(pi)/2 carry their FP32 rounding
-// remainders: a table value serves every (point, pair) near its node, so
-// its rounding would bias all of them the same way.
+// sum_{j=1..6} c[j] x^j (Horner)
+__device__ __forceinline__ float taylor6(const float* c, float x) {
+    float s = c[6];
+#pragma unroll
+    for (int j = 5; j >= 1; --j) s = fmaf(s, x, c[j]);
+    return s * x;
+}
+
+// F_0..F_L(T); tab is the order-L table.  The leading table values and
+// sqrt(pi)/2 carry their FP32 rounding remainders: a table value serves
+// every (point, pair) near its node, so its rounding would bias all of them
+// the same way.
 template <int L>
 __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
     if (T < B32_SWITCH) {
         const int k = (int)(T * 4.0f + 0.5f);
         const float x = 0.25f * (float)k - T;          // -h, exact
-        const float* row = tab + k * B32_ROW;
-        float p[7];
-        p[0] = 1.0f;
-        p[1] = x;
-#pragma unroll
-        for (int j = 2; j < 7; ++j) p[j] = p[j - 1] * x * (1.0f / (float)j);
-        if (L <= 1) {
-#pragma unroll
-            for (int n = 0; n <= L; ++n) {
-                float s = row[n + 6] * p[6];
-#pragma unroll
-                for (int j = 5; j >= 1; --j) s = fmaf(row[n + j], p[j], s);
-                F[n] = row[n] + (row[B32_LO + n] + s);
-            }
+        const float* row = tab + k * b32_row(L);
+        if constexpr (L == 0) {
+            F[0] = row[0] + (row[7] + taylor6(row, x));
+        } else if constexpr (L == 1) {
+            F[0] = row[0] + (row[14] + taylor6(row, x));
+            F[1] = row[7] + (row[15] + taylor6(row + 7, x));
         } else {
-            float s = row[L + 6] * p[6];
-#pragma unroll
-            for (int j = 5; j >= 1; --j) s = fmaf(row[L + j], p[j], s);
-            F[L] = row[L] + (row[B32_LO + L] + s);
-            float es = p[6];
-#pragma unroll
-            for (int j = 5; j >= 1; --j) es += p[j];
+            F[L] = row[0] + (row[7] + taylor6(row, x));
+            // e^{x} - 1 to O(x^7)
+            float es = 1.0f / 720.0f;
+            es = fmaf(es, x, 1.0f / 120.0f);
+            es = fmaf(es, x, 1.0f / 24.0f);
+            es = fmaf(es, x, 1.0f / 6.0f);
+            es = fmaf(es, x, 0.5f);
+            es = fmaf(es, x, 1.0f);
+            es *= x;
             // e^{-T} = e^{-T_k} e^{x} = (e_hi + e_lo)(1 + es)
-            const float e = row[13] + fmaf(row[13], es, row[B32_LO + 13]);
+            const float e = row[8] + fmaf(row[8], es, row[9]);
             const float t2 = T + T;
 #pragma unroll
             for (int n = L; n > 0; --n)

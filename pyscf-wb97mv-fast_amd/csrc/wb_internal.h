@@ -116,6 +116,9 @@ void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int na
 void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
                   int nshell, float tol, int* counts, int* ids, unsigned long long* work);
 // f32: FP32 integrals (else FP64); g64: accumulate G in FP64 (always with FP64 integrals)
+// F_0..F_L at n values T (device arrays), out [L+1][n]
+void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
+                      const void* b32, const void* b64, double* out);
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
                const void* F, void* G, size_t gstride, int ncopy,
                const int* counts, const int* ids, const void* C, const void* boys);

@@ -104,11 +104,13 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     // (<= 36 entries); the larger f-shell blocks belong to single-primitive
     // shells in the def2 sets and stay in the arithmetic type
     using vacc = typename std::conditional<(NCA * NCB <= 36), double, real>::type;
-    constexpr int TAB = F32 ? B32_NODES * B32_ROW : 1;
+    // FP32: this class's Boys table (order L) is staged in LDS
+    constexpr int TAB = F32 ? B32_NODES * b32_row(L) : 1;
     __shared__ real s_tab[TAB];
     const real* tab = boys_g;
     if constexpr (F32) {
-        for (int i = threadIdx.x; i < TAB; i += blockDim.x) s_tab[i] = boys_g[i];
+        const real* src = boys_g + b32_off(L);
+        for (int i = threadIdx.x; i < TAB; i += blockDim.x) s_tab[i] = src[i];
         __syncthreads();
         tab = s_tab;
     }
@@ -225,6 +227,42 @@ void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool 
     WB_CASE(2, 2) WB_CASE(3, 0) WB_CASE(3, 1) WB_CASE(3, 2) WB_CASE(3, 3)
 #undef WB_CASE
     throw Error(WB_UNSUPPORTED, "shell-pair class beyond (f, f)");
+}
+
+// F_0..F_L(T) of the device Boys routines for a list of T (exposed for
+// tests: the same code the exchange kernels inline)
+template <int L, typename real>
+__global__ __launch_bounds__(256)
+void boys_eval_kernel(int n, const double* __restrict__ T, const real* __restrict__ tab,
+                      double* __restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    real F[L + 1];
+    Boys<real>::template eval<L>((real)T[i], tab, F);
+#pragma unroll
+    for (int m = 0; m <= L; ++m) out[(size_t)m * n + i] = (double)F[m];
+}
+
+void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
+                      const void* b32, const void* b64, double* out)
+{
+    if (n == 0) return;
+    const int blocks = (n + 255) / 256;
+#define WB_BCASE(XL)                                                                     \
+    if (L == XL) {                                                                       \
+        if (f32)                                                                         \
+            hipLaunchKernelGGL((boys_eval_kernel<XL, float>), dim3(blocks), dim3(256), 0, s, n, T, \
+                               (const float*)b32 + b32_off(XL), out);                    \
+        else                                                                             \
+            hipLaunchKernelGGL((boys_eval_kernel<XL, double>), dim3(blocks), dim3(256), 0, s, n, T, \
+                               (const double*)b64, out);                                 \
+        WB_HIP(hipGetLastError());                                                       \
+        return;                                                                          \
+    }
+    WB_BCASE(0) WB_BCASE(1) WB_BCASE(2) WB_BCASE(3) WB_BCASE(4) WB_BCASE(5) WB_BCASE(6)
+#undef WB_BCASE
+    throw Error(WB_INVALID, "Boys order beyond 6");
 }
 
 // ------------------------------------------------------------ screening ----

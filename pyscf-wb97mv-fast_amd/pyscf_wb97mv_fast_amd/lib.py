@@ -51,6 +51,7 @@ _SIGS = {
     'wb_nlc_vxc_dev': (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
     'wb_vv10_sum': (_i, [_vp, _i, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
     'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
+    'wb_boys_eval': (_i, [_vp, _i, _vp, _i, _i, _vp]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
 }
 EXPORTS = tuple(_SIGS)
