@@ -800,8 +800,11 @@ int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, cons
         }
         upload(dt, ts.data(), sizeof(float4) * ts.size(), c->stream);
         double* out = (double*)dout.ensure(sizeof(double) * 3 * (size_t)n);
-        launch_vv10_sum(c->stream, n, dp.p, dk.p, dt.as<float>(), ntiles, (float)tile_tol, f64,
-                        out, out + n, out + 2 * (size_t)n);
+        DevBuf dn, dws;
+        upload(dn, &n, sizeof(int), c->stream);
+        dws.ensure(vv10_sym_bytes(n, f64, nullptr));
+        launch_vv10_sym(c->stream, dn.as<int>(), n, dp.p, dk.p, (const float4*)dt.p,
+                        (float)tile_tol, f64, dws.p, out, out + n, out + 2 * (size_t)n);
         WB_HIP(hipMemcpyAsync(F, out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         WB_HIP(hipMemcpyAsync(U, out + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         WB_HIP(hipMemcpyAsync(W, out + 2 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));

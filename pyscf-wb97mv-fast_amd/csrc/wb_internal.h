@@ -126,11 +126,14 @@ void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool 
 // VV10
 void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T,
                 const double* X, double* rho /* [4][Ppad] */);
-// pts: (x, y, z, W0) and pk: (kappa, q) per point, float or double (f64);
-// tiles: per 64-point tile two float4 (centre xyz, radius | W0min, kmin, qsum, 0)
-void launch_vv10_sum(hipStream_t s, int n, const void* pts, const void* pk,
-                     const float* tiles, int ntiles, float tol, bool f64, double* F,
-                     double* U, double* W);
+// VV10 double sum over n = *nptr <= ncap compacted points.  pts: (x, y, z,
+// W0) and pk: (kappa, q) per point, float or double (f64); tiles: per
+// 64-point tile two float4 (centre xyz, radius | W0min, kmin, qsum, 0); ws:
+// vv10_sym_bytes(ncap) bytes.  Out: F (times -3/2), U, W per point.
+size_t vv10_sym_bytes(int ncap, bool f64, int* nsplit_out);
+void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, const void* pk,
+                     const float4* tiles, float tol, bool f64, void* ws, double* F, double* U,
+                     double* W);
 
 void zero_bytes(hipStream_t s, void* p, size_t bytes);
 void transpose(hipStream_t s, int n, const double* a, double* b);

@@ -610,69 +610,140 @@ void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T, con
 }
 
 // --------------------------------------------------------- VV10 sum -------
-// Outer points i: two per lane (128 per wave).  Inner points j stream
-// through LDS in tiles of VJ = 256 (staged by the whole block); each wave
-// decides per 64-point sub-tile, from the tile statistics and its own
-// i-range (wave reductions), whether the whole sub-tile can be skipped:
+// Tile pruning (tol > 0): a wave skips a 64-point tile when, from the tile
+// statistics and its own points' (wave reductions),
 //   g >= Rmin^2 W0min_I + kmin_I, g' >= Rmin^2 W0min_J + kmin_J,
-//   F <= qsum_J / (g g' (g+g')), U <= 2F/g, W <= U Rmax^2
-// (the reference's host-side tile-pair bound, gpu/vv10.py:21-31, 147-164).
-// Per pair: T = q_j / (g g' (g+g')), U += T (1/g + 1/(g+g')), W += that R^2,
-// with one reciprocal (refined by a Newton step) per pair.  Sums over 8
-// inner points are FP32, beyond that FP64, in a fixed order.
-constexpr int VJ = 256;
-constexpr int VG = 8;      // inner points per FP32 partial sum
-
-template <typename real>
-__device__ __forceinline__ real vv_rcp(real x);
-template <> __device__ __forceinline__ float vv_rcp<float>(float x) {
-    float r = __builtin_amdgcn_rcpf(x);
-    return fmaf(r, fmaf(-x, r, 1.0f), r);
-}
-template <> __device__ __forceinline__ double vv_rcp<double>(double x) { return 1.0 / x; }
-
+//   F <= qsum / (g g' (g+g')), U <= 2F/g, W <= U Rmax^2
+// bounds every contribution in both directions below tol (the reference's
+// host-side tile-pair bound, gpu/vv10.py:21-31, 147-164).
 template <typename real> struct VP4 { real x, y, z, w; };   // coords, W0
 template <typename real> struct VP2 { real k, q; };         // kappa, q = rho w
 
+// ------------------------------------------------ symmetric VV10 sum -----
+// Each unordered pair of points is evaluated once and feeds both sides:
+//   D = 1 / (g g' (g+g')),  T_i = q_j D,  T_j = q_i D,
+//   U_i += T_i (1/g + 1/(g+g')),  U_j += T_j (1/g' + 1/(g+g')),  W += U R^2
+// with 1/g = g' (g+g') D, 1/g' = g (g+g') D, 1/(g+g') = g g' D.
+//
+// Points are grouped by 512 (one block: 4 waves x 2 points per lane); tiles
+// are 64 points, 8 per group.  Group b pairs with the tiles of groups
+// b+1 .. b+h (mod B), h = (B-1)/2, and for even B group b < B/2 also with
+// group b + B/2: every pair of distinct groups is covered exactly once.  A
+// group's own tiles are summed one-sided (every ordered pair, i = j
+// included).  nsplit blocks share one group's partner range.
+//
+// Within a tile the wave runs 64 steps; at step k lane l pairs its points
+// with tile point (l + k) mod 64, read from LDS.  The tile point's
+// accumulators travel with it: after every step they rotate one lane down
+// (DPP wave rotate), so lane l holds the accumulators of the point it
+// pairs with next.  Every 16 steps both sides' FP32 sums are moved into FP64
+// (i side: registers; tile side: LDS, added up at the end of the tile over
+// the 4 waves in a fixed order) and written as one record per (group,
+// partner tile).  A final pass adds, per point, its i-side parts and the
+// records of the groups that visited its tile, in a fixed order: no
+// atomics, bit-reproducible.
+constexpr int VS_IG = 512;           // points per group (block)
+constexpr int VS_GT = VS_IG / 64;    // tiles per group
+constexpr int VS_SUB = 16;           // steps per FP32 partial sum
+
+__host__ __device__ inline int vs_groups(int n) { return (n + VS_IG - 1) / VS_IG; }
+__host__ __device__ inline int vs_hmax(int B) { return B / 2; }
+__host__ __device__ inline int vs_npartner(int B, int b) {
+    return (B - 1) / 2 + ((B % 2 == 0 && b < B / 2) ? 1 : 0);
+}
+
+// lane l receives the value of lane (l + 1) mod 64 (DPP wave_rol:1)
+__device__ __forceinline__ float vs_rot(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double vs_rot(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 template <typename real>
-__device__ __forceinline__ void vv10_body(int n, const VP4<real>* __restrict__ pt,
-                                          const VP2<real>* __restrict__ pk,
-                                          const float4* __restrict__ ts, int ntiles, float tol,
-                                          double* __restrict__ Fo, double* __restrict__ Uo,
-                                          double* __restrict__ Wo)
+__device__ __forceinline__ real vs_rcp(real x);
+template <> __device__ __forceinline__ float vs_rcp<float>(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+template <> __device__ __forceinline__ double vs_rcp<double>(double x) { return 1.0 / x; }
+
+// one pair (i = a, j = p): i-side sums (f, u, w) and, when SYM, j-side
+// contributions added to (cf, cu, cw)
+template <typename real, bool SYM>
+__device__ __forceinline__ void vs_pair(const VP4<real>& a, const VP2<real>& ka,
+                                        const VP4<real>& p, const VP2<real>& kp,
+                                        real& f, real& u, real& w,
+                                        real& cf, real& cu, real& cw)
 {
-    __shared__ VP4<real> s_pt[VJ];
-    __shared__ VP2<real> s_pk[VJ];
+    const real dx = a.x - p.x, dy = a.y - p.y, dz = a.z - p.z;
+    const real R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const real g = fma(R2, a.w, ka.k);
+    const real gp = fma(R2, p.w, kp.k);
+    const real gt = g + gp;
+    const real gg = g * gp;
+    const real D = vs_rcp<real>(gg * gt);
+    const real h = gt * D;                 // 1 / (g g')
+    const real it = gg * D;                // 1 / (g + g')
+    const real Ti = kp.q * D;
+    const real Ui = Ti * fma(gp, h, it);   // Ti (1/g + 1/(g+g'))
+    f += Ti; u += Ui; w = fma(Ui, R2, w);
+    if constexpr (SYM) {
+        const real Tj = ka.q * D;
+        const real Uj = Tj * fma(g, h, it);
+        cf += Tj; cu += Uj; cw = fma(Uj, R2, cw);
+    }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256)
+void vv10_sym_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__ pt,
+                     const VP2<real>* __restrict__ pk, const float4* __restrict__ ts, float tol,
+                     int nsplit, size_t pcap, double* __restrict__ ipart, real* __restrict__ jrec)
+{
+    // the tile twice over, so that point (l + k) mod 64 is entry l + k
+    __shared__ VP4<real> s_pt[128];
+    __shared__ VP2<real> s_pk[128];
+    __shared__ real s_j[4][3][4][64];        // [wave][quantity][flush][slot]
+    const int n = *nptr;
+    const int B = vs_groups(n);
+    const int b = blockIdx.x / nsplit, part = blockIdx.x % nsplit;
+    if (b >= B) return;                      // uniform per block
+    const int ntiles = (n + 63) / 64;
+    const int hmax = vs_hmax(B);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ibase = (blockIdx.x * 4 + wave) * 128;
-    const int i0 = ibase + lane, i1 = ibase + 64 + lane;
+    const int i0 = b * VS_IG + wave * 128 + lane, i1 = i0 + 64;
     const bool v0 = i0 < n, v1 = i1 < n;
-    const VP4<real> dummy{0, 0, 0, 1};
-    const VP2<real> dk{1, 0};
-    const VP4<real> a = v0 ? pt[i0] : dummy;
-    const VP4<real> b = v1 ? pt[i1] : a;
-    const VP2<real> ka = v0 ? pk[i0] : dk;
-    const VP2<real> kb = v1 ? pk[i1] : ka;
-    // wave statistics of the outer points (for pruning)
-    float cx = 0, cy = 0, cz = 0, rI = 0, wminI = 0, kminI = 0;
+    // padding points carry q = 0 (and kappa = 1): they add nothing to anyone
+    auto ld4 = [&](int i, bool v) { VP4<real> r{0, 0, 0, 0}; if (v) r = pt[i]; return r; };
+    auto ld2 = [&](int i, bool v) { VP2<real> r{1, 0}; if (v) r = pk[i]; return r; };
+    const VP4<real> a = ld4(i0, v0), c = ld4(i1, v1);
+    const VP2<real> ka = ld2(i0, v0), kc = ld2(i1, v1);
+    // wave statistics of its points (pruning)
+    float cx = 0, cy = 0, cz = 0, rI = 0, wminI = 0, kminI = 0, qI = 0;
     if (tol > 0.f) {
         const float cnt = (float)__popcll(__ballot(v0)) + (float)__popcll(__ballot(v1));
-        float sx = (v0 ? (float)a.x : 0.f) + (v1 ? (float)b.x : 0.f);
-        float sy = (v0 ? (float)a.y : 0.f) + (v1 ? (float)b.y : 0.f);
-        float sz = (v0 ? (float)a.z : 0.f) + (v1 ? (float)b.z : 0.f);
+        float sx = (v0 ? (float)a.x : 0.f) + (v1 ? (float)c.x : 0.f);
+        float sy = (v0 ? (float)a.y : 0.f) + (v1 ? (float)c.y : 0.f);
+        float sz = (v0 ? (float)a.z : 0.f) + (v1 ? (float)c.z : 0.f);
+        float sq = (float)ka.q + (float)kc.q;
         for (int o = 32; o > 0; o >>= 1) {
             sx += __shfl_xor(sx, o, 64);
             sy += __shfl_xor(sy, o, 64);
             sz += __shfl_xor(sz, o, 64);
+            sq += __shfl_xor(sq, o, 64);
         }
-        cx = sx / fmaxf(cnt, 1.f); cy = sy / fmaxf(cnt, 1.f); cz = sz / fmaxf(cnt, 1.f);
+        const float cn = fmaxf(cnt, 1.f);
+        cx = sx / cn; cy = sy / cn; cz = sz / cn; qI = sq * 1.0001f;
         auto dist = [&](const VP4<real>& p) {
             const float ex = (float)p.x - cx, ey = (float)p.y - cy, ez = (float)p.z - cz;
             return sqrtf(ex * ex + ey * ey + ez * ez);
         };
-        float r = fmaxf(v0 ? dist(a) : 0.f, v1 ? dist(b) : 0.f);
-        float wm = fminf(v0 ? (float)a.w : 3e38f, v1 ? (float)b.w : 3e38f);
-        float km = fminf(v0 ? (float)ka.k : 3e38f, v1 ? (float)kb.k : 3e38f);
+        float r = fmaxf(v0 ? dist(a) : 0.f, v1 ? dist(c) : 0.f);
+        float wm = fminf(v0 ? (float)a.w : 3e38f, v1 ? (float)c.w : 3e38f);
+        float km = fminf(v0 ? (float)ka.k : 3e38f, v1 ? (float)kc.k : 3e38f);
         for (int o = 32; o > 0; o >>= 1) {
             r = fmaxf(r, __shfl_xor(r, o, 64));
             wm = fminf(wm, __shfl_xor(wm, o, 64));
@@ -681,104 +752,175 @@ __device__ __forceinline__ void vv10_body(int n, const VP4<real>* __restrict__ p
         rI = r * 1.0001f + 1e-5f; wminI = wm * 0.9999f; kminI = km * 0.9999f;
     }
     double F0 = 0, U0 = 0, W0 = 0, F1 = 0, U1 = 0, W1 = 0;
-    for (int j0 = 0; j0 < n; j0 += VJ) {
+
+    // the block's work list: own tiles (part 0, one-sided), then its share
+    // of the partner tiles
+    const int np = vs_npartner(B, b);
+    const int tot = np * VS_GT;
+    const int lo = (int)((long long)tot * part / nsplit), hi = (int)((long long)tot * (part + 1) / nsplit);
+    const int nown = part == 0 ? VS_GT : 0;
+    for (int it = -nown; it < hi - lo; ++it) {
+        const bool own = it < 0;
+        int t, d = 0, u = 0;
+        if (own) {
+            t = b * VS_GT + (it + VS_GT);
+        } else {
+            const int x = lo + it;
+            d = x / VS_GT + 1;
+            u = x % VS_GT;
+            t = ((b + d) % B) * VS_GT + u;
+        }
+        if (t >= ntiles) continue;           // uniform per block
         __syncthreads();
-        {
-            const int jj = j0 + threadIdx.x;
-            s_pt[threadIdx.x] = jj < n ? pt[jj] : dummy;
-            s_pk[threadIdx.x] = jj < n ? pk[jj] : dk;
+        if (threadIdx.x < 128) {
+            const int j = t * 64 + (threadIdx.x & 63);
+            s_pt[threadIdx.x] = ld4(j, j < n);
+            s_pk[threadIdx.x] = ld2(j, j < n);
         }
         __syncthreads();
-        for (int sub = 0; sub < VJ / 64; ++sub) {
-            const int jt = (j0 >> 6) + sub;
-            if (jt >= ntiles) break;
-            if (tol > 0.f) {
-                const float4 c = ts[2 * jt], st = ts[2 * jt + 1];   // c.xyz r | wmin kmin qsum
-                const float d = sqrtf((c.x - cx) * (c.x - cx) + (c.y - cy) * (c.y - cy) +
-                                      (c.z - cz) * (c.z - cz));
-                const float rmin = fmaxf(d - rI - c.w, 0.f);
-                const float rmax = d + rI + c.w;
-                const float gI = rmin * rmin * wminI + kminI;
-                const float gJ = rmin * rmin * st.x + st.y;
-                const float bF = st.z / (gI * gJ * (gI + gJ));
-                const float bU = 2.f * bF / gI;
-                const float bW = bU * rmax * rmax;
-                if (fmaxf(bF, fmaxf(bU, bW)) < tol) continue;       // uniform
-            }
-            const int jend = min(64, n - (j0 + sub * 64));
-            // FP32 partial sums over groups of VG inner points, FP64 across
-            for (int jg = 0; jg < jend; jg += VG) {
-                real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0;
-                const int ge = min(jend, jg + VG);
-                for (int jj = jg; jj < ge; ++jj) {
-                    const VP4<real> p = s_pt[sub * 64 + jj];
-                    const VP2<real> k2 = s_pk[sub * 64 + jj];
-                    {
-                        const real dx = a.x - p.x, dy = a.y - p.y, dz = a.z - p.z;
-                        const real R2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        const real g = fma(R2, a.w, ka.k);
-                        const real gp = fma(R2, p.w, k2.k);
-                        const real gt = g + gp;
-                        const real inv = vv_rcp<real>(g * gp * gt);
-                        const real T = k2.q * inv;
-                        const real Tu = T * (gp * (g + gt)) * inv;
-                        f0 += T; u0 += Tu; w0 = fma(Tu, R2, w0);
+        bool skip = false;
+        if (tol > 0.f) {
+            const float4 tc = ts[2 * t], st = ts[2 * t + 1];   // c.xyz r | wmin kmin qsum
+            const float dd = sqrtf((tc.x - cx) * (tc.x - cx) + (tc.y - cy) * (tc.y - cy) +
+                                   (tc.z - cz) * (tc.z - cz));
+            const float rmin = fmaxf(dd - rI - tc.w, 0.f);
+            const float rmax = dd + rI + tc.w;
+            const float gI = rmin * rmin * wminI + kminI;
+            const float gJ = rmin * rmin * st.x + st.y;
+            const float den = gI * gJ * (gI + gJ);
+            const float bI = st.z / den, bJ = qI / den;
+            const float bound = fmaxf(fmaxf(bI, 2.f * bI / gI * fmaxf(1.f, rmax * rmax)),
+                                      fmaxf(bJ, 2.f * bJ / gJ * fmaxf(1.f, rmax * rmax)));
+            skip = bound < tol;              // uniform per wave
+        }
+        if (own) {
+            if (!skip) {
+#pragma unroll 1
+                for (int m = 0; m < 64 / VS_SUB; ++m) {
+                    real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0, z = 0;
+                    const VP4<real>* tp = s_pt + ((lane + m * VS_SUB) & 63);
+                    const VP2<real>* tk = s_pk + ((lane + m * VS_SUB) & 63);
+#pragma unroll
+                    for (int kk = 0; kk < VS_SUB; ++kk) {
+                        const VP4<real> p = tp[kk];
+                        const VP2<real> kp = tk[kk];
+                        vs_pair<real, false>(a, ka, p, kp, f0, u0, w0, z, z, z);
+                        vs_pair<real, false>(c, kc, p, kp, f1, u1, w1, z, z, z);
                     }
-                    {
-                        const real dx = b.x - p.x, dy = b.y - p.y, dz = b.z - p.z;
-                        const real R2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        const real g = fma(R2, b.w, kb.k);
-                        const real gp = fma(R2, p.w, k2.k);
-                        const real gt = g + gp;
-                        const real inv = vv_rcp<real>(g * gp * gt);
-                        const real T = k2.q * inv;
-                        const real Tu = T * (gp * (g + gt)) * inv;
-                        f1 += T; u1 += Tu; w1 = fma(Tu, R2, w1);
-                    }
+                    F0 += (double)f0; U0 += (double)u0; W0 += (double)w0;
+                    F1 += (double)f1; U1 += (double)u1; W1 += (double)w1;
                 }
+            }
+            continue;
+        }
+        real (*sj)[4][64] = s_j[wave];
+        if (skip) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) { sj[0][m][lane] = 0; sj[1][m][lane] = 0; sj[2][m][lane] = 0; }
+        } else {
+#pragma unroll 1
+            for (int m = 0; m < 64 / VS_SUB; ++m) {
+                real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0;
+                real cf = 0, cu = 0, cw = 0;     // accumulators of point (lane + k) mod 64
+                const VP4<real>* tp = s_pt + ((lane + m * VS_SUB) & 63);
+                const VP2<real>* tk = s_pk + ((lane + m * VS_SUB) & 63);
+#pragma unroll
+                for (int kk = 0; kk < VS_SUB; ++kk) {
+                    const VP4<real> p = tp[kk];
+                    const VP2<real> kp = tk[kk];
+                    vs_pair<real, true>(a, ka, p, kp, f0, u0, w0, cf, cu, cw);
+                    vs_pair<real, true>(c, kc, p, kp, f1, u1, w1, cf, cu, cw);
+                    cf = vs_rot(cf); cu = vs_rot(cu); cw = vs_rot(cw);
+                }
+                // lane l now holds the sums of point (l + 16 (m + 1)) mod 64
+                const int slot = (lane + (m + 1) * VS_SUB) & 63;
+                sj[0][m][slot] = cf; sj[1][m][slot] = cu; sj[2][m][slot] = cw;
                 F0 += (double)f0; U0 += (double)u0; W0 += (double)w0;
                 F1 += (double)f1; U1 += (double)u1; W1 += (double)w1;
             }
         }
+        __syncthreads();
+        if (threadIdx.x < 192) {
+            const int q = threadIdx.x >> 6, slot = threadIdx.x & 63;
+            double acc = 0;
+#pragma unroll
+            for (int wv = 0; wv < 4; ++wv)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acc += (double)s_j[wv][q][m][slot];
+            real* rec = jrec + (((size_t)b * hmax + (d - 1)) * VS_GT + u) * 192;
+            rec[q * 64 + slot] = (real)acc;
+        }
     }
-    if (v0) { Fo[i0] = -1.5 * F0; Uo[i0] = U0; Wo[i0] = W0; }
-    if (v1) { Fo[i1] = -1.5 * F1; Uo[i1] = U1; Wo[i1] = W1; }
+    double* ip = ipart + (size_t)part * 3 * pcap;
+    if (v0) { ip[i0] = F0; ip[pcap + i0] = U0; ip[2 * pcap + i0] = W0; }
+    if (v1) { ip[i1] = F1; ip[pcap + i1] = U1; ip[2 * pcap + i1] = W1; }
 }
 
+// per point: i-side parts in part order, then the records of the groups
+// that visited its tile, by increasing distance d
 template <typename real>
 __global__ __launch_bounds__(256)
-void vv10_kernel(int n, const VP4<real>* __restrict__ pt, const VP2<real>* __restrict__ pk,
-                 const float4* __restrict__ ts, int ntiles, float tol,
-                 double* __restrict__ Fo, double* __restrict__ Uo, double* __restrict__ Wo)
-{
-    vv10_body<real>(n, pt, pk, ts, ntiles, tol, Fo, Uo, Wo);
-}
-
-// Same sum with the point count read from device memory (compacted sets).
-template <typename real>
-__global__ __launch_bounds__(256)
-void vv10_kernel_n(const int* __restrict__ nptr, const VP4<real>* __restrict__ pt,
-                   const VP2<real>* __restrict__ pk, const float4* __restrict__ ts, float tol,
-                   double* __restrict__ Fo, double* __restrict__ Uo, double* __restrict__ Wo)
+void vv10_sym_reduce(const int* __restrict__ nptr, int nsplit, size_t pcap,
+                     const double* __restrict__ ipart, const real* __restrict__ jrec,
+                     double* __restrict__ Fo, double* __restrict__ Uo, double* __restrict__ Wo)
 {
     const int n = *nptr;
-    if (blockIdx.x * 512 >= n) return;
-    vv10_body<real>(n, pt, pk, ts, (n + 63) / 64, tol, Fo, Uo, Wo);
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const int B = vs_groups(n), hmax = vs_hmax(B);
+    const int t = p >> 6, slot = p & 63, g = t / VS_GT, u = t % VS_GT;
+    double F = 0, U = 0, W = 0;
+    for (int s = 0; s < nsplit; ++s) {
+        const double* ip = ipart + (size_t)s * 3 * pcap;
+        F += ip[p]; U += ip[pcap + p]; W += ip[2 * pcap + p];
+    }
+    for (int d = 1; d <= hmax; ++d) {
+        const int b = (g - d + B) % B;
+        if (d > vs_npartner(B, b)) continue;
+        const real* rec = jrec + (((size_t)b * hmax + (d - 1)) * VS_GT + u) * 192;
+        F += (double)rec[slot]; U += (double)rec[64 + slot]; W += (double)rec[128 + slot];
+    }
+    Fo[p] = -1.5 * F; Uo[p] = U; Wo[p] = W;
 }
 
-void launch_vv10_sum(hipStream_t s, int n, const void* pts, const void* pk, const float* tiles,
-                     int ntiles, float tol, bool f64, double* F, double* U, double* W)
+// workspace bytes of vv10_sym for up to ncap points
+size_t vv10_sym_bytes(int ncap, bool f64, int* nsplit_out)
 {
-    if (n == 0) return;
-    const int blocks = (n + 511) / 512;
-    if (f64)
-        hipLaunchKernelGGL(vv10_kernel<double>, dim3(blocks), dim3(256), 0, s, n,
-                           (const VP4<double>*)pts, (const VP2<double>*)pk,
-                           (const float4*)tiles, ntiles, tol, F, U, W);
-    else
-        hipLaunchKernelGGL(vv10_kernel<float>, dim3(blocks), dim3(256), 0, s, n,
-                           (const VP4<float>*)pts, (const VP2<float>*)pk,
-                           (const float4*)tiles, ntiles, tol, F, U, W);
+    const int Bm = vs_groups(ncap);
+    const int nsplit = std::max(1, std::min(16, (2048 + Bm - 1) / std::max(Bm, 1)));
+    if (nsplit_out) *nsplit_out = nsplit;
+    const size_t rs = f64 ? 8 : 4;
+    const size_t pcap = (size_t)Bm * VS_IG;
+    const size_t nrec = (size_t)Bm * std::max(vs_hmax(Bm), 1) * VS_GT;
+    return (sizeof(double) * 3 * pcap * nsplit + 255) / 256 * 256 + rs * 192 * nrec;
+}
+
+// symmetric sum for n = *nptr <= ncap compacted points; ws holds
+// vv10_sym_bytes(ncap) bytes
+void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, const void* pk,
+                     const float4* tiles, float tol, bool f64, void* ws, double* F, double* U,
+                     double* W)
+{
+    int nsplit = 1;
+    vv10_sym_bytes(ncap, f64, &nsplit);
+    const int Bm = vs_groups(ncap);
+    const size_t pcap = (size_t)Bm * VS_IG;
+    double* ipart = (double*)ws;
+    void* jrec = (char*)ws + (sizeof(double) * 3 * pcap * nsplit + 255) / 256 * 256;
+    const dim3 grid(Bm * nsplit), red((ncap + 255) / 256);
+    if (f64) {
+        hipLaunchKernelGGL(vv10_sym_kernel<double>, grid, dim3(256), 0, s, nptr,
+                           (const VP4<double>*)pts, (const VP2<double>*)pk, tiles, tol, nsplit,
+                           pcap, ipart, (double*)jrec);
+        hipLaunchKernelGGL(vv10_sym_reduce<double>, red, dim3(256), 0, s, nptr, nsplit, pcap,
+                           ipart, (const double*)jrec, F, U, W);
+    } else {
+        hipLaunchKernelGGL(vv10_sym_kernel<float>, grid, dim3(256), 0, s, nptr,
+                           (const VP4<float>*)pts, (const VP2<float>*)pk, tiles, tol, nsplit,
+                           pcap, ipart, (float*)jrec);
+        hipLaunchKernelGGL(vv10_sym_reduce<float>, red, dim3(256), 0, s, nptr, nsplit, pcap,
+                           ipart, (const float*)jrec, F, U, W);
+    }
     WB_HIP(hipGetLastError());
 }
 
@@ -1011,6 +1153,7 @@ void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, co
     const size_t o_ts = take(sizeof(float4) * 2 * ntile_max);
     const size_t o_F = take(sizeof(double) * Ppad), o_U = take(sizeof(double) * Ppad),
                  o_W = take(sizeof(double) * Ppad);
+    const size_t o_sym = take(vv10_sym_bytes(Ppad, f64, nullptr));
     char* base = (char*)ws.ensure(o);
     int* cnt = (int*)(base + o_cnt);
     int* off = (int*)(base + o_off);
@@ -1033,8 +1176,7 @@ void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, co
                            idx, aux);
         hipLaunchKernelGGL(vv_tile_kernel<double>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
                            nptr, pt, pk, (int)ntile_max, ts);
-        hipLaunchKernelGGL(vv10_kernel_n<double>, dim3((Ppad + 511) / 512), dim3(256), 0, s,
-                           nptr, pt, pk, ts, (float)tol, F, U, W);
+        launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, true, base + o_sym, F, U, W);
     } else {
         auto* pt = (VP4<float>*)(base + o_pt);
         auto* pk = (VP2<float>*)(base + o_pk);
@@ -1043,8 +1185,7 @@ void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, co
                            idx, aux);
         hipLaunchKernelGGL(vv_tile_kernel<float>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
                            nptr, pt, pk, (int)ntile_max, ts);
-        hipLaunchKernelGGL(vv10_kernel_n<float>, dim3((Ppad + 511) / 512), dim3(256), 0, s,
-                           nptr, pt, pk, ts, (float)tol, F, U, W);
+        launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, false, base + o_sym, F, U, W);
     }
     (void)pi;
     if (prof) prof->mark(PH_VV10);
