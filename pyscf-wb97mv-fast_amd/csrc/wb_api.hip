@@ -7,6 +7,7 @@
 //   NLC:      X, dX = AO(grid) -> T = X D -> rho, grad rho -> VV10 pointwise,
 //             compaction, double sum -> potential -> vmat   (gpu/vv10.py:268-342)
 #include <algorithm>
+#include <tuple>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -254,6 +255,50 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         C.npairs++;
     }
     B.npairs_total = total;
+    // B blocks and pair groups per class
+    for (int k = 0; k < NCLASS; ++k) {
+        ClassData& C = B.cls[k];
+        if (!C.npairs) continue;
+        const int bb = md_bb(C.la, C.lb);
+        std::vector<int> rank(nshell, -1);
+        int nb = 0;
+        for (int i = 0; i < nshell; ++i)
+            if (l[i] == C.lb) rank[i] = nb++;
+        const int nblk = (nb + bb - 1) / bb;
+        C.bbao.assign((size_t)nblk * bb, -1);
+        for (int i = 0; i < nshell; ++i)
+            if (rank[i] >= 0) C.bbao[rank[i]] = B.shells[i].ao0;
+        std::vector<int> ord(C.npairs);
+        for (int i = 0; i < C.npairs; ++i) ord[i] = i;
+        auto key = [&](int i) {
+            const PairRec& pr = C.pairs[i];
+            return std::make_tuple(rank[pr.shB] / bb, pr.shA, pr.shB);
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key(x) < key(y); });
+        std::vector<PairRec> pairs(C.npairs);
+        std::vector<PairScreen> scr(C.npairs);
+        C.groups.clear();
+        for (int i = 0; i < C.npairs; ++i) {
+            pairs[i] = C.pairs[ord[i]];
+            scr[i] = C.scr[ord[i]];
+            const int blk = rank[pairs[i].shB] / bb, pos = rank[pairs[i].shB] % bb;
+            if (C.groups.empty() || C.groups.back().bblk != blk ||
+                C.groups.back().shA != pairs[i].shA) {
+                PairGroup gr{};
+                gr.shA = pairs[i].shA;
+                gr.a0 = pairs[i].a0;
+                gr.bblk = blk;
+                for (int j = 0; j < MD_BBMAX; ++j) gr.pidx[j] = -1;
+                C.groups.push_back(gr);
+            }
+            C.groups.back().pidx[pos] = i;
+            C.groups.back().npair++;
+        }
+        C.pairs.swap(pairs);
+        C.scr.swap(scr);
+        C.ngroups = (int)C.groups.size();
+        if (C.ngroups >= (1 << 23)) throw Error(WB_UNSUPPORTED, "too many shell-pair groups");
+    }
 }
 
 // synchronous upload (setup paths; the host buffer may die on return)
@@ -281,6 +326,8 @@ static void upload_basis(Basis& B, hipStream_t s)
         if (!C.npairs) continue;
         upload(C.d_pairs, C.pairs.data(), sizeof(PairRec) * C.npairs, s);
         upload(C.d_scr, C.scr.data(), sizeof(PairScreen) * C.npairs, s);
+        upload(C.d_groups, C.groups.data(), sizeof(PairGroup) * C.ngroups, s);
+        upload(C.d_bbao, C.bbao.data(), sizeof(int) * C.bbao.size(), s);
         upload(C.d_P, C.P.data(), sizeof(double) * C.P.size(), s);
         // per primitive pair: the erl leading FP32 values, then their remainders
         std::vector<float> e32(2 * C.E.size());
@@ -453,20 +500,23 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     const int* d_sh = B.d_shao.as<int>();
     float* fmax = (float*)c.ws_fmax.ensure(sizeof(float) * (size_t)G.nsub * B.nshell);
     launch_fmax(s, F, f32, G.d_w.as<double>(), nao, Ppad, G.nsub, d_sh, d_sh + B.nshell, B.nshell, fmax);
-    size_t ntask = 0;
-    for (auto& C : B.cls) ntask += (size_t)C.npairs * G.nsub;
+    size_t ntask = 0, nitem = 0;
+    for (auto& C : B.cls) {
+        ntask += (size_t)C.npairs * G.nsub;
+        nitem += (size_t)C.ngroups * G.nsub;
+    }
     int* counts = (int*)c.ws_counts.ensure(sizeof(int) * (size_t)NCLASS * G.nsub);
     zero_bytes(s, counts, sizeof(int) * (size_t)NCLASS * G.nsub);
     auto* work = (unsigned long long*)c.ws_work.ensure(sizeof(unsigned long long) * 2 * NCLASS);
     zero_bytes(s, work, sizeof(unsigned long long) * 2 * NCLASS);
-    int* ids = (int*)c.ws_tasks.ensure(sizeof(int) * std::max<size_t>(ntask, 1));
+    int* ids = (int*)c.ws_tasks.ensure(sizeof(int) * std::max<size_t>(nitem, 1));
     size_t toff = 0;
     std::vector<size_t> cls_off(NCLASS);
     for (int k = 0; k < NCLASS; ++k) {
         cls_off[k] = toff;
         launch_tasks(s, B.cls[k], G, fmax, B.nshell, (float)tol, counts + (size_t)k * G.nsub,
                      ids + toff, work + 2 * k);
-        toff += (size_t)B.cls[k].npairs * G.nsub;
+        toff += (size_t)B.cls[k].ngroups * G.nsub;
     }
     c.tasks_total = (long long)ntask;
     const int ncopy = c.ncopy;
@@ -493,13 +543,15 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     collect(c);
 }
 
-static long long count_kept(Ctx& c, Grid& G)
+// kept tasks (sub-block x shell pair) of the last screening, all classes
+static long long count_kept(Ctx& c)
 {
-    std::vector<int> h((size_t)NCLASS * G.nsub);
-    WB_HIP(hipMemcpyAsync(h.data(), c.ws_counts.p, sizeof(int) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    unsigned long long h[2 * NCLASS] = {0};
+    if (!c.ws_work.p) return 0;
+    WB_HIP(hipMemcpyAsync(h, c.ws_work.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
     WB_HIP(hipStreamSynchronize(c.stream));
     long long s = 0;
-    for (int v : h) s += v;
+    for (int k = 0; k < NCLASS; ++k) s += (long long)h[2 * k];
     return s;
 }
 
@@ -716,7 +768,8 @@ int wb_k_stats(wb_ctx_t ctx, wb_grid_t grid, long long* out2)
     WB_TRY({
         auto* c = (Ctx*)ctx;
         out2[0] = c->tasks_total;
-        out2[1] = count_kept(*c, *(Grid*)grid);
+        out2[1] = count_kept(*c);
+        (void)grid;
     })
 }
 

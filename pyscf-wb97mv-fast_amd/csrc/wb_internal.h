@@ -19,6 +19,20 @@ struct PairRec {              // one shell pair of a class, la >= lb
     int diag, onec;           // diag: A == B; onec: A and B on one centre
 };
 
+// B blocks: the B shells of a class (shells of angular momentum lb, in
+// shell order) are cut into blocks of md_bb(la, lb); an exchange kernel keeps
+// one block's F values and G accumulators in registers while it runs
+// through the A shells paired with that block.
+constexpr int MD_BBMAX = 8;
+__host__ __device__ constexpr int md_bb(int la, int lb) {
+    return lb == 0 ? 8 : (lb == 1 ? (la == 1 ? 8 : 4) : (lb == 2 ? 2 : 1));
+}
+
+struct PairGroup {            // the pairs (A, B) of one shell A and one B block
+    int shA, a0, bblk, npair;
+    int pidx[MD_BBMAX];       // pair index for B at block position j, -1 if none
+};
+
 struct PairScreen {           // screening data per pair (pair order of its class)
     float cx, cy, cz;         // centre of the most diffuse primitive pair
     float delta;              // max distance of any primitive centre from (cx,cy,cz)
@@ -36,7 +50,11 @@ struct ClassData {
     std::vector<double> pref;         // (nprims,) c_a c_b K_AB 2 pi / p
     std::vector<double> E;            // (nprims, erl) Hermite E records
     int erl = 0;                      // E record length
-    DevBuf d_pairs, d_scr, d_P, d_E32, d_E64;
+    // pairs are ordered by (B block, A, B); groups: runs of one (B block, A)
+    std::vector<PairGroup> groups;
+    std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, -1 none
+    int ngroups = 0;
+    DevBuf d_pairs, d_scr, d_P, d_E32, d_E64, d_groups, d_bbao;
     std::map<double, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;  // omega -> (C32, C64)
 };
 

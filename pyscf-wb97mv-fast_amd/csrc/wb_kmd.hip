@@ -2,16 +2,18 @@
 //
 // For one shell-pair class (la >= lb) and one operator (1/r or erf(w r)/r),
 // a wave owns one sub-block of 64 grid points (one point per lane) and one
-// chunk of that sub-block's screened task list.  Per task (shell pair A, B)
+// chunk of that sub-block's screened item list.  Per task (shell pair A, B)
 // it forms the Cartesian block
 //     V_ab(g) = sum_prims sum_tuv Ex Ey Ez R_tuv(p, P - g)
 // in registers (never stored), and contracts it with F (F = X Q D):
 //     G[g, B] += A(g)^T F[g, A],   G[g, A] += A(g) F[g, B]  (A != B)
-// Tasks are sorted by A, so G[A] accumulates in registers along a run of
-// tasks with the same A and is written once per run.  Each chunk writes its
-// own copy of G (no atomics, fixed order, bit-reproducible); the copies are
-// summed inside the K GEMM.  The reference does the same contraction with
-// one warp per 32-point sub-block (gpu/sgx_k.py:335-400).
+// An item is one A shell with the kept pairs of one B block (a bit mask);
+// items are ordered by B block, so the block's F values and G accumulators
+// stay in registers along a run of items and G[B] is written once per run,
+// G[A] once per item.  Each chunk writes its own copy of G (no atomics,
+// fixed order, bit-reproducible); the copies are summed inside the K GEMM.
+// The reference does the same contraction with one warp per 32-point
+// sub-block and one task per shell pair (gpu/sgx_k.py:335-400).
 #include "wb_boys.h"
 #include "wb_internal.h"
 #include <type_traits>
@@ -92,6 +94,7 @@ __global__ __launch_bounds__(64 * MD_WAVES)
 void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  gacc* __restrict__ G, size_t gstride,
                  const int* __restrict__ counts, const int* __restrict__ ids, int cap,
+                 const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
                  const PairRec* __restrict__ pairs, const double* __restrict__ ppP,
                  const real* __restrict__ ppE, const real* __restrict__ ppC,
                  const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk)
@@ -99,6 +102,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     constexpr int L = LA + LB;
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
     constexpr int NSA = 2 * LA + 1, NSB = 2 * LB + 1;
+    constexpr int BB = md_bb(LA, LB);
     constexpr bool F32 = sizeof(real) == 4;
     // the primitive sum of the Cartesian block is FP64 when it fits
     // (<= 36 entries); the larger f-shell blocks belong to single-primitive
@@ -128,69 +132,93 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     const int g = s * SUB + lane;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
 
-    int curA = -1;
-    real FaC[NCA];
-    gacc acc[NCA];
-    int a0 = 0;
-    for (int t = t0; t < t1; ++t) {
-        const int k = tl[t];
-        const PairRec pr = pairs[k];
-        if (pr.shA != curA) {
-            if (curA >= 0) {
-                gacc gs[NSA];
-                gsph<LA, gacc>(acc, gs);
+    // the current B block: F (Cartesian) and G accumulators of its shells
+    int curB = -1;
+    unsigned touched = 0;
+    real FbC[BB][NCB];
+    gacc accB[BB][NCB];
+    auto flush_b = [&]() {
 #pragma unroll
-                for (int m = 0; m < NSA; ++m) Gc[(size_t)(a0 + m) * Ppad + g] += gs[m];
+        for (int q = 0; q < BB; ++q) {
+            if (touched >> q & 1u) {
+                const int b0 = bbao[curB * BB + q];
+                gacc gs[NSB];
+                gsph<LB, gacc>(accB[q], gs);
+#pragma unroll
+                for (int m = 0; m < NSB; ++m) Gc[(size_t)(b0 + m) * Ppad + g] += gs[m];
             }
-            curA = pr.shA;
-            a0 = pr.a0;
-            real fa[NSA];
-#pragma unroll
-            for (int m = 0; m < NSA; ++m) fa[m] = F[(size_t)(a0 + m) * Ppad + g];
-            fcart<LA, real>(fa, FaC);
-#pragma unroll
-            for (int c = 0; c < NCA; ++c) acc[c] = (gacc)0;
         }
-        vacc Vs[NCA * NCB];
+    };
+    for (int t = t0; t < t1; ++t) {
+        const int item = tl[t];
+        const PairGroup* gr = groups + (item >> 8);
+        const unsigned mask = (unsigned)item & 0xffu;
+        const int bblk = gr->bblk;
+        if (bblk != curB) {
+            if (curB >= 0) flush_b();
+            curB = bblk;
+            touched = 0;
 #pragma unroll
-        for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
-        // one-centre pairs (uniform across the wave) take the variant whose
-        // FP32 E products all carry their rounding remainders
-        if (pr.onec) md_block<LA, LB, true, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
-        else md_block<LA, LB, false, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
-        real V[NCA * NCB];
+            for (int q = 0; q < BB; ++q) {
+                const int b0 = bbao[curB * BB + q];
+                real fb[NSB];
 #pragma unroll
-        for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];
-        real fb[NSB], FbC[NCB];
+                for (int m = 0; m < NSB; ++m) fb[m] = b0 >= 0 ? F[(size_t)(b0 + m) * Ppad + g] : (real)0;
+                fcart<LB, real>(fb, FbC[q]);
 #pragma unroll
-        for (int m = 0; m < NSB; ++m) fb[m] = F[(size_t)(pr.b0 + m) * Ppad + g];
-        fcart<LB, real>(fb, FbC);
-        real gbc[NCB];
-#pragma unroll
-        for (int b = 0; b < NCB; ++b) {
-            real v = (real)0;
-#pragma unroll
-            for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + b], FaC[a], v);
-            gbc[b] = v;
+                for (int c = 0; c < NCB; ++c) accB[q][c] = (gacc)0;
+            }
         }
-        real gbs[NSB];
-        gsph<LB, real>(gbc, gbs);
+        const int a0 = gr->a0;
+        real fa[NSA], FaC[NCA];
 #pragma unroll
-        for (int m = 0; m < NSB; ++m) Gc[(size_t)(pr.b0 + m) * Ppad + g] += (gacc)gbs[m];
-        if (!pr.diag) {
+        for (int m = 0; m < NSA; ++m) fa[m] = F[(size_t)(a0 + m) * Ppad + g];
+        fcart<LA, real>(fa, FaC);
+        gacc accA[NCA];
 #pragma unroll
-            for (int a = 0; a < NCA; ++a) {
+        for (int c = 0; c < NCA; ++c) accA[c] = (gacc)0;
+        bool anyA = false;
+#pragma unroll
+        for (int q = 0; q < BB; ++q) {
+            if (!(mask >> q & 1u)) continue;                 // uniform
+            const PairRec pr = pairs[gr->pidx[q]];
+            vacc Vs[NCA * NCB];
+#pragma unroll
+            for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
+            // one-centre pairs take the variant whose FP32 E products all
+            // carry their rounding remainders
+            if (pr.onec) md_block<LA, LB, true, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
+            else md_block<LA, LB, false, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
+            real V[NCA * NCB];
+#pragma unroll
+            for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];
+#pragma unroll
+            for (int b = 0; b < NCB; ++b) {
                 real v = (real)0;
 #pragma unroll
-                for (int b = 0; b < NCB; ++b) v = fma(V[a * NCB + b], FbC[b], v);
-                acc[a] += (gacc)v;
+                for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + b], FaC[a], v);
+                accB[q][b] += (gacc)v;
+            }
+            if (!pr.diag) {                                  // uniform
+                anyA = true;
+#pragma unroll
+                for (int a = 0; a < NCA; ++a) {
+                    real v = (real)0;
+#pragma unroll
+                    for (int b = 0; b < NCB; ++b) v = fma(V[a * NCB + b], FbC[q][b], v);
+                    accA[a] += (gacc)v;
+                }
             }
         }
-    }
-    gacc gs[NSA];
-    gsph<LA, gacc>(acc, gs);
+        touched |= mask;
+        if (anyA) {
+            gacc gs[NSA];
+            gsph<LA, gacc>(accA, gs);
 #pragma unroll
-    for (int m = 0; m < NSA; ++m) Gc[(size_t)(a0 + m) * Ppad + g] += gs[m];
+            for (int m = 0; m < NSA; ++m) Gc[(size_t)(a0 + m) * Ppad + g] += gs[m];
+        }
+    }
+    if (curB >= 0) flush_b();
 }
 
 template <int LA, int LB, typename real, typename gacc>
@@ -203,7 +231,8 @@ static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const 
     const real* E = (sizeof(real) == 4) ? c.d_E32.as<real>() : c.d_E64.as<real>();
     hipLaunchKernelGGL((md_k_kernel<LA, LB, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES), 0, s,
                        g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, counts, ids,
-                       c.npairs, c.d_pairs.as<PairRec>(), c.d_P.as<double>(), E,
+                       c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
+                       c.d_pairs.as<PairRec>(), c.d_P.as<double>(), E,
                        (const real*)C, (const real*)boys, g.Ppad, g.nsub, ncopy);
     WB_HIP(hipGetLastError());
 }
@@ -309,7 +338,8 @@ void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int na
 // the totals do not depend on the order)
 __global__ __launch_bounds__(256)
 void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__ pairs,
-                 int npairs, const float4* __restrict__ sub, int nsub,
+                 const PairGroup* __restrict__ groups, int ngroups, int bb,
+                 const float4* __restrict__ sub, int nsub,
                  const float* __restrict__ fmax, int nshell, float tol,
                  int* __restrict__ counts, int* __restrict__ ids,
                  unsigned long long* __restrict__ work)
@@ -320,41 +350,52 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
     const int lane = threadIdx.x & 63;
     const float4 sb = sub[s];
     const float* fm = fmax + (size_t)s * nshell;
-    int* out = ids + (size_t)s * npairs;
+    int* out = ids + (size_t)s * ngroups;
     int n = 0;
-    unsigned long long npp = 0;
-    for (int k0 = 0; k0 < npairs; k0 += 64) {
+    unsigned long long ntask = 0, npp = 0;
+    for (int k0 = 0; k0 < ngroups; k0 += 64) {
         const int k = k0 + lane;
-        bool keep = false;
-        if (k < npairs) {
-            if (tol <= 0.f) {
-                keep = true;
-            } else {
-                const PairScreen ps = scr[k];
-                const PairRec pr = pairs[k];
-                const float dx = ps.cx - sb.x, dy = ps.cy - sb.y, dz = ps.cz - sb.z;
-                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-                float d = dist - sb.w - ps.delta;
-                d -= 1e-5f * (dist + sb.w + ps.delta) + 1e-6f;   // FP32 safety
-                float f0 = 1.0f;
-                if (d > 0.f) f0 = fminf(1.0f, 0.8862269254527580f / (sqrtf(ps.q) * d));
-                const float est = ps.beta * f0 * fmaxf(fm[pr.shA], fm[pr.shB]);
-                keep = est * 1.001f >= tol;
+        unsigned mask = 0;
+        if (k < ngroups) {
+            const PairGroup gr = groups[k];
+#pragma unroll
+            for (int q = 0; q < MD_BBMAX; ++q) {
+                const int p = gr.pidx[q];
+                if (q >= bb || p < 0) continue;
+                bool keep = true;
+                if (tol > 0.f) {
+                    const PairScreen ps = scr[p];
+                    const PairRec pr = pairs[p];
+                    const float dx = ps.cx - sb.x, dy = ps.cy - sb.y, dz = ps.cz - sb.z;
+                    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                    float d = dist - sb.w - ps.delta;
+                    d -= 1e-5f * (dist + sb.w + ps.delta) + 1e-6f;   // FP32 safety
+                    float f0 = 1.0f;
+                    if (d > 0.f) f0 = fminf(1.0f, 0.8862269254527580f / (sqrtf(ps.q) * d));
+                    const float est = ps.beta * f0 * fmaxf(fm[pr.shA], fm[pr.shB]);
+                    keep = est * 1.001f >= tol;
+                }
+                if (keep) {
+                    mask |= 1u << q;
+                    npp += (unsigned long long)pairs[p].npp;
+                }
             }
         }
+        ntask += (unsigned long long)__popc(mask);
+        const bool keep = mask != 0;
         const unsigned long long bal = __ballot(keep);
         const int pos = n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-        if (keep) {
-            out[pos] = k;
-            npp += (unsigned long long)pairs[k].npp;
-        }
+        if (keep) out[pos] = (k << 8) | (int)mask;
         n += __popcll(bal);
     }
-    for (int o = 32; o > 0; o >>= 1) npp += __shfl_xor(npp, o, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+        npp += __shfl_xor(npp, o, 64);
+        ntask += __shfl_xor(ntask, o, 64);
+    }
     if (lane == 0) {
         counts[s] = n;
-        atomicAdd(work, (unsigned long long)n);
+        atomicAdd(work, ntask);
         atomicAdd(work + 1, npp);
     }
 }
@@ -365,8 +406,9 @@ void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float*
     if (c.npairs == 0) return;
     const int blocks = (g.nsub + 3) / 4;
     hipLaunchKernelGGL(task_kernel, dim3(blocks), dim3(256), 0, s, c.d_scr.as<PairScreen>(),
-                       c.d_pairs.as<PairRec>(), c.npairs, g.d_sub.as<float4>(), g.nsub, fmax,
-                       nshell, tol, counts, ids, work);
+                       c.d_pairs.as<PairRec>(), c.d_groups.as<PairGroup>(), c.ngroups,
+                       md_bb(c.la, c.lb), g.d_sub.as<float4>(), g.nsub, fmax, nshell, tol,
+                       counts, ids, work);
     WB_HIP(hipGetLastError());
 }
 
