@@ -26,6 +26,11 @@
 extern "C" {
 #endif
 
+/* the library is built with hidden visibility: these are its only exports */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
 #define WB_ABI_VERSION 1
 
 /* status codes; the reference raises gpu.xc.Unsupported (gpu/xc.py:30-32)
@@ -52,7 +57,7 @@ int wb_version(void);
  * stream (the reference's per-flow streams, gpu/install.py:118-135). */
 int wb_ctx_create(int device, void* stream, wb_ctx_t* out);
 int wb_ctx_destroy(wb_ctx_t ctx);
-/* number of G partial copies of the exchange kernel (default 4) */
+/* number of G partial copies of the exchange kernel (default 8) */
 int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
 /* mixed form: accumulate G in FP64 (1) or FP32 (0, default) */
 int wb_ctx_set_g64(wb_ctx_t ctx, int on);
@@ -139,6 +144,10 @@ int wb_boys_eval(wb_ctx_t ctx, int n, const double* T, int nmax, int precision, 
  * tile kernels. */
 int wb_test_gemm(wb_ctx_t ctx, int kind, int M, int N, int K, const double* A, const double* B,
                  const double* w, double* C);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, 'build')
 OUT = os.path.join(HERE, 'pyscf_wb97mv_fast_amd', 'libwb97mv_amd.so')
 UNITS = ['wb_api', 'wb_kmd', 'wb_kmisc']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
-FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
+FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-fvisibility=hidden',
          # no packed FP32 instructions anywhere (correctness policy)
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-I', CSRC, '-I', os.path.join(BUILD, 'gen'), '-I', os.path.join(ROOT, 'include')]
