@@ -265,7 +265,9 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         for (int i = 0; i < nshell; ++i)
             if (l[i] == C.lb) rank[i] = nb++;
         const int nblk = (nb + bb - 1) / bb;
-        C.bbao.assign((size_t)nblk * bb, -1);
+        // absent positions of the last block point at the spare rows after
+        // the nao AO rows of F and G (MD_SPARE of them; never read back)
+        C.bbao.assign((size_t)nblk * bb, B.nao);
         for (int i = 0; i < nshell; ++i)
             if (rank[i] >= 0) C.bbao[rank[i]] = B.shells[i].ao0;
         std::vector<int> ord(C.npairs);
@@ -493,7 +495,9 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     c.mark(PH_AO);
     double* QDT = (double*)c.ws_D.ensure(sizeof(double) * nao * (size_t)nao);
     transpose(s, nao, d_QD, QDT);
-    void* F = c.ws_F.ensure(rsz * nao * (size_t)Ppad);
+    // F and G carry MD_SPARE spare rows after the AO rows for the padding
+    // positions of the exchange kernels' B blocks
+    void* F = c.ws_F.ensure(rsz * (nao + MD_SPARE) * (size_t)Ppad);
     gemm_nn_fast(s, nao, Ppad, nao, QDT, X, F, f32);
     c.mark(PH_FGEMM);
     // screening and task lists
@@ -520,7 +524,7 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     }
     c.tasks_total = (long long)ntask;
     const int ncopy = c.ncopy;
-    const size_t gstride = (size_t)nao * Ppad;
+    const size_t gstride = (size_t)(nao + MD_SPARE) * Ppad;
     const bool g64 = !f32 || c.g64;
     const size_t gsz = g64 ? 8 : 4;
     void* Gb = c.ws_G.ensure(gsz * gstride * ncopy);

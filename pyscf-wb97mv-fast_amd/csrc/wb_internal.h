@@ -24,6 +24,9 @@ struct PairRec {              // one shell pair of a class, la >= lb
 // one block's F values and G accumulators in registers while it runs
 // through the A shells paired with that block.
 constexpr int MD_BBMAX = 8;
+// spare rows after the nao AO rows of F and G: the padding positions of the
+// last B block of a class point at row nao and touch rows nao .. nao + 2lb
+constexpr int MD_SPARE = 7;
 __host__ __device__ constexpr int md_bb(int la, int lb) {
     return lb == 0 ? 8 : (lb == 1 ? (la == 1 ? 8 : 4) : (lb == 2 ? 2 : 1));
 }
@@ -52,7 +55,7 @@ struct ClassData {
     int erl = 0;                      // E record length
     // pairs are ordered by (B block, A, B); groups: runs of one (B block, A)
     std::vector<PairGroup> groups;
-    std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, -1 none
+    std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, nao (spare rows) if none
     int ngroups = 0;
     DevBuf d_pairs, d_scr, d_P, d_E32, d_E64, d_groups, d_bbao;
     std::map<double, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;  // omega -> (C32, C64)

@@ -134,10 +134,13 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     gacc* Gc = G + (size_t)j * gstride;
     const int g = s * SUB + lane;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
+    // wait for the point's coordinates here, once: otherwise the first use
+    // inside the item loop waits on the vector-memory counter in program
+    // order, i.e. also on the next item's prefetched F and G rows
+    __builtin_amdgcn_s_waitcnt(0);
 
     // the current B block: F (Cartesian) and G accumulators of its shells
     int curB = -1;
-    unsigned touched = 0;
     real FbC[BB][NCB];
     gacc accB[BB][NCB];
     // G[B] += the block's accumulators: all row loads first, then the stores
@@ -145,20 +148,20 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
         int bo[BB];
 #pragma unroll
         for (int q = 0; q < BB; ++q) bo[q] = bbao[curB * BB + q];
+        // every position is loaded and stored, untouched ones unchanged
+        // (absent positions use the spare row): branch-free, so the loads
+        // issue as one batch and no store waits for more than its own row
         gacc old[BB][NSB];
 #pragma unroll
         for (int q = 0; q < BB; ++q)
-            if (touched >> q & 1u)
 #pragma unroll
-                for (int m = 0; m < NSB; ++m) old[q][m] = Gc[(size_t)(bo[q] + m) * Ppad + g];
+            for (int m = 0; m < NSB; ++m) old[q][m] = Gc[(size_t)(bo[q] + m) * Ppad + g];
 #pragma unroll
         for (int q = 0; q < BB; ++q) {
-            if (touched >> q & 1u) {
-                gacc gs[NSB];
-                gsph<LB, gacc>(accB[q], gs);
+            gacc gs[NSB];
+            gsph<LB, gacc>(accB[q], gs);
 #pragma unroll
-                for (int m = 0; m < NSB; ++m) Gc[(size_t)(bo[q] + m) * Ppad + g] = old[q][m] + gs[m];
-            }
+            for (int m = 0; m < NSB; ++m) Gc[(size_t)(bo[q] + m) * Ppad + g] = old[q][m] + gs[m];
         }
     };
     // software pipeline: while an item computes, the next item's F_A rows
@@ -201,9 +204,8 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
         if (bblk != curB) {
             if (curB >= 0) flush_b();
             curB = bblk;
-            touched = 0;
-            // the block's F rows: addresses clamped (absent positions read
-            // row 0 and are zeroed), so all loads issue back to back
+            // the block's F rows, all loads back to back (absent positions
+            // read the spare row; their values are never used: no mask bit)
             int bo[BB];
 #pragma unroll
             for (int q = 0; q < BB; ++q) bo[q] = bbao[curB * BB + q];
@@ -211,11 +213,9 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 #pragma unroll
             for (int q = 0; q < BB; ++q)
 #pragma unroll
-                for (int m = 0; m < NSB; ++m) fb[q][m] = F[(size_t)(max(bo[q], 0) + m) * Ppad + g];
+                for (int m = 0; m < NSB; ++m) fb[q][m] = F[(size_t)(bo[q] + m) * Ppad + g];
 #pragma unroll
             for (int q = 0; q < BB; ++q) {
-#pragma unroll
-                for (int m = 0; m < NSB; ++m) fb[q][m] = bo[q] >= 0 ? fb[q][m] : (real)0;
                 fcart<LB, real>(fb[q], FbC[q]);
 #pragma unroll
                 for (int c = 0; c < NCB; ++c) accB[q][c] = (gacc)0;
@@ -262,7 +262,6 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                 }
             }
         }
-        touched |= mask;
         if (anyA) {
             gacc gs[NSA];
             gsph<LA, gacc>(accA, gs);

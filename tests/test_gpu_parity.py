@@ -97,6 +97,43 @@ def test_k_two_operators_one_call_and_repeatable(ctx):
     assert numpy.array_equal(both[1], kb.k_raw(g, dm, [0.3])[0])
 
 
+def test_k_block_padding_and_copies(ctx):
+    """(H2O)5 in def2-SVP has 35 s, 20 p and 5 d shells: the last B block of
+    the ss/sp/ds (8), pp (8) and dd (2) classes is partly empty.  K in FP64
+    matches the oracle for 1, 3 and 8 G copies, and the copy count changes
+    only the summation order."""
+    mol, g, dm = _system(atoms=geometry.water_cluster(5, seed=3), level=0, npts=1800)
+    ref = oracle.k_build(oracle.OracleBasis.from_mol(mol), g.coords, g.weights, dm, 0.0,
+                         pair_cut=1e-14)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
+    Ks = []
+    try:
+        for nc in (1, 3, 8):
+            lib.check(lib.load().wb_ctx_set_copies(ctx.h, nc))
+            Ks.append(kb.k_raw(g, dm, [0.0])[0])
+    finally:
+        lib.check(lib.load().wb_ctx_set_copies(ctx.h, 8))
+    for K in Ks:
+        assert numpy.allclose(K, ref, rtol=0, atol=1e-10 * numpy.abs(ref).max())
+        assert numpy.allclose(K, Ks[0], rtol=0, atol=1e-13 * numpy.abs(ref).max())
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_k_drug_like_tzvp(ctx, precision):
+    """A small seeded drug-like chain (C/N/O backbone with F, Cl, S-CH3
+    substituents) in def2-TZVP: every class up to (f, f), both operators."""
+    mol, g, dm = _system('def2-tzvp', atoms=geometry.drug_like(10, seed=2), level=0, npts=600)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision=precision)
+    Kb = kb.k_raw(g, dm, [0.0, 0.3])
+    ob = oracle.OracleBasis.from_mol(mol)
+    for K, omega in zip(Kb, (0.0, 0.3)):
+        ref = oracle.k_build(ob, g.coords, g.weights, dm, omega, pair_cut=1e-14)
+        err = numpy.abs(K - ref).max() / numpy.abs(ref).max()
+        print('drug-like tzvp %s omega=%.1f nao %d max|dK|/max|K| %.2e' % (precision, omega,
+                                                                      mol.nao, err))
+        assert err < (1e-10 if precision == 'fp64' else 1e-6)
+
+
 def test_k_screening(ctx):
     mol, g, dm = _system(atoms=geometry.water_cluster(8, seed=1), level=0, npts=6000)
     kb0 = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
