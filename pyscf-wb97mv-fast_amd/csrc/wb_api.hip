@@ -255,26 +255,30 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         C.npairs++;
     }
     B.npairs_total = total;
-    // B blocks and pair groups per class
+    // shell blocks and pair groups per class
     for (int k = 0; k < NCLASS; ++k) {
         ClassData& C = B.cls[k];
         if (!C.npairs) continue;
         const int bb = md_bb(C.la, C.lb);
+        const bool ba = md_block_a(C.la, C.lb);
+        const int lk = ba ? C.la : C.lb;
         std::vector<int> rank(nshell, -1);
         int nb = 0;
         for (int i = 0; i < nshell; ++i)
-            if (l[i] == C.lb) rank[i] = nb++;
+            if (l[i] == lk) rank[i] = nb++;
         const int nblk = (nb + bb - 1) / bb;
         // absent positions of the last block point at the spare rows after
         // the nao AO rows of F and G (MD_SPARE of them; never read back)
         C.bbao.assign((size_t)nblk * bb, B.nao);
         for (int i = 0; i < nshell; ++i)
             if (rank[i] >= 0) C.bbao[rank[i]] = B.shells[i].ao0;
+        auto kshell = [&](const PairRec& pr) { return ba ? pr.shA : pr.shB; };
+        auto ishell = [&](const PairRec& pr) { return ba ? pr.shB : pr.shA; };
         std::vector<int> ord(C.npairs);
         for (int i = 0; i < C.npairs; ++i) ord[i] = i;
         auto key = [&](int i) {
             const PairRec& pr = C.pairs[i];
-            return std::make_tuple(rank[pr.shB] / bb, pr.shA, pr.shB);
+            return std::make_tuple(rank[kshell(pr)] / bb, ishell(pr), kshell(pr));
         };
         std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key(x) < key(y); });
         std::vector<PairRec> pairs(C.npairs);
@@ -283,17 +287,21 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         for (int i = 0; i < C.npairs; ++i) {
             pairs[i] = C.pairs[ord[i]];
             scr[i] = C.scr[ord[i]];
-            const int blk = rank[pairs[i].shB] / bb, pos = rank[pairs[i].shB] % bb;
-            if (C.groups.empty() || C.groups.back().bblk != blk ||
-                C.groups.back().shA != pairs[i].shA) {
+            const int ks = kshell(pairs[i]), is = ishell(pairs[i]);
+            const int blk = rank[ks] / bb, pos = rank[ks] % bb;
+            if (C.groups.empty() || C.groups.back().bblk != blk || C.groups.back().shX != is) {
                 PairGroup gr{};
-                gr.shA = pairs[i].shA;
-                gr.a0 = pairs[i].a0;
+                gr.shX = is;
+                gr.x0 = B.shells[is].ao0;
                 gr.bblk = blk;
-                for (int j = 0; j < MD_BBMAX; ++j) gr.pidx[j] = -1;
+                for (int j = 0; j < MD_BBMAX; ++j) gr.pidx[j] = gr.pw[j] = -1;
                 C.groups.push_back(gr);
             }
+            const PairRec& pr = pairs[i];
+            if (pr.pp0 >= (1 << 23) || pr.npp >= 64)
+                throw Error(WB_UNSUPPORTED, "shell-pair class too large for the packed task words");
             C.groups.back().pidx[pos] = i;
+            C.groups.back().pw[pos] = pr.pp0 << 8 | pr.npp << 2 | (pr.onec ? 2 : 0) | (pr.diag ? 1 : 0);
             C.groups.back().npair++;
         }
         C.pairs.swap(pairs);

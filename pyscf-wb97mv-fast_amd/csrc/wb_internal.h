@@ -19,21 +19,29 @@ struct PairRec {              // one shell pair of a class, la >= lb
     int diag, onec;           // diag: A == B; onec: A and B on one centre
 };
 
-// B blocks: the B shells of a class (shells of angular momentum lb, in
-// shell order) are cut into blocks of md_bb(la, lb); an exchange kernel keeps
-// one block's F values and G accumulators in registers while it runs
-// through the A shells paired with that block.
+// Shell blocks: the shells of one side of a class -- A, of angular
+// momentum la, when la > lb, else B -- are cut, in shell order, into blocks
+// of md_bb(la, lb); an exchange kernel keeps one block's F values and G
+// accumulators in registers while it runs through the shells of the other
+// side paired with that block (its items).
 constexpr int MD_BBMAX = 8;
 // spare rows after the nao AO rows of F and G: the padding positions of the
-// last B block of a class point at row nao and touch rows nao .. nao + 2lb
+// last block of a class point at row nao and touch rows nao .. nao + 2l
 constexpr int MD_SPARE = 7;
+__host__ __device__ constexpr bool md_block_a(int la, int lb) { return la > lb; }
+// (sizes measured on water27 def2-SVP: for (p, s) 4 beats 8 by 7 % of the
+// exchange time -- 8 p shells in registers halve the occupancy)
 __host__ __device__ constexpr int md_bb(int la, int lb) {
-    return lb == 0 ? 8 : (lb == 1 ? (la == 1 ? 8 : 4) : (lb == 2 ? 2 : 1));
+    return la == lb ? (la <= 1 ? 8 : (la == 2 ? 2 : 1))
+                    : (la == 1 ? 4 : (la == 2 ? 2 : (lb == 0 ? 4 : (lb == 1 ? 2 : 1))));
 }
 
-struct PairGroup {            // the pairs (A, B) of one shell A and one B block
-    int shA, a0, bblk, npair;
-    int pidx[MD_BBMAX];       // pair index for B at block position j, -1 if none
+struct PairGroup {            // the pairs of one item shell and one block
+    int shX, x0, bblk, npair; // item shell, its AO offset, block index, pairs
+    int pidx[MD_BBMAX];       // pair index for the block position j, -1 if none
+    // the same pairs packed for the exchange kernels:
+    // pp0 << 8 | npp << 2 | one-centre << 1 | diag
+    int pw[MD_BBMAX];
 };
 
 struct PairScreen {           // screening data per pair (pair order of its class)

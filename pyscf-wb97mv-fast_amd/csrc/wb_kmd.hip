@@ -7,10 +7,11 @@
 //     V_ab(g) = sum_prims sum_tuv Ex Ey Ez R_tuv(p, P - g)
 // in registers (never stored), and contracts it with F (F = X Q D):
 //     G[g, B] += A(g)^T F[g, A],   G[g, A] += A(g) F[g, B]  (A != B)
-// An item is one A shell with the kept pairs of one B block (a bit mask);
-// items are ordered by B block, so the block's F values and G accumulators
-// stay in registers along a run of items and G[B] is written once per run,
-// G[A] once per item.  Each chunk writes its own copy of G (no atomics,
+// An item is one shell of one side with the kept pairs of one block of
+// shells of the other side (a bit mask).  The blocked side is the one with
+// the larger l (B when la = lb); items are ordered by block, so the block's
+// F values and G accumulators stay in registers along a run of items and
+// its G rows are written once per run, the item shell's once per item.  Each chunk writes its own copy of G (no atomics,
 // fixed order, bit-reproducible); the copies are summed inside the K GEMM.
 // The reference does the same contraction with one warp per 32-point
 // sub-block and one task per shell pair (gpu/sgx_k.py:335-400).
@@ -41,7 +42,7 @@ __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R
 // the Cartesian block of one shell pair at this lane's grid point, summed
 // over the pair's primitive pairs
 template <int LA, int LB, bool ONEC, typename real, typename vacc>
-__device__ __forceinline__ void md_block(const PairRec& pr, double px, double py, double pz,
+__device__ __forceinline__ void md_block(int pp0, int npp, double px, double py, double pz,
                                          const double* __restrict__ ppP,
                                          const real* __restrict__ ppE,
                                          const real* __restrict__ ppC,
@@ -54,7 +55,7 @@ __device__ __forceinline__ void md_block(const PairRec& pr, double px, double py
     // unrolled by two: the second primitive pair's scalar loads issue with
     // the first's (measured: exchange -1.2 %)
 #pragma unroll 2
-    for (int m = pr.pp0; m < pr.pp0 + pr.npp; ++m) {
+    for (int m = pp0; m < pp0 + npp; ++m) {
         const real dx = (real)(ppP[3 * m] - px);
         const real dy = (real)(ppP[3 * m + 1] - py);
         const real dz = (real)(ppP[3 * m + 2] - pz);
@@ -98,7 +99,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  gacc* __restrict__ G, size_t gstride,
                  const int* __restrict__ counts, const int* __restrict__ ids, int cap,
                  const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
-                 const PairRec* __restrict__ pairs, const double* __restrict__ ppP,
+                 const double* __restrict__ ppP,
                  const real* __restrict__ ppE, const real* __restrict__ ppC,
                  const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk)
 {
@@ -139,142 +140,155 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     // order, i.e. also on the next item's prefetched F and G rows
     __builtin_amdgcn_s_waitcnt(0);
 
-    // the current B block: F (Cartesian) and G accumulators of its shells
+    // the blocked side K (the shells of one block: F in Cartesian form and
+    // G accumulators in registers) and the item side I (one shell per item)
+    constexpr bool BA = md_block_a(LA, LB);
+    constexpr int LK = BA ? LA : LB, LI = BA ? LB : LA;
+    constexpr int NCK = (LK + 1) * (LK + 2) / 2, NCI = (LI + 1) * (LI + 2) / 2;
+    constexpr int NSK = 2 * LK + 1, NSI = 2 * LI + 1;
     int curB = -1;
-    real FbC[BB][NCB];
-    gacc accB[BB][NCB];
-    // G[B] += the block's accumulators: all row loads first, then the stores
-    auto flush_b = [&]() {
+    real FkC[BB][NCK];
+    gacc accK[BB][NCK];
+    // G[K] += the block's accumulators; every position is loaded and stored,
+    // untouched ones unchanged (absent positions use the spare rows):
+    // branch-free, so the loads issue as one batch and no store waits for
+    // more than its own row
+    auto flush_k = [&]() {
         int bo[BB];
 #pragma unroll
         for (int q = 0; q < BB; ++q) bo[q] = bbao[curB * BB + q];
-        // every position is loaded and stored, untouched ones unchanged
-        // (absent positions use the spare row): branch-free, so the loads
-        // issue as one batch and no store waits for more than its own row
-        gacc old[BB][NSB];
+        gacc old[BB][NSK];
 #pragma unroll
         for (int q = 0; q < BB; ++q)
 #pragma unroll
-            for (int m = 0; m < NSB; ++m) old[q][m] = Gc[(size_t)(bo[q] + m) * Ppad + g];
+            for (int m = 0; m < NSK; ++m) old[q][m] = Gc[(size_t)(bo[q] + m) * Ppad + g];
 #pragma unroll
         for (int q = 0; q < BB; ++q) {
-            gacc gs[NSB];
-            gsph<LB, gacc>(accB[q], gs);
+            gacc gs[NSK];
+            gsph<LK, gacc>(accK[q], gs);
 #pragma unroll
-            for (int m = 0; m < NSB; ++m) Gc[(size_t)(bo[q] + m) * Ppad + g] = old[q][m] + gs[m];
+            for (int m = 0; m < NSK; ++m) Gc[(size_t)(bo[q] + m) * Ppad + g] = old[q][m] + gs[m];
         }
     };
-    // software pipeline: while an item computes, the next item's F_A rows
-    // and (inside one B-block run, where nothing else writes them before
-    // its own update) its G_A rows are already on their way
+    // software pipeline: while an item computes, the next item's F rows and
+    // (inside one block run, where nothing else writes them before its own
+    // update) its G rows are already on their way
     int item = tl[t0];
-    real fa_n[NSA];
-    gacc ga_n[NSA];
-    bool ga_ok = false;
+    real fi_n[NSI];
+    gacc gi_n[NSI];
+    bool gi_ok = false;
     {
-        const int a0 = groups[item >> 8].a0;
+        const int x0 = groups[item >> 8].x0;
 #pragma unroll
-        for (int m = 0; m < NSA; ++m) fa_n[m] = F[(size_t)(a0 + m) * Ppad + g];
+        for (int m = 0; m < NSI; ++m) fi_n[m] = F[(size_t)(x0 + m) * Ppad + g];
     }
     for (int t = t0; t < t1; ++t) {
         const PairGroup* gr = groups + (item >> 8);
         const unsigned mask = (unsigned)item & 0xffu;
         const int bblk = gr->bblk;
-        const int a0 = gr->a0;
-        real fa[NSA];
-        gacc ga[NSA];
-        const bool have_ga = ga_ok;
+        const int x0 = gr->x0;
+        real fi[NSI];
+        gacc gi[NSI];
+        const bool have_gi = gi_ok;
 #pragma unroll
-        for (int m = 0; m < NSA; ++m) { fa[m] = fa_n[m]; ga[m] = ga_n[m]; }
+        for (int m = 0; m < NSI; ++m) { fi[m] = fi_n[m]; gi[m] = gi_n[m]; }
         int item_n = 0;
         if (t + 1 < t1) {                                    // uniform
             item_n = tl[t + 1];
             const PairGroup* gn = groups + (item_n >> 8);
-            const int a0n = gn->a0;
+            const int x0n = gn->x0;
 #pragma unroll
-            for (int m = 0; m < NSA; ++m) fa_n[m] = F[(size_t)(a0n + m) * Ppad + g];
-            // A shells within a run are distinct and a run change flushes
-            // the B block first: only a same-run prefetch of G_A is valid
-            ga_ok = gn->bblk == bblk;
-            if (ga_ok) {
+            for (int m = 0; m < NSI; ++m) fi_n[m] = F[(size_t)(x0n + m) * Ppad + g];
+            // item shells within a run are distinct and a run change
+            // flushes the block first: only a same-run prefetch is valid
+            gi_ok = gn->bblk == bblk;
+            if (gi_ok) {
 #pragma unroll
-                for (int m = 0; m < NSA; ++m) ga_n[m] = Gc[(size_t)(a0n + m) * Ppad + g];
+                for (int m = 0; m < NSI; ++m) gi_n[m] = Gc[(size_t)(x0n + m) * Ppad + g];
             }
         }
         if (bblk != curB) {
-            if (curB >= 0) flush_b();
+            if (curB >= 0) flush_k();
             curB = bblk;
             // the block's F rows, all loads back to back (absent positions
-            // read the spare row; their values are never used: no mask bit)
+            // read a spare row; their values are never used: no mask bit)
             int bo[BB];
 #pragma unroll
             for (int q = 0; q < BB; ++q) bo[q] = bbao[curB * BB + q];
-            real fb[BB][NSB];
+            real fk[BB][NSK];
 #pragma unroll
             for (int q = 0; q < BB; ++q)
 #pragma unroll
-                for (int m = 0; m < NSB; ++m) fb[q][m] = F[(size_t)(bo[q] + m) * Ppad + g];
+                for (int m = 0; m < NSK; ++m) fk[q][m] = F[(size_t)(bo[q] + m) * Ppad + g];
 #pragma unroll
             for (int q = 0; q < BB; ++q) {
-                fcart<LB, real>(fb[q], FbC[q]);
+                fcart<LK, real>(fk[q], FkC[q]);
 #pragma unroll
-                for (int c = 0; c < NCB; ++c) accB[q][c] = (gacc)0;
+                for (int c = 0; c < NCK; ++c) accK[q][c] = (gacc)0;
             }
         }
-        real FaC[NCA];
-        fcart<LA, real>(fa, FaC);
-        gacc accA[NCA];
+        real FiC[NCI];
+        fcart<LI, real>(fi, FiC);
+        gacc accI[NCI];
 #pragma unroll
-        for (int c = 0; c < NCA; ++c) accA[c] = (gacc)0;
-        bool anyA = false;
-        int pi[BB];
+        for (int c = 0; c < NCI; ++c) accI[c] = (gacc)0;
+        bool anyI = false;
+        // the group's packed task words: no dependent pair-record load
+        int pw[BB];
 #pragma unroll
-        for (int q = 0; q < BB; ++q) pi[q] = gr->pidx[q];
+        for (int q = 0; q < BB; ++q) pw[q] = gr->pw[q];
 #pragma unroll
         for (int q = 0; q < BB; ++q) {
             if (!(mask >> q & 1u)) continue;                 // uniform
-            const PairRec pr = pairs[pi[q]];
+            const int pp0 = pw[q] >> 8, npp = (pw[q] >> 2) & 63;
+            const bool onec = pw[q] & 2, diag = pw[q] & 1;
             vacc Vs[NCA * NCB];
 #pragma unroll
             for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
             // one-centre pairs take the variant whose FP32 E products all
             // carry their rounding remainders
-            if (pr.onec) md_block<LA, LB, true, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
-            else md_block<LA, LB, false, real, vacc>(pr, px, py, pz, ppP, ppE, ppC, tab, Vs);
+            if (onec) md_block<LA, LB, true, real, vacc>(pp0, npp, px, py, pz, ppP, ppE, ppC, tab, Vs);
+            else md_block<LA, LB, false, real, vacc>(pp0, npp, px, py, pz, ppP, ppE, ppC, tab, Vs);
             real V[NCA * NCB];
 #pragma unroll
             for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];
+            // G[B] += V^T F[A], G[A] += V F[B]; for a diagonal pair (A = B,
+            // blocked side B) only the first
+            const real* FaC = BA ? FkC[q] : FiC;
+            const real* FbC = BA ? FiC : FkC[q];
+            gacc* gA = BA ? accK[q] : accI;
+            gacc* gB = BA ? accI : accK[q];
 #pragma unroll
-            for (int b = 0; b < NCB; ++b) {
+            for (int bb = 0; bb < NCB; ++bb) {
                 real v = (real)0;
 #pragma unroll
-                for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + b], FaC[a], v);
-                accB[q][b] += (gacc)v;
+                for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + bb], FaC[a], v);
+                gB[bb] += (gacc)v;
             }
-            if (!pr.diag) {                                  // uniform
-                anyA = true;
+            if (BA || !diag) {                               // uniform
 #pragma unroll
                 for (int a = 0; a < NCA; ++a) {
                     real v = (real)0;
 #pragma unroll
-                    for (int b = 0; b < NCB; ++b) v = fma(V[a * NCB + b], FbC[q][b], v);
-                    accA[a] += (gacc)v;
+                    for (int bb = 0; bb < NCB; ++bb) v = fma(V[a * NCB + bb], FbC[bb], v);
+                    gA[a] += (gacc)v;
                 }
             }
+            anyI = anyI || BA || !diag;
         }
-        if (anyA) {
-            gacc gs[NSA];
-            gsph<LA, gacc>(accA, gs);
-            if (!have_ga) {
+        if (anyI) {
+            gacc gs[NSI];
+            gsph<LI, gacc>(accI, gs);
+            if (!have_gi) {
 #pragma unroll
-                for (int m = 0; m < NSA; ++m) ga[m] = Gc[(size_t)(a0 + m) * Ppad + g];
+                for (int m = 0; m < NSI; ++m) gi[m] = Gc[(size_t)(x0 + m) * Ppad + g];
             }
 #pragma unroll
-            for (int m = 0; m < NSA; ++m) Gc[(size_t)(a0 + m) * Ppad + g] = ga[m] + gs[m];
+            for (int m = 0; m < NSI; ++m) Gc[(size_t)(x0 + m) * Ppad + g] = gi[m] + gs[m];
         }
         item = item_n;
     }
-    if (curB >= 0) flush_b();
+    if (curB >= 0) flush_k();
 }
 
 template <int LA, int LB, typename real, typename gacc>
@@ -288,7 +302,7 @@ static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const 
     hipLaunchKernelGGL((md_k_kernel<LA, LB, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES), 0, s,
                        g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, counts, ids,
                        c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
-                       c.d_pairs.as<PairRec>(), c.d_P.as<double>(), E,
+                       c.d_P.as<double>(), E,
                        (const real*)C, (const real*)boys, g.Ppad, g.nsub, ncopy);
     WB_HIP(hipGetLastError());
 }

@@ -424,9 +424,12 @@ __device__ __forceinline__ void store_tile(OutT* __restrict__ C, int ldc, int M,
             }
 }
 
-// C[M][N] = A[M][K] B[K][N]; m tiles fastest (they share the B slice)
+// C[M][N] = A[M][K] B[K][N]; m tiles fastest (they share the B slice).
+// One block per CU: the compiler then keeps the accumulators in AGPRs and
+// schedules the slice better (3.8 -> 3.3-3.5 ms for nao x P x nao, 648 x
+// 218624 x 648; the split-K NT kernel below is faster at two)
 template <int WM, int WN, typename OutT>
-__global__ __launch_bounds__(256, 2)
+__global__ __launch_bounds__(256, 1)
 void gemm_nn_t_kernel(int M, int N, int K, const double* __restrict__ A,
                       const double* __restrict__ B, OutT* __restrict__ C, int mt)
 {
