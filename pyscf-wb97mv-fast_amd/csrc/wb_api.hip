@@ -338,52 +338,58 @@ static void upload_basis(Basis& B, hipStream_t s)
         upload(C.d_scr, C.scr.data(), sizeof(PairScreen) * C.npairs, s);
         upload(C.d_groups, C.groups.data(), sizeof(PairGroup) * C.ngroups, s);
         upload(C.d_bbao, C.bbao.data(), sizeof(int) * C.bbao.size(), s);
-        upload(C.d_P, C.P.data(), sizeof(double) * C.P.size(), s);
-        // per primitive pair: the erl leading FP32 values, then their remainders
-        std::vector<float> e32(2 * C.E.size());
-        for (size_t m = 0; m < C.E.size() / C.erl; ++m)
-            for (int i = 0; i < C.erl; ++i) {
-                const double v = C.E[m * C.erl + i];
-                const float hi = (float)v;
-                e32[2 * m * C.erl + i] = hi;
-                e32[2 * m * C.erl + C.erl + i] = (float)(v - (double)hi);
-            }
-        upload(C.d_E32, e32.data(), sizeof(float) * e32.size(), s);
-        upload(C.d_E64, C.E.data(), sizeof(double) * C.E.size(), s);
     }
     WB_HIP(hipStreamSynchronize(s));
 }
 
-// per-omega primitive-pair operator constants [tp, c_0..c_L]:
-// theta = w^2/(w^2+p) (1 for 1/r), tp = theta p, c_n = pref sqrt(theta)(-2 theta p)^n,
-// formed in FP64; the FP32 record holds (hi, lo) pairs [tp, c_0..c_L] so the
-// kernel forms each constant product with one rounding (fma(hi, x, lo x))
+// Per-omega primitive-pair records (md_rec_bytes layout).  Operator
+// constants [tp, c_0..c_L]: theta = w^2/(w^2+p) (1 for 1/r), tp = theta p,
+// c_n = pref sqrt(theta)(-2 theta p)^n, formed in FP64; FP32 records hold
+// (hi, lo) pairs so the kernel forms each constant product with one
+// rounding (fma(hi, x, lo x)), and the E coefficients followed by their
+// FP32 rounding remainders.
 static const void* op_consts(ClassData& C, double omega, bool f32, hipStream_t s)
 {
     auto it = C.opc.find(omega);
     if (it == C.opc.end()) {
-        const int L = C.la + C.lb, crl = L + 2;
-        std::vector<double> c64((size_t)C.nprims * crl);
+        const int L = C.la + C.lb, nc = L + 2, erl = C.erl;
+        const size_t rb32 = md_rec_bytes(C.la, C.lb, true), rb64 = md_rec_bytes(C.la, C.lb, false);
+        std::vector<char> r32((size_t)C.nprims * rb32, 0), r64((size_t)C.nprims * rb64, 0);
+        std::vector<double> c(nc);
         for (int m = 0; m < C.nprims; ++m) {
             const double p = C.p[m];
             const double th = omega > 0 ? omega * omega / (omega * omega + p) : 1.0;
-            c64[(size_t)m * crl] = th * p;
-            double c = C.pref[m] * std::sqrt(th);
+            c[0] = th * p;
+            double cn = C.pref[m] * std::sqrt(th);
             for (int n = 0; n <= L; ++n) {
-                c64[(size_t)m * crl + 1 + n] = c;
-                c *= -2.0 * th * p;
+                c[1 + n] = cn;
+                cn *= -2.0 * th * p;
+            }
+            char* q32 = r32.data() + m * rb32;
+            char* q64 = r64.data() + m * rb64;
+            std::memcpy(q32, &C.P[3 * (size_t)m], 24);
+            std::memcpy(q64, &C.P[3 * (size_t)m], 24);
+            float* f = (float*)(q32 + 24);
+            double* d = (double*)(q64 + 24);
+            for (int i = 0; i < nc; ++i) {
+                const float hi = (float)c[i];
+                f[2 * i] = hi;
+                f[2 * i + 1] = (float)(c[i] - (double)hi);
+                d[i] = c[i];
+            }
+            float* e32 = f + 2 * nc;
+            double* e64 = d + nc;
+            for (int i = 0; i < erl; ++i) {
+                const double v = C.E[(size_t)m * erl + i];
+                const float hi = (float)v;
+                e32[i] = hi;
+                e32[erl + i] = (float)(v - (double)hi);
+                e64[i] = v;
             }
         }
-        std::vector<float> c32(2 * c64.size());
-        for (size_t i = 0; i < c64.size(); ++i) {
-            const float hi = (float)c64[i];
-            c32[2 * i] = hi;
-            c32[2 * i + 1] = (float)(c64[i] - (double)hi);
-        }
         auto b32 = std::make_shared<DevBuf>(), b64 = std::make_shared<DevBuf>();
-        upload(*b32, c32.data(), sizeof(float) * c32.size(), s);
-        upload(*b64, c64.data(), sizeof(double) * c64.size(), s);
-        WB_HIP(hipStreamSynchronize(s));
+        upload(*b32, r32.data(), r32.size(), s);
+        upload(*b64, r64.data(), r64.size(), s);
         it = C.opc.emplace(omega, std::make_pair(b32, b64)).first;
     }
     return f32 ? it->second.first->p : it->second.second->p;

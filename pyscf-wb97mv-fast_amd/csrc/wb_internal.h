@@ -36,6 +36,18 @@ __host__ __device__ constexpr int md_bb(int la, int lb) {
                     : (la == 1 ? 4 : (la == 2 ? 2 : (lb == 0 ? 4 : (lb == 1 ? 2 : 1))));
 }
 
+// One record per primitive pair and operator, for the exchange kernels:
+//   P (3 doubles) | operator constants C | Hermite E coefficients,
+// padded to 16 bytes.  FP32: C = (hi, lo) pairs of [theta p, c_0..c_L], E =
+// the erl leading values then their rounding remainders; FP64: plain.
+__host__ __device__ constexpr int md_crl(int la, int lb, bool f32) { return (la + lb + 2) * (f32 ? 2 : 1); }
+__host__ __device__ constexpr int md_erl(int la, int lb, bool f32) {
+    return 3 * (la + 1) * (lb + 1) * (la + lb + 1) * (f32 ? 2 : 1);
+}
+__host__ __device__ constexpr int md_rec_bytes(int la, int lb, bool f32) {
+    return (24 + (md_crl(la, lb, f32) + md_erl(la, lb, f32)) * (f32 ? 4 : 8) + 15) / 16 * 16;
+}
+
 struct PairGroup {            // the pairs of one item shell and one block
     int shX, x0, bblk, npair; // item shell, its AO offset, block index, pairs
     int pidx[MD_BBMAX];       // pair index for the block position j, -1 if none
@@ -65,8 +77,9 @@ struct ClassData {
     std::vector<PairGroup> groups;
     std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, nao (spare rows) if none
     int ngroups = 0;
-    DevBuf d_pairs, d_scr, d_P, d_E32, d_E64, d_groups, d_bbao;
-    std::map<double, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;  // omega -> (C32, C64)
+    DevBuf d_pairs, d_scr, d_groups, d_bbao;
+    // omega -> primitive-pair records (FP32, FP64), built on first use
+    std::map<double, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;
 };
 
 struct Basis {
@@ -148,9 +161,10 @@ void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float*
 // F_0..F_L at n values T (device arrays), out [L+1][n]
 void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
                       const void* b32, const void* b64, double* out);
+// recs: the class's primitive-pair records for one operator (md_rec_bytes)
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
                const void* F, void* G, size_t gstride, int ncopy,
-               const int* counts, const int* ids, const void* C, const void* boys);
+               const int* counts, const int* ids, const void* recs, const void* boys);
 
 // VV10
 void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T,

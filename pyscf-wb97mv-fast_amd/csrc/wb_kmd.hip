@@ -43,23 +43,23 @@ __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R
 // over the pair's primitive pairs
 template <int LA, int LB, bool ONEC, typename real, typename vacc>
 __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py, double pz,
-                                         const double* __restrict__ ppP,
-                                         const real* __restrict__ ppE,
-                                         const real* __restrict__ ppC,
+                                         const char* __restrict__ recs,
                                          const real* __restrict__ tab, vacc* Vs)
 {
     constexpr int L = LA + LB;
     constexpr bool F32 = sizeof(real) == 4;
-    constexpr int ERL = 3 * (LA + 1) * (LB + 1) * (L + 1) * (F32 ? 2 : 1);
-    constexpr int CRL = (L + 2) * (F32 ? 2 : 1);   // FP32: (hi, lo) pairs
+    constexpr int CRL = md_crl(LA, LB, F32);
+    constexpr int RB = md_rec_bytes(LA, LB, F32);
     // unrolled by two: the second primitive pair's scalar loads issue with
     // the first's (measured: exchange -1.2 %)
 #pragma unroll 2
     for (int m = pp0; m < pp0 + npp; ++m) {
-        const real dx = (real)(ppP[3 * m] - px);
-        const real dy = (real)(ppP[3 * m + 1] - py);
-        const real dz = (real)(ppP[3 * m + 2] - pz);
-        const real* C = ppC + (size_t)m * CRL;
+        const char* rec = recs + (size_t)m * RB;
+        const double* P = (const double*)rec;
+        const real* C = (const real*)(rec + 24);
+        const real dx = (real)(P[0] - px);
+        const real dy = (real)(P[1] - py);
+        const real dz = (real)(P[2] - pz);
         const real r2 = fma(dx, dx, fma(dy, dy, dz * dz));
         real Fb[L + 1], R0[L + 1];
         if constexpr (F32) {
@@ -71,7 +71,7 @@ __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py,
 #pragma unroll
             for (int n = 0; n <= L; ++n) R0[n] = C[1 + n] * Fb[n];
         }
-        md_prim<LA, LB, ONEC, real, vacc>(dx, dy, dz, R0, ppE + (size_t)m * ERL, Vs);
+        md_prim<LA, LB, ONEC, real, vacc>(dx, dy, dz, R0, C + CRL, Vs);
     }
 }
 
@@ -99,8 +99,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  gacc* __restrict__ G, size_t gstride,
                  const int* __restrict__ counts, const int* __restrict__ ids, int cap,
                  const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
-                 const double* __restrict__ ppP,
-                 const real* __restrict__ ppE, const real* __restrict__ ppC,
+                 const char* __restrict__ recs,
                  const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk)
 {
     constexpr int L = LA + LB;
@@ -247,8 +246,8 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
             // one-centre pairs take the variant whose FP32 E products all
             // carry their rounding remainders
-            if (onec) md_block<LA, LB, true, real, vacc>(pp0, npp, px, py, pz, ppP, ppE, ppC, tab, Vs);
-            else md_block<LA, LB, false, real, vacc>(pp0, npp, px, py, pz, ppP, ppE, ppC, tab, Vs);
+            if (onec) md_block<LA, LB, true, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
+            else md_block<LA, LB, false, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
             real V[NCA * NCB];
 #pragma unroll
             for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];
@@ -294,32 +293,30 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 template <int LA, int LB, typename real, typename gacc>
 static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
                         void* G, size_t gstride, int ncopy, const int* counts,
-                        const int* ids, const void* C, const void* boys)
+                        const int* ids, const void* recs, const void* boys)
 {
     const int workers = g.nsub * ncopy;
     const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
-    const real* E = (sizeof(real) == 4) ? c.d_E32.as<real>() : c.d_E64.as<real>();
     hipLaunchKernelGGL((md_k_kernel<LA, LB, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES), 0, s,
                        g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, counts, ids,
                        c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
-                       c.d_P.as<double>(), E,
-                       (const real*)C, (const real*)boys, g.Ppad, g.nsub, ncopy);
+                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy);
     WB_HIP(hipGetLastError());
 }
 
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
                const void* F, void* G, size_t gstride, int ncopy, const int* counts,
-               const int* ids, const void* C, const void* boys)
+               const int* ids, const void* recs, const void* boys)
 {
     if (c.npairs == 0) return;
 #define WB_CASE(XA, XB)                                                                \
     if (c.la == XA && c.lb == XB) {                                                    \
         if (f32 && !g64)                                                               \
-            md_launch_t<XA, XB, float, float>(s, c, g, F, G, gstride, ncopy, counts, ids, C, boys); \
+            md_launch_t<XA, XB, float, float>(s, c, g, F, G, gstride, ncopy, counts, ids, recs, boys); \
         else if (f32)                                                                  \
-            md_launch_t<XA, XB, float, double>(s, c, g, F, G, gstride, ncopy, counts, ids, C, boys); \
+            md_launch_t<XA, XB, float, double>(s, c, g, F, G, gstride, ncopy, counts, ids, recs, boys); \
         else                                                                           \
-            md_launch_t<XA, XB, double, double>(s, c, g, F, G, gstride, ncopy, counts, ids, C, boys); \
+            md_launch_t<XA, XB, double, double>(s, c, g, F, G, gstride, ncopy, counts, ids, recs, boys); \
         return;                                                                        \
     }
     WB_CASE(0, 0) WB_CASE(1, 0) WB_CASE(1, 1) WB_CASE(2, 0) WB_CASE(2, 1)
