@@ -342,43 +342,48 @@ static void upload_basis(Basis& B, hipStream_t s)
     WB_HIP(hipStreamSynchronize(s));
 }
 
-// Per-omega primitive-pair records (md_rec_bytes layout).  Operator
+// Primitive-pair records of an operator set (md_rec_bytes layout).  Operator
 // constants [tp, c_0..c_L]: theta = w^2/(w^2+p) (1 for 1/r), tp = theta p,
 // c_n = pref sqrt(theta)(-2 theta p)^n, formed in FP64; FP32 records hold
 // (hi, lo) pairs so the kernel forms each constant product with one
 // rounding (fma(hi, x, lo x)), and the E coefficients followed by their
 // FP32 rounding remainders.
-static const void* op_consts(ClassData& C, double omega, bool f32, hipStream_t s)
+static const void* op_consts(ClassData& C, const std::vector<double>& omegas, bool f32,
+                             hipStream_t s)
 {
-    auto it = C.opc.find(omega);
+    auto it = C.opc.find(omegas);
     if (it == C.opc.end()) {
-        const int L = C.la + C.lb, nc = L + 2, erl = C.erl;
-        const size_t rb32 = md_rec_bytes(C.la, C.lb, true), rb64 = md_rec_bytes(C.la, C.lb, false);
+        const int L = C.la + C.lb, nc = L + 2, erl = C.erl, nop = (int)omegas.size();
+        const size_t rb32 = md_rec_bytes(C.la, C.lb, true, nop);
+        const size_t rb64 = md_rec_bytes(C.la, C.lb, false, nop);
         std::vector<char> r32((size_t)C.nprims * rb32, 0), r64((size_t)C.nprims * rb64, 0);
         std::vector<double> c(nc);
         for (int m = 0; m < C.nprims; ++m) {
-            const double p = C.p[m];
-            const double th = omega > 0 ? omega * omega / (omega * omega + p) : 1.0;
-            c[0] = th * p;
-            double cn = C.pref[m] * std::sqrt(th);
-            for (int n = 0; n <= L; ++n) {
-                c[1 + n] = cn;
-                cn *= -2.0 * th * p;
-            }
             char* q32 = r32.data() + m * rb32;
             char* q64 = r64.data() + m * rb64;
             std::memcpy(q32, &C.P[3 * (size_t)m], 24);
             std::memcpy(q64, &C.P[3 * (size_t)m], 24);
             float* f = (float*)(q32 + 24);
             double* d = (double*)(q64 + 24);
-            for (int i = 0; i < nc; ++i) {
-                const float hi = (float)c[i];
-                f[2 * i] = hi;
-                f[2 * i + 1] = (float)(c[i] - (double)hi);
-                d[i] = c[i];
+            const double p = C.p[m];
+            for (int o = 0; o < nop; ++o) {
+                const double omega = omegas[o];
+                const double th = omega > 0 ? omega * omega / (omega * omega + p) : 1.0;
+                c[0] = th * p;
+                double cn = C.pref[m] * std::sqrt(th);
+                for (int n = 0; n <= L; ++n) {
+                    c[1 + n] = cn;
+                    cn *= -2.0 * th * p;
+                }
+                for (int i = 0; i < nc; ++i) {
+                    const float hi = (float)c[i];
+                    f[2 * (o * nc + i)] = hi;
+                    f[2 * (o * nc + i) + 1] = (float)(c[i] - (double)hi);
+                    d[o * nc + i] = c[i];
+                }
             }
-            float* e32 = f + 2 * nc;
-            double* e64 = d + nc;
+            float* e32 = f + 2 * nc * nop;
+            double* e64 = d + nc * nop;
             for (int i = 0; i < erl; ++i) {
                 const double v = C.E[(size_t)m * erl + i];
                 const float hi = (float)v;
@@ -390,7 +395,7 @@ static const void* op_consts(ClassData& C, double omega, bool f32, hipStream_t s
         auto b32 = std::make_shared<DevBuf>(), b64 = std::make_shared<DevBuf>();
         upload(*b32, r32.data(), r32.size(), s);
         upload(*b64, r64.data(), r64.size(), s);
-        it = C.opc.emplace(omega, std::make_pair(b32, b64)).first;
+        it = C.opc.emplace(omegas, std::make_pair(b32, b64)).first;
     }
     return f32 ? it->second.first->p : it->second.second->p;
 }
@@ -539,23 +544,34 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     c.tasks_total = (long long)ntask;
     const int ncopy = c.ncopy;
     const size_t gstride = (size_t)(nao + MD_SPARE) * Ppad;
+    const size_t opstride = gstride * ncopy;              // elements between operators' G
     const bool g64 = !f32 || c.g64;
     const size_t gsz = g64 ? 8 : 4;
-    void* Gb = c.ws_G.ensure(gsz * gstride * ncopy);
+    char* Gb = (char*)c.ws_G.ensure(gsz * opstride * nop);
     const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
     c.mark(PH_SCREEN);
-    for (int op = 0; op < nop; ++op) {
-        zero_bytes(s, Gb, gsz * gstride * ncopy);
-        for (int k = 0; k < NCLASS; ++k) {
-            ClassData& C = B.cls[k];
-            if (!C.npairs) continue;
-            const void* Cc = op_consts(C, omegas[op], f32, s);
-            launch_md(s, C, G, f32, g64, F, Gb, gstride, ncopy, counts + (size_t)k * G.nsub,
-                      ids + cls_off[k], Cc, boys);
+    zero_bytes(s, Gb, gsz * opstride * nop);
+    for (int k = 0; k < NCLASS; ++k) {
+        ClassData& C = B.cls[k];
+        if (!C.npairs) continue;
+        const int* cnt = counts + (size_t)k * G.nsub;
+        const int* ids_k = ids + cls_off[k];
+        if (nop == 2 && md_fuse(C.la, C.lb)) {
+            // both operators in one pass over the tasks
+            const void* rc = op_consts(C, {omegas[0], omegas[1]}, f32, s);
+            launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, rc, boys);
+        } else {
+            for (int op = 0; op < nop; ++op) {
+                const void* rc = op_consts(C, {omegas[op]}, f32, s);
+                launch_md(s, C, G, f32, g64, 1, F, Gb + gsz * opstride * op, gstride, opstride,
+                          ncopy, cnt, ids_k, rc, boys);
+            }
         }
-        c.mark(PH_MD);
-        gemm_nt_fast(s, nao, nao, Ppad, X, Gb, !g64, ncopy, gstride, G.d_w.as<double>(),
-                       d_K + (size_t)op * nao * nao, c.ws_Kp);
+    }
+    c.mark(PH_MD);
+    for (int op = 0; op < nop; ++op) {
+        gemm_nt_fast(s, nao, nao, Ppad, X, Gb + gsz * opstride * op, !g64, ncopy, gstride,
+                     G.d_w.as<double>(), d_K + (size_t)op * nao * nao, c.ws_Kp);
         c.mark(PH_KGEMM);
     }
     collect(c);

@@ -32,21 +32,28 @@ __host__ __device__ constexpr bool md_block_a(int la, int lb) { return la > lb; 
 // (sizes measured on water27 def2-SVP: for (p, s) 4 beats 8 by 7 % of the
 // exchange time -- 8 p shells in registers halve the occupancy)
 __host__ __device__ constexpr int md_bb(int la, int lb) {
-    return la == lb ? (la <= 1 ? 8 : (la == 2 ? 2 : 1))
+    return la == lb ? (la <= 1 ? 8 : 1)
                     : (la == 1 ? 4 : (la == 2 ? 2 : (lb == 0 ? 4 : (lb == 1 ? 2 : 1))));
 }
 
-// One record per primitive pair and operator, for the exchange kernels:
-//   P (3 doubles) | operator constants C | Hermite E coefficients,
+// One record per primitive pair and operator set, for the exchange kernels:
+//   P (3 doubles) | operator constants C per operator | Hermite E,
 // padded to 16 bytes.  FP32: C = (hi, lo) pairs of [theta p, c_0..c_L], E =
 // the erl leading values then their rounding remainders; FP64: plain.
 __host__ __device__ constexpr int md_crl(int la, int lb, bool f32) { return (la + lb + 2) * (f32 ? 2 : 1); }
 __host__ __device__ constexpr int md_erl(int la, int lb, bool f32) {
     return 3 * (la + 1) * (lb + 1) * (la + lb + 1) * (f32 ? 2 : 1);
 }
-__host__ __device__ constexpr int md_rec_bytes(int la, int lb, bool f32) {
-    return (24 + (md_crl(la, lb, f32) + md_erl(la, lb, f32)) * (f32 ? 4 : 8) + 15) / 16 * 16;
+// nop operators: P | C_0 .. C_{nop-1} | E
+__host__ __device__ constexpr int md_rec_bytes(int la, int lb, bool f32, int nop) {
+    return (24 + (nop * md_crl(la, lb, f32) + md_erl(la, lb, f32)) * (f32 ? 4 : 8) + 15) / 16 * 16;
 }
+// classes whose kernel evaluates both operators of a K build in one pass
+// (the registers of two sets of blocks and accumulators fit: (s, s),
+// (p, s), (d, s))
+// (measured on water27: fusing (p, p) too -- 223 VGPRs, two waves per SIMD
+// -- costs 0.9 ms of exchange time instead of saving it)
+__host__ __device__ constexpr bool md_fuse(int la, int lb) { return lb == 0 && la <= 2; }
 
 struct PairGroup {            // the pairs of one item shell and one block
     int shX, x0, bblk, npair; // item shell, its AO offset, block index, pairs
@@ -78,8 +85,9 @@ struct ClassData {
     std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, nao (spare rows) if none
     int ngroups = 0;
     DevBuf d_pairs, d_scr, d_groups, d_bbao;
-    // omega -> primitive-pair records (FP32, FP64), built on first use
-    std::map<double, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;
+    // operator set (omegas) -> primitive-pair records (FP32, FP64), built
+    // on first use
+    std::map<std::vector<double>, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;
 };
 
 struct Basis {
@@ -161,9 +169,10 @@ void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float*
 // F_0..F_L at n values T (device arrays), out [L+1][n]
 void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
                       const void* b32, const void* b64, double* out);
-// recs: the class's primitive-pair records for one operator (md_rec_bytes)
-void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
-               const void* F, void* G, size_t gstride, int ncopy,
+// recs: the class's primitive-pair records for nop operators (md_rec_bytes);
+// nop = 2 only where md_fuse; G of operator o at G + o opstride (elements)
+void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
+               const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
                const int* counts, const int* ids, const void* recs, const void* boys);
 
 // VV10

@@ -39,9 +39,10 @@ __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R
 #undef WB_MD_PRIM
 }
 
-// the Cartesian block of one shell pair at this lane's grid point, summed
-// over the pair's primitive pairs
-template <int LA, int LB, bool ONEC, typename real, typename vacc>
+// the Cartesian blocks of one shell pair at this lane's grid point, summed
+// over the pair's primitive pairs, for NOP operators at once (they share
+// P - g, the E coefficients and every load; Vs holds NOP blocks)
+template <int LA, int LB, bool ONEC, int NOP, typename real, typename vacc>
 __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py, double pz,
                                          const char* __restrict__ recs,
                                          const real* __restrict__ tab, vacc* Vs)
@@ -49,7 +50,8 @@ __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py,
     constexpr int L = LA + LB;
     constexpr bool F32 = sizeof(real) == 4;
     constexpr int CRL = md_crl(LA, LB, F32);
-    constexpr int RB = md_rec_bytes(LA, LB, F32);
+    constexpr int RB = md_rec_bytes(LA, LB, F32, NOP);
+    constexpr int NV = (LA + 1) * (LA + 2) / 2 * ((LB + 1) * (LB + 2) / 2);
     // unrolled by two: the second primitive pair's scalar loads issue with
     // the first's (measured: exchange -1.2 %)
 #pragma unroll 2
@@ -61,17 +63,21 @@ __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py,
         const real dy = (real)(P[1] - py);
         const real dz = (real)(P[2] - pz);
         const real r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        real Fb[L + 1], R0[L + 1];
-        if constexpr (F32) {
-            Boys<real>::template eval<L>(fmaf(C[0], r2, C[1] * r2), tab, Fb);
 #pragma unroll
-            for (int n = 0; n <= L; ++n) R0[n] = fmaf(C[2 + 2 * n], Fb[n], C[3 + 2 * n] * Fb[n]);
-        } else {
-            Boys<real>::template eval<L>(C[0] * r2, tab, Fb);
+        for (int o = 0; o < NOP; ++o) {
+            const real* Co = C + o * CRL;
+            real Fb[L + 1], R0[L + 1];
+            if constexpr (F32) {
+                Boys<real>::template eval<L>(fmaf(Co[0], r2, Co[1] * r2), tab, Fb);
 #pragma unroll
-            for (int n = 0; n <= L; ++n) R0[n] = C[1 + n] * Fb[n];
+                for (int n = 0; n <= L; ++n) R0[n] = fmaf(Co[2 + 2 * n], Fb[n], Co[3 + 2 * n] * Fb[n]);
+            } else {
+                Boys<real>::template eval<L>(Co[0] * r2, tab, Fb);
+#pragma unroll
+                for (int n = 0; n <= L; ++n) R0[n] = Co[1 + n] * Fb[n];
+            }
+            md_prim<LA, LB, ONEC, real, vacc>(dx, dy, dz, R0, C + NOP * CRL, Vs + o * NV);
         }
-        md_prim<LA, LB, ONEC, real, vacc>(dx, dy, dz, R0, C + CRL, Vs);
     }
 }
 
@@ -92,11 +98,12 @@ __device__ __forceinline__ void gsph(const real* Gc, real* Gs) {
 
 constexpr int MD_WAVES = 4;
 
-// real: integral arithmetic; gacc: G accumulation (registers and memory)
-template <int LA, int LB, typename real, typename gacc>
+// real: integral arithmetic; gacc: G accumulation (registers and memory);
+// NOP operators at once (G of operator o at G + o opstride)
+template <int LA, int LB, int NOP, typename real, typename gacc>
 __global__ __launch_bounds__(64 * MD_WAVES)
 void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
-                 gacc* __restrict__ G, size_t gstride,
+                 gacc* __restrict__ G, size_t gstride, size_t opstride,
                  const int* __restrict__ counts, const int* __restrict__ ids, int cap,
                  const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
                  const char* __restrict__ recs,
@@ -131,7 +138,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     const int t1 = (int)(((long long)cnt * (j + 1)) / nchunk);
     if (t0 >= t1) return;
     const int* tl = ids + (size_t)s * cap;
-    gacc* Gc = G + (size_t)j * gstride;
+    gacc* const Gc = G + (size_t)j * gstride;         // operator o: Gc + o opstride
     const int g = s * SUB + lane;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
     // wait for the point's coordinates here, once: otherwise the first use
@@ -145,9 +152,10 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     constexpr int LK = BA ? LA : LB, LI = BA ? LB : LA;
     constexpr int NCK = (LK + 1) * (LK + 2) / 2, NCI = (LI + 1) * (LI + 2) / 2;
     constexpr int NSK = 2 * LK + 1, NSI = 2 * LI + 1;
+    constexpr int NV = NCA * NCB;
     int curB = -1;
     real FkC[BB][NCK];
-    gacc accK[BB][NCK];
+    gacc accK[NOP][BB][NCK];
     // G[K] += the block's accumulators; every position is loaded and stored,
     // untouched ones unchanged (absent positions use the spare rows):
     // branch-free, so the loads issue as one batch and no store waits for
@@ -156,25 +164,31 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
         int bo[BB];
 #pragma unroll
         for (int q = 0; q < BB; ++q) bo[q] = bbao[curB * BB + q];
-        gacc old[BB][NSK];
+        gacc old[NOP][BB][NSK];
 #pragma unroll
-        for (int q = 0; q < BB; ++q)
+        for (int o = 0; o < NOP; ++o)
 #pragma unroll
-            for (int m = 0; m < NSK; ++m) old[q][m] = Gc[(size_t)(bo[q] + m) * Ppad + g];
+            for (int q = 0; q < BB; ++q)
 #pragma unroll
-        for (int q = 0; q < BB; ++q) {
-            gacc gs[NSK];
-            gsph<LK, gacc>(accK[q], gs);
+                for (int m = 0; m < NSK; ++m)
+                    old[o][q][m] = Gc[o * opstride + (size_t)(bo[q] + m) * Ppad + g];
 #pragma unroll
-            for (int m = 0; m < NSK; ++m) Gc[(size_t)(bo[q] + m) * Ppad + g] = old[q][m] + gs[m];
-        }
+        for (int o = 0; o < NOP; ++o)
+#pragma unroll
+            for (int q = 0; q < BB; ++q) {
+                gacc gs[NSK];
+                gsph<LK, gacc>(accK[o][q], gs);
+#pragma unroll
+                for (int m = 0; m < NSK; ++m)
+                    Gc[o * opstride + (size_t)(bo[q] + m) * Ppad + g] = old[o][q][m] + gs[m];
+            }
     };
     // software pipeline: while an item computes, the next item's F rows and
     // (inside one block run, where nothing else writes them before its own
     // update) its G rows are already on their way
     int item = tl[t0];
     real fi_n[NSI];
-    gacc gi_n[NSI];
+    gacc gi_n[NOP][NSI];
     bool gi_ok = false;
     {
         const int x0 = groups[item >> 8].x0;
@@ -187,10 +201,14 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
         const int bblk = gr->bblk;
         const int x0 = gr->x0;
         real fi[NSI];
-        gacc gi[NSI];
+        gacc gi[NOP][NSI];
         const bool have_gi = gi_ok;
 #pragma unroll
-        for (int m = 0; m < NSI; ++m) { fi[m] = fi_n[m]; gi[m] = gi_n[m]; }
+        for (int m = 0; m < NSI; ++m) fi[m] = fi_n[m];
+#pragma unroll
+        for (int o = 0; o < NOP; ++o)
+#pragma unroll
+            for (int m = 0; m < NSI; ++m) gi[o][m] = gi_n[o][m];
         int item_n = 0;
         if (t + 1 < t1) {                                    // uniform
             item_n = tl[t + 1];
@@ -203,7 +221,10 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             gi_ok = gn->bblk == bblk;
             if (gi_ok) {
 #pragma unroll
-                for (int m = 0; m < NSI; ++m) gi_n[m] = Gc[(size_t)(x0n + m) * Ppad + g];
+                for (int o = 0; o < NOP; ++o)
+#pragma unroll
+                    for (int m = 0; m < NSI; ++m)
+                        gi_n[o][m] = Gc[o * opstride + (size_t)(x0n + m) * Ppad + g];
             }
         }
         if (bblk != curB) {
@@ -223,14 +244,18 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             for (int q = 0; q < BB; ++q) {
                 fcart<LK, real>(fk[q], FkC[q]);
 #pragma unroll
-                for (int c = 0; c < NCK; ++c) accK[q][c] = (gacc)0;
+                for (int o = 0; o < NOP; ++o)
+#pragma unroll
+                    for (int c = 0; c < NCK; ++c) accK[o][q][c] = (gacc)0;
             }
         }
         real FiC[NCI];
         fcart<LI, real>(fi, FiC);
-        gacc accI[NCI];
+        gacc accI[NOP][NCI];
 #pragma unroll
-        for (int c = 0; c < NCI; ++c) accI[c] = (gacc)0;
+        for (int o = 0; o < NOP; ++o)
+#pragma unroll
+            for (int c = 0; c < NCI; ++c) accI[o][c] = (gacc)0;
         bool anyI = false;
         // the group's packed task words: no dependent pair-record load
         int pw[BB];
@@ -241,87 +266,104 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             if (!(mask >> q & 1u)) continue;                 // uniform
             const int pp0 = pw[q] >> 8, npp = (pw[q] >> 2) & 63;
             const bool onec = pw[q] & 2, diag = pw[q] & 1;
-            vacc Vs[NCA * NCB];
+            vacc Vs[NOP * NV];
 #pragma unroll
-            for (int i = 0; i < NCA * NCB; ++i) Vs[i] = (vacc)0;
+            for (int i = 0; i < NOP * NV; ++i) Vs[i] = (vacc)0;
             // one-centre pairs take the variant whose FP32 E products all
             // carry their rounding remainders
-            if (onec) md_block<LA, LB, true, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
-            else md_block<LA, LB, false, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
-            real V[NCA * NCB];
-#pragma unroll
-            for (int i = 0; i < NCA * NCB; ++i) V[i] = (real)Vs[i];
+            if (onec) md_block<LA, LB, true, NOP, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
+            else md_block<LA, LB, false, NOP, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
             // G[B] += V^T F[A], G[A] += V F[B]; for a diagonal pair (A = B,
             // blocked side B) only the first
             const real* FaC = BA ? FkC[q] : FiC;
             const real* FbC = BA ? FiC : FkC[q];
-            gacc* gA = BA ? accK[q] : accI;
-            gacc* gB = BA ? accI : accK[q];
 #pragma unroll
-            for (int bb = 0; bb < NCB; ++bb) {
-                real v = (real)0;
+            for (int o = 0; o < NOP; ++o) {
+                real V[NV];
 #pragma unroll
-                for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + bb], FaC[a], v);
-                gB[bb] += (gacc)v;
-            }
-            if (BA || !diag) {                               // uniform
+                for (int i = 0; i < NV; ++i) V[i] = (real)Vs[o * NV + i];
+                gacc* gA = BA ? accK[o][q] : accI[o];
+                gacc* gB = BA ? accI[o] : accK[o][q];
 #pragma unroll
-                for (int a = 0; a < NCA; ++a) {
+                for (int bb = 0; bb < NCB; ++bb) {
                     real v = (real)0;
 #pragma unroll
-                    for (int bb = 0; bb < NCB; ++bb) v = fma(V[a * NCB + bb], FbC[bb], v);
-                    gA[a] += (gacc)v;
+                    for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + bb], FaC[a], v);
+                    gB[bb] += (gacc)v;
+                }
+                if (BA || !diag) {                           // uniform
+#pragma unroll
+                    for (int a = 0; a < NCA; ++a) {
+                        real v = (real)0;
+#pragma unroll
+                        for (int bb = 0; bb < NCB; ++bb) v = fma(V[a * NCB + bb], FbC[bb], v);
+                        gA[a] += (gacc)v;
+                    }
                 }
             }
             anyI = anyI || BA || !diag;
         }
         if (anyI) {
-            gacc gs[NSI];
-            gsph<LI, gacc>(accI, gs);
             if (!have_gi) {
 #pragma unroll
-                for (int m = 0; m < NSI; ++m) gi[m] = Gc[(size_t)(x0 + m) * Ppad + g];
+                for (int o = 0; o < NOP; ++o)
+#pragma unroll
+                    for (int m = 0; m < NSI; ++m) gi[o][m] = Gc[o * opstride + (size_t)(x0 + m) * Ppad + g];
             }
 #pragma unroll
-            for (int m = 0; m < NSI; ++m) Gc[(size_t)(x0 + m) * Ppad + g] = gi[m] + gs[m];
+            for (int o = 0; o < NOP; ++o) {
+                gacc gs[NSI];
+                gsph<LI, gacc>(accI[o], gs);
+#pragma unroll
+                for (int m = 0; m < NSI; ++m)
+                    Gc[o * opstride + (size_t)(x0 + m) * Ppad + g] = gi[o][m] + gs[m];
+            }
         }
         item = item_n;
     }
     if (curB >= 0) flush_k();
 }
 
-template <int LA, int LB, typename real, typename gacc>
+template <int LA, int LB, int NOP, typename real, typename gacc>
 static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
-                        void* G, size_t gstride, int ncopy, const int* counts,
+                        void* G, size_t gstride, size_t opstride, int ncopy, const int* counts,
                         const int* ids, const void* recs, const void* boys)
 {
     const int workers = g.nsub * ncopy;
     const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
-    hipLaunchKernelGGL((md_k_kernel<LA, LB, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES), 0, s,
-                       g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, counts, ids,
-                       c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
+    hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES),
+                       0, s, g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, opstride,
+                       counts, ids, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
                        (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy);
     WB_HIP(hipGetLastError());
 }
 
-void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64,
-               const void* F, void* G, size_t gstride, int ncopy, const int* counts,
-               const int* ids, const void* recs, const void* boys)
+void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
+               const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
+               const int* counts, const int* ids, const void* recs, const void* boys)
 {
     if (c.npairs == 0) return;
-#define WB_CASE(XA, XB)                                                                \
-    if (c.la == XA && c.lb == XB) {                                                    \
-        if (f32 && !g64)                                                               \
-            md_launch_t<XA, XB, float, float>(s, c, g, F, G, gstride, ncopy, counts, ids, recs, boys); \
-        else if (f32)                                                                  \
-            md_launch_t<XA, XB, float, double>(s, c, g, F, G, gstride, ncopy, counts, ids, recs, boys); \
-        else                                                                           \
-            md_launch_t<XA, XB, double, double>(s, c, g, F, G, gstride, ncopy, counts, ids, recs, boys); \
-        return;                                                                        \
+    if (nop != 1 && !(nop == 2 && md_fuse(c.la, c.lb)))
+        throw Error(WB_INVALID, "operator count not available for this shell-pair class");
+#define WB_LAUNCH(XA, XB, NOPV)                                                                      \
+    if (f32 && !g64)                                                                                 \
+        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys);  \
+    else if (f32)                                                                                    \
+        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys); \
+    else                                                                                             \
+        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys);
+#define WB_CASE(XA, XB)                                                                    \
+    if (c.la == XA && c.lb == XB) {                                                        \
+        if constexpr (md_fuse(XA, XB)) {                                                   \
+            if (nop == 2) { WB_LAUNCH(XA, XB, 2) return; }                                 \
+        }                                                                                  \
+        WB_LAUNCH(XA, XB, 1)                                                               \
+        return;                                                                            \
     }
     WB_CASE(0, 0) WB_CASE(1, 0) WB_CASE(1, 1) WB_CASE(2, 0) WB_CASE(2, 1)
     WB_CASE(2, 2) WB_CASE(3, 0) WB_CASE(3, 1) WB_CASE(3, 2) WB_CASE(3, 3)
 #undef WB_CASE
+#undef WB_LAUNCH
     throw Error(WB_UNSUPPORTED, "shell-pair class beyond (f, f)");
 }
 

@@ -7,7 +7,9 @@ Same removable hooks as the reference (gpu/install.py:209-336):
 
 The K hook dispatches only for ``mf.with_df`` and the long-range copies in
 ``mf.with_df._rsh_df`` (gpu/install.py:300-304); every other SGX object
-keeps the original.  Any ``Unsupported`` input or GPU error emits a
+keeps the original.  The operators of one SCF step (full range and the
+functional's range-separated ones) are evaluated together at the first
+request and the others served from that result (sgx_k.rsh_companions).  Any ``Unsupported`` input or GPU error emits a
 RuntimeWarning with the traceback and calls the original PySCF function
 (gpu/install.py:91-96); an out-of-memory error first retries with half the
 number of G partial copies (down to one), the memory lever of this design, then
@@ -139,9 +141,11 @@ def install_gpu(mf, backend=None, precision='mixed', vv10_tile_tol=0.0, k=False,
                 try:
                     if sgx.mol is not ctx.mol:
                         raise Unsupported('molecule changed since install_gpu')
+                    comp = sgx_k.rsh_companions(mf)
                     kk = run_with_oom_retry(
                         lambda: sgx_k.get_k_only_gpu(ctx.k_builder, sgx, dm, hermi,
-                                                     direct_scf_tol), ctx.set_copies)
+                                                     direct_scf_tol, companions=comp),
+                        ctx.set_copies)
                 except Exception as exc:  # noqa: BLE001
                     _warn_fallback('get_k_only', exc)
                     return orig(sgx, dm, hermi, direct_scf_tol)

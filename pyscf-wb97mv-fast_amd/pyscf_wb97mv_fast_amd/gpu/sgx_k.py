@@ -18,6 +18,14 @@ Device state (basis / shell pairs and, per grids object, the sorted grid)
 is cached; a rebuilt grids object gets a fresh state (stale-grid rule of
 gpu/sgx_k.py:856-864).  ``get_k_only_gpu`` reads omega from sgx.mol.omega
 (gpu/sgx_k.py:983-991).
+
+Companion operators: an RSH functional asks for K(1/r) and K(erf(w r)/r)
+of the same density on the same grid in two consecutive get_k_only calls.
+Given the operators to expect (``companions``), the first call evaluates
+all of them in one device call -- the exchange kernels of the light
+classes then run both operators in one pass over the tasks -- and keeps
+the raw K of the others; a later call with the same grid and the same
+Q dm (compared exactly) returns the kept one.
 """
 import time
 
@@ -43,6 +51,7 @@ class GpuKBuilder:
         self._device = device
         self._basis = None
         self._grids = {}
+        self._kcache = None
         self.last_stats = {}
 
     # -- device state ---------------------------------------------------------
@@ -92,7 +101,21 @@ class GpuKBuilder:
                            'wall': time.perf_counter() - t0, 'kernel': self.ctx.last_ms * 1e-3}
         return out
 
-    def get_k(self, sgx, dm, omega, hermi=1, direct_scf_tol=1e-13):
+    def k_raw_shared(self, grids, qd, omega, companions=()):
+        """Raw K for omega; with companions, all of them in one device call,
+        the others kept for a later request with the same grids and qd."""
+        c = self._kcache
+        if (c is not None and c['grids'] is grids and omega in c['K']
+                and c['qd'].shape == qd.shape and numpy.array_equal(c['qd'], qd)):
+            self.last_stats = dict(c['stats'], cached=True)
+            return c['K'].pop(omega)
+        oms = [float(omega)] + [float(w) for w in companions if float(w) != float(omega)]
+        Ks = self.k_raw(grids, qd, oms)
+        self._kcache = {'grids': grids, 'qd': numpy.array(qd, copy=True),
+                        'K': dict(zip(oms[1:], Ks[1:])), 'stats': dict(self.last_stats)}
+        return Ks[0]
+
+    def get_k(self, sgx, dm, omega, hermi=1, direct_scf_tol=1e-13, companions=()):
         """K (nao, nao) FP64 for one RKS density matrix on sgx's grid."""
         if hermi != 1:
             raise Unsupported('GPU COSX K supports hermi=1 only')
@@ -111,7 +134,7 @@ class GpuKBuilder:
             pd = sgx._pjs_data
         proj = numpy.asarray(pd._overlap_correction_matrix, dtype=numpy.float64)
         sym = bool(pd.sym_ovlp)
-        K = self.k_raw(grids, proj @ dm, [omega])[0]
+        K = self.k_raw_shared(grids, proj @ dm, omega, companions)
         if sym:
             K = proj.T @ K
         if hermi == 1:
@@ -119,8 +142,31 @@ class GpuKBuilder:
         return K
 
 
-def get_k_only_gpu(builder, sgx, dm, hermi=1, direct_scf_tol=1e-13):
+def get_k_only_gpu(builder, sgx, dm, hermi=1, direct_scf_tol=1e-13, companions=()):
     """pyscf.sgx.sgx_jk.get_k_only's signature and contract for one RKS dm;
     omega from sgx.mol.omega (set by mol.with_range_coulomb)."""
     omega = float(getattr(sgx.mol, 'omega', 0.0) or 0.0)
-    return builder.get_k(sgx, dm, omega, hermi=hermi, direct_scf_tol=direct_scf_tol)
+    return builder.get_k(sgx, dm, omega, hermi=hermi, direct_scf_tol=direct_scf_tol,
+                         companions=companions)
+
+
+def rsh_companions(mf):
+    """The exchange operators an SCF step of mf asks for: the full-range
+    operator of mf.with_df, the range-separated copies it holds
+    (_rsh_df keys, '%.6f' % omega as core/sgx_patch.py:39-48 makes them)
+    and, before those exist, the omega of mf's functional."""
+    oms = {0.0}
+    df = getattr(mf, 'with_df', None)
+    for key in (getattr(df, '_rsh_df', None) or {}):
+        try:
+            oms.add(float(key))
+        except (TypeError, ValueError):
+            pass
+    ni = getattr(mf, '_numint', None)
+    try:
+        omega = float(ni.rsh_coeff(mf.xc)[0])
+        if omega:
+            oms.add(omega)
+    except Exception:                     # noqa: BLE001 -- no RSH information
+        pass
+    return tuple(sorted(oms))

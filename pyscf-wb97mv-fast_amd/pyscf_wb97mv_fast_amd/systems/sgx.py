@@ -38,7 +38,15 @@ class SyntheticSGX:
 WB97MV_NLC = ((6.0, 0.01), 1.0)
 
 
+WB97MV_RSH = (0.3, 1.0, 0.15)      # (omega, alpha, hyb): E_x = hyb K + (alpha - hyb) K_lr
+
+
 class SyntheticNumInt:
+    def rsh_coeff(self, xc_code):
+        if 'wb97m' not in str(xc_code).lower().replace('_', '').replace('-', ''):
+            return (0.0, 0.0, 0.0)
+        return WB97MV_RSH
+
     def nlc_coeff(self, xc_code):
         if 'wb97m' not in str(xc_code).lower().replace('_', '').replace('-', ''):
             return []

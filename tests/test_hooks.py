@@ -140,3 +140,50 @@ def test_no_gpu_installs_nothing():
         hooks = install.install_gpu(mf)
     assert 'nr_nlc_vxc' not in vars(mf._numint)
     hooks.restore_all()
+
+
+class _RecordingBuilder:
+    """GpuKBuilder with the device call replaced by a record of what was
+    asked (raw K = omega * I + trace(qd) so results are distinguishable)."""
+
+    def __new__(cls):
+        from pyscf_wb97mv_fast_amd.gpu import sgx_k
+
+        class B(sgx_k.GpuKBuilder):
+            def __init__(self):
+                super().__init__(mol=None, ctx=object())
+                self.calls = []
+
+            def k_raw(self, grids, qd, omegas):
+                self.calls.append(list(omegas))
+                return numpy.array([w * numpy.eye(qd.shape[0]) + numpy.trace(qd)
+                                    for w in omegas])
+        return B()
+
+
+def test_companion_operators_evaluated_together_and_cached():
+    b = _RecordingBuilder()
+    g1, g2 = object(), object()
+    qd = numpy.diag([1.0, 2.0])
+    k0 = b.k_raw_shared(g1, qd, 0.0, companions=(0.0, 0.3))
+    assert b.calls == [[0.0, 0.3]]
+    k1 = b.k_raw_shared(g1, qd.copy(), 0.3, companions=(0.0, 0.3))
+    assert b.calls == [[0.0, 0.3]]                           # served from the first call
+    assert numpy.allclose(k1, 0.3 * numpy.eye(2) + 3.0) and numpy.allclose(k0, 3.0)
+    b.k_raw_shared(g1, qd, 0.3, companions=(0.0, 0.3))        # taken once only
+    assert b.calls[-1] == [0.3, 0.0]
+    b.k_raw_shared(g1, qd + 1e-15, 0.0, companions=(0.0, 0.3))  # another density
+    assert b.calls[-1] == [0.0, 0.3]
+    b.k_raw_shared(g2, qd + 1e-15, 0.3, companions=(0.0, 0.3))  # another grid
+    assert b.calls[-1] == [0.3, 0.0]
+
+
+def test_rsh_companions_from_functional_and_rsh_copies():
+    from pyscf_wb97mv_fast_amd.gpu import sgx_k
+    mf = types.SimpleNamespace(xc='wb97m-v', _numint=sgx.SyntheticNumInt(),
+                               with_df=types.SimpleNamespace(_rsh_df={}))
+    assert sgx_k.rsh_companions(mf) == (0.0, 0.3)
+    mf.with_df._rsh_df['%.6f' % 0.4] = object()
+    assert sgx_k.rsh_companions(mf) == (0.0, 0.3, 0.4)
+    mf2 = types.SimpleNamespace(xc='pbe', _numint=sgx.SyntheticNumInt(), with_df=None)
+    assert sgx_k.rsh_companions(mf2) == (0.0,)
