@@ -550,7 +550,9 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     char* Gb = (char*)c.ws_G.ensure(gsz * opstride * nop);
     const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
     c.mark(PH_SCREEN);
-    zero_bytes(s, Gb, gsz * opstride * nop);
+    // G needs no separate clearing pass: the first class launched for an
+    // operator clears each wave's rows (all nao + MD_SPARE) before it adds
+    int zrows = nao + MD_SPARE;
     for (int k = 0; k < NCLASS; ++k) {
         ClassData& C = B.cls[k];
         if (!C.npairs) continue;
@@ -559,15 +561,18 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
         if (nop == 2 && md_fuse(C.la, C.lb)) {
             // both operators in one pass over the tasks
             const void* rc = op_consts(C, {omegas[0], omegas[1]}, f32, s);
-            launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, rc, boys);
+            launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, rc, boys,
+                      zrows);
         } else {
             for (int op = 0; op < nop; ++op) {
                 const void* rc = op_consts(C, {omegas[op]}, f32, s);
                 launch_md(s, C, G, f32, g64, 1, F, Gb + gsz * opstride * op, gstride, opstride,
-                          ncopy, cnt, ids_k, rc, boys);
+                          ncopy, cnt, ids_k, rc, boys, zrows);
             }
         }
+        zrows = 0;
     }
+    if (zrows) zero_bytes(s, Gb, gsz * opstride * nop);   // no shell pair at all
     c.mark(PH_MD);
     for (int op = 0; op < nop; ++op) {
         gemm_nt_fast(s, nao, nao, Ppad, X, Gb + gsz * opstride * op, !g64, ncopy, gstride,

@@ -170,10 +170,13 @@ void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float*
 void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
                       const void* b32, const void* b64, double* out);
 // recs: the class's primitive-pair records for nop operators (md_rec_bytes);
-// nop = 2 only where md_fuse; G of operator o at G + o opstride (elements)
+// nop = 2 only where md_fuse; G of operator o at G + o opstride (elements).
+// zero_rows > 0: each wave first clears rows [0, zero_rows) of its sub-block
+// in its copy of the nop operators' G (the first launch of a K build)
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
                const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
-               const int* counts, const int* ids, const void* recs, const void* boys);
+               const int* counts, const int* ids, const void* recs, const void* boys,
+               int zero_rows);
 
 // VV10
 void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T,

@@ -107,7 +107,8 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  const int* __restrict__ counts, const int* __restrict__ ids, int cap,
                  const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
                  const char* __restrict__ recs,
-                 const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk)
+                 const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk,
+                 int zero_rows)
 {
     constexpr int L = LA + LB;
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
@@ -133,13 +134,21 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     const int worker = blockIdx.x * MD_WAVES + wave;
     if (worker >= nsub * nchunk) return;
     const int s = worker % nsub, j = worker / nsub;
+    gacc* const Gc = G + (size_t)j * gstride;         // operator o: Gc + o opstride
+    const int g = s * SUB + lane;
+    // the first class launched of a K build clears this wave's part of its
+    // operators' G copies (rows x 64 points, owned by this wave alone in
+    // every class launch) instead of a separate fill pass over all copies
+    if (zero_rows > 0) {
+#pragma unroll
+        for (int o = 0; o < NOP; ++o)
+            for (int r = 0; r < zero_rows; ++r) Gc[o * opstride + (size_t)r * Ppad + g] = (gacc)0;
+    }
     const int cnt = counts[s];
     const int t0 = (int)(((long long)cnt * j) / nchunk);
     const int t1 = (int)(((long long)cnt * (j + 1)) / nchunk);
     if (t0 >= t1) return;
     const int* tl = ids + (size_t)s * cap;
-    gacc* const Gc = G + (size_t)j * gstride;         // operator o: Gc + o opstride
-    const int g = s * SUB + lane;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
     // wait for the point's coordinates here, once: otherwise the first use
     // inside the item loop waits on the vector-memory counter in program
@@ -327,31 +336,32 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 template <int LA, int LB, int NOP, typename real, typename gacc>
 static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
                         void* G, size_t gstride, size_t opstride, int ncopy, const int* counts,
-                        const int* ids, const void* recs, const void* boys)
+                        const int* ids, const void* recs, const void* boys, int zero_rows)
 {
     const int workers = g.nsub * ncopy;
     const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
     hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES),
                        0, s, g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, opstride,
                        counts, ids, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
-                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy);
+                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy, zero_rows);
     WB_HIP(hipGetLastError());
 }
 
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
                const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
-               const int* counts, const int* ids, const void* recs, const void* boys)
+               const int* counts, const int* ids, const void* recs, const void* boys,
+               int zero_rows)
 {
     if (c.npairs == 0) return;
     if (nop != 1 && !(nop == 2 && md_fuse(c.la, c.lb)))
         throw Error(WB_INVALID, "operator count not available for this shell-pair class");
 #define WB_LAUNCH(XA, XB, NOPV)                                                                      \
     if (f32 && !g64)                                                                                 \
-        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys);  \
+        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows);  \
     else if (f32)                                                                                    \
-        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys); \
+        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows); \
     else                                                                                             \
-        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys);
+        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows);
 #define WB_CASE(XA, XB)                                                                    \
     if (c.la == XA && c.lb == XB) {                                                        \
         if constexpr (md_fuse(XA, XB)) {                                                   \
