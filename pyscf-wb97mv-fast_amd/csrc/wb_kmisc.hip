@@ -312,13 +312,21 @@ void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A, const v
 // chosen per call to minimize padding, e.g. nao = 648 -> 7 x 96 rather than
 // 6 x 128).  BK = 16; the next K slice is fetched into registers while the
 // current one is consumed from LDS (two LDS buffers, one barrier per slice).
-// (Two slices in flight measured slower: 3.8 -> 5.2 ms for the nao x P x nao
-// product, from the extra registers.)
+// Global loads are 16 bytes per lane, 8 (k-contiguous operand) or BN/2
+// (n-contiguous operand) consecutive lanes per row.  A k-contiguous slice
+// sits in LDS as rows of eight 16-byte chunks, chunk ch of row r at position
+// ch ^ ((r >> 1) & 7): the 16 rows an MFMA operand read touches then cover
+// all 64 banks.  The k order inside a slice is permuted, identically for
+// both operands: lane (c, kq) of MFMA step s takes k = 4 kq + s, i.e. two
+// 128-bit LDS reads per row and slice.
 // Block ids are remapped so that consecutive work items run on the same XCD
 // (blocks are dealt round-robin over the 8 XCDs): the tiles of one K split
 // then share its operand slices in one L2.
-constexpr int T_K = 16, T_PAD = 2;
+// (Water27, 648 x 218624 x 648: K GEMM 4.08 -> 3.24 ms, F GEMM 3.59 -> 3.11
+// ms against the previous 8-byte-load, unswizzled version.)
+constexpr int T_K = 16;
 constexpr int NXCD = 8;
+typedef double dv2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks)
 {
@@ -330,80 +338,66 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks)
     return base + local;
 }
 
-// Row-major rows x K (k contiguous) slice [r0, r0+BR) x [k0, k0+16) into
-// registers: thread t < 2 BR holds row t/2, k half (t%2)*8 .. +8.
+// LDS position of 16-byte chunk ch (0..7) of row r of a k-contiguous slice
+__device__ __forceinline__ int kslice_pos(int r, int ch) { return r * 8 + (ch ^ ((r >> 1) & 7)); }
+
+// k-contiguous [rows][lda] operand, slice rows [r0, r0 + BR) x k [k0, k0 + 16):
+// thread t holds chunks id = i 256 + t (row id / 8, chunk id % 8).  Rows past
+// the end are clamped to the last row (they feed only output rows or columns
+// that are never stored); chunks at or past kend read as zero (kend even).
 template <int BR>
-__device__ __forceinline__ void fetch_rows(const double* __restrict__ A, int lda, int rows,
-                                           int kend, int r0, int k0, double (&v)[8])
+__device__ __forceinline__ void fetch_kslice(const double* __restrict__ A, int lda, int rows,
+                                             int kend, int r0, int k0, dv2 (&v)[BR / 32])
 {
-    const int t = threadIdx.x;
-    if (t >= 2 * BR) return;
-    const int r = r0 + (t >> 1), kb = k0 + (t & 1) * 8;
-    if (r < rows && kb + 8 <= kend) {
-        const double2* p = (const double2*)(A + (size_t)r * lda + kb);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) { double2 q = p[i]; v[2 * i] = q.x; v[2 * i + 1] = q.y; }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = (r < rows && kb + i < kend) ? A[(size_t)r * lda + kb + i] : 0.0;
+    for (int i = 0; i < BR / 32; ++i) {
+        const int id = i * 256 + threadIdx.x;
+        const int r = min(r0 + (id >> 3), rows - 1), k = k0 + 2 * (id & 7);
+        const dv2 t = *(const dv2*)(A + (size_t)r * lda + min(k, kend - 2));
+        v[i] = k < kend ? t : (dv2){0.0, 0.0};
     }
 }
 template <int BR>
-__device__ __forceinline__ void store_rows(double (*S)[BR + T_PAD], const double (&v)[8])
+__device__ __forceinline__ void store_kslice(dv2* __restrict__ S, const dv2 (&v)[BR / 32])
 {
-    const int t = threadIdx.x;
-    if (t >= 2 * BR) return;
-    const int r = t >> 1, kq = (t & 1) * 8;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) S[kq + i][r] = v[i];
-}
-// K x N (n contiguous) slice [k0, k0+16) x [n0, n0+BC): thread t < 2 BC
-// holds k row t/(BC/8), n columns (t % (BC/8)) * 8 .. +8
-template <int BC>
-__device__ __forceinline__ void fetch_cols(const double* __restrict__ B, int ldb, int kdim,
-                                           int ncols, int k0, int n0, double (&v)[8])
-{
-    const int t = threadIdx.x;
-    if (t >= 2 * BC) return;
-    constexpr int PER = BC / 8;
-    const int k = k0 + t / PER, nb = n0 + (t % PER) * 8;
-    if (k < kdim && nb + 8 <= ncols) {
-        const double2* p = (const double2*)(B + (size_t)k * ldb + nb);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) { double2 q = p[i]; v[2 * i] = q.x; v[2 * i + 1] = q.y; }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = (k < kdim && nb + i < ncols) ? B[(size_t)k * ldb + nb + i] : 0.0;
+    for (int i = 0; i < BR / 32; ++i) {
+        const int id = i * 256 + threadIdx.x;
+        S[kslice_pos(id >> 3, id & 7)] = v[i];
     }
 }
-template <int BC>
-__device__ __forceinline__ void store_cols(double (*S)[BC + T_PAD], const double (&v)[8])
+// the two chunks (k = 4 kq .. 4 kq + 3) of row r that lane (c, kq) feeds to MFMA
+__device__ __forceinline__ void read_kslice(const dv2* __restrict__ S, int r, int kq, dv2 (&x)[2])
 {
-    const int t = threadIdx.x;
-    if (t >= 2 * BC) return;
-    constexpr int PER = BC / 8;
-    const int k = t / PER, nq = (t % PER) * 8;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) S[k][nq + i] = v[i];
+    x[0] = S[kslice_pos(r, 2 * kq)];
+    x[1] = S[kslice_pos(r, 2 * kq + 1)];
 }
 
-template <int WM, int WN>
-__device__ __forceinline__ void mma_slice(const double (*As)[32 * WM + T_PAD],
-                                          const double (*Bs)[32 * WN + T_PAD], int wm, int wn,
-                                          d4 (&acc)[WM][WN])
+// n-contiguous [K][ldb] operand, slice k [k0, k0 + 16) x n [n0, n0 + BN) into
+// LDS rows of BN + 4 doubles (a pitch of 8 mod 16 dwords puts rows 4 apart in
+// opposite bank halves).  k rows past K are clamped to the last row (the
+// k-contiguous operand is zero there); n past N is clamped likewise (columns
+// never stored).
+template <int BN>
+__device__ __forceinline__ void fetch_nslice(const double* __restrict__ B, int ldb, int K, int N,
+                                             int k0, int n0, dv2 (&v)[BN / 32])
 {
-    const int lane = threadIdx.x & 63, c = lane & 15, kq = lane >> 4;
+    constexpr int CPR = BN / 2;
 #pragma unroll
-    for (int kk = 0; kk < T_K; kk += 4) {
-        double a[WM], b[WN];
+    for (int i = 0; i < BN / 32; ++i) {
+        const int id = i * 256 + threadIdx.x;
+        const int k = min(k0 + id / CPR, K - 1), n = min(n0 + 2 * (id % CPR), N - 2);
+        v[i] = *(const dv2*)(B + (size_t)k * ldb + n);
+    }
+}
+template <int BN>
+__device__ __forceinline__ void store_nslice(double* __restrict__ S, const dv2 (&v)[BN / 32])
+{
+    constexpr int CPR = BN / 2, PITCH = BN + 4;
 #pragma unroll
-        for (int i = 0; i < WM; ++i) a[i] = As[kk + kq][wm + i * 16 + c];
-#pragma unroll
-        for (int j = 0; j < WN; ++j) b[j] = Bs[kk + kq][wn + j * 16 + c];
-#pragma unroll
-        for (int i = 0; i < WM; ++i)
-#pragma unroll
-            for (int j = 0; j < WN; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+    for (int i = 0; i < BN / 32; ++i) {
+        const int id = i * 256 + threadIdx.x;
+        *(dv2*)(S + (id / CPR) * PITCH + 2 * (id % CPR)) = v[i];
     }
 }
 
@@ -424,47 +418,58 @@ __device__ __forceinline__ void store_tile(OutT* __restrict__ C, int ldc, int M,
             }
 }
 
-// C[M][N] = A[M][K] B[K][N]; m tiles fastest (they share the B slice).
-// One block per CU: the compiler then keeps the accumulators in AGPRs and
-// schedules the slice better (3.8 -> 3.3-3.5 ms for nao x P x nao, 648 x
-// 218624 x 648; the split-K NT kernel below is faster at two)
+// C[M][N] = A[M][K] B[K][N] (A k-contiguous, B n-contiguous; K, N even);
+// m tiles fastest (they share the B slice)
 template <int WM, int WN, typename OutT>
-__global__ __launch_bounds__(256, 1)
+__global__ __launch_bounds__(256, 2)
 void gemm_nn_t_kernel(int M, int N, int K, const double* __restrict__ A,
                       const double* __restrict__ B, OutT* __restrict__ C, int mt)
 {
-    constexpr int BM = 32 * WM, BN = 32 * WN;
-    __shared__ double As[2][T_K][BM + T_PAD];
-    __shared__ double Bs[2][T_K][BN + T_PAD];
+    constexpr int BM = 32 * WM, BN = 32 * WN, PITCH = BN + 4;
+    __shared__ dv2 As[2][BM * 8];
+    __shared__ double Bs[2][T_K * PITCH];
     const int w = xcd_remap(blockIdx.x, gridDim.x);
     const int m0 = (w % mt) * BM, n0 = (w / mt) * BN;
-    const int wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
+    const int c = lane & 15, kq = lane >> 4;
     d4 acc[WM][WN];
 #pragma unroll
     for (int i = 0; i < WM; ++i)
 #pragma unroll
         for (int j = 0; j < WN; ++j) acc[i][j] = (d4){0, 0, 0, 0};
-    double va[8], vb[8];
-    fetch_rows<BM>(A, K, M, K, m0, 0, va);
-    fetch_cols<BN>(B, N, K, N, 0, n0, vb);
+    dv2 va[BM / 32], vb[BN / 32];
+    fetch_kslice<BM>(A, K, M, K, m0, 0, va);
+    fetch_nslice<BN>(B, N, K, N, 0, n0, vb);
     int buf = 0;
     for (int k0 = 0; k0 < K; k0 += T_K) {
-        store_rows<BM>(As[buf], va);
-        store_cols<BN>(Bs[buf], vb);
+        store_kslice<BM>(As[buf], va);
+        store_nslice<BN>(Bs[buf], vb);
         __syncthreads();
         if (k0 + T_K < K) {
-            fetch_rows<BM>(A, K, M, K, m0, k0 + T_K, va);
-            fetch_cols<BN>(B, N, K, N, k0 + T_K, n0, vb);
+            fetch_kslice<BM>(A, K, M, K, m0, k0 + T_K, va);
+            fetch_nslice<BN>(B, N, K, N, k0 + T_K, n0, vb);
         }
-        mma_slice<WM, WN>(As[buf], Bs[buf], wm, wn, acc);
+        dv2 a[WM][2];
+#pragma unroll
+        for (int i = 0; i < WM; ++i) read_kslice(As[buf], wm + 16 * i + c, kq, a[i]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double b[WN];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) b[j] = Bs[buf][(4 * kq + s) * PITCH + wn + 16 * j + c];
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[i][j] = mfma64(a[i][s >> 1][s & 1], b[j], acc[i][j]);
+        }
         buf ^= 1;
     }
     store_tile<WM, WN>(C, N, M, N, m0, n0, wm, wn, acc);
 }
 
-// Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k]; tiles fastest within a
-// split (they share the A and B slices of that split)
+// Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k] (both k-contiguous, K
+// even); tiles fastest within a split (they share the A and B slices)
 template <int WM, int WN>
 __global__ __launch_bounds__(256, 2)
 void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
@@ -472,32 +477,44 @@ void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
                       double* __restrict__ Cp)
 {
     constexpr int BM = 32 * WM, BN = 32 * WN;
-    __shared__ double As[2][T_K][BM + T_PAD];
-    __shared__ double Bs[2][T_K][BN + T_PAD];
+    __shared__ dv2 As[2][BM * 8];
+    __shared__ dv2 Bs[2][BN * 8];
     const int w = xcd_remap(blockIdx.x, gridDim.x);
     const int tile = w % ntile, z = w / ntile;
     const int m0 = (tile % mt) * BM, n0 = (tile / mt) * BN;
     const int kb = z * kchunk, ke = min(K, kb + kchunk);
-    const int wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
+    const int c = lane & 15, kq = lane >> 4;
     d4 acc[WM][WN];
 #pragma unroll
     for (int i = 0; i < WM; ++i)
 #pragma unroll
         for (int j = 0; j < WN; ++j) acc[i][j] = (d4){0, 0, 0, 0};
-    double va[8], vb[8];
-    fetch_rows<BM>(A, K, M, ke, m0, kb, va);
-    fetch_rows<BN>(B, K, N, ke, n0, kb, vb);
+    dv2 va[BM / 32], vb[BN / 32];
+    fetch_kslice<BM>(A, K, M, ke, m0, kb, va);
+    fetch_kslice<BN>(B, K, N, ke, n0, kb, vb);
     int buf = 0;
     for (int k0 = kb; k0 < ke; k0 += T_K) {
-        store_rows<BM>(As[buf], va);
-        store_rows<BN>(Bs[buf], vb);
+        store_kslice<BM>(As[buf], va);
+        store_kslice<BN>(Bs[buf], vb);
         __syncthreads();
         if (k0 + T_K < ke) {
-            fetch_rows<BM>(A, K, M, ke, m0, k0 + T_K, va);
-            fetch_rows<BN>(B, K, N, ke, n0, k0 + T_K, vb);
+            fetch_kslice<BM>(A, K, M, ke, m0, k0 + T_K, va);
+            fetch_kslice<BN>(B, K, N, ke, n0, k0 + T_K, vb);
         }
-        mma_slice<WM, WN>(As[buf], Bs[buf], wm, wn, acc);
+        dv2 a[WM][2], b[WN][2];
+#pragma unroll
+        for (int i = 0; i < WM; ++i) read_kslice(As[buf], wm + 16 * i + c, kq, a[i]);
+#pragma unroll
+        for (int j = 0; j < WN; ++j) read_kslice(Bs[buf], wn + 16 * j + c, kq, b[j]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j)
+                    acc[i][j] = mfma64(a[i][s >> 1][s & 1], b[j][s >> 1][s & 1], acc[i][j]);
         buf ^= 1;
     }
     store_tile<WM, WN>(Cp + (size_t)z * M * N, N, M, N, m0, n0, wm, wn, acc);
@@ -550,9 +567,15 @@ static int wave_tiles(int D)
     return p96 < p128 ? 3 : 4;
 }
 
+static bool aligned16(const void* p) { return ((uintptr_t)p % 16) == 0; }
+
 void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B, void* C,
                   bool out_f32)
 {
+    if (K % 2 || N % 2 || !aligned16(A) || !aligned16(B)) {   // 16-byte loads impossible
+        gemm_nn(s, M, N, K, A, B, C, out_f32);
+        return;
+    }
     const int wm = wave_tiles(M);
     const int mt = (M + 32 * wm - 1) / (32 * wm), nt = (N + 127) / 128;
     dim3 grid(mt * nt);
@@ -567,6 +590,10 @@ void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const dou
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B, bool b_f32,
                   int nb, size_t bstride, const double* w, double* C, DevBuf& ws)
 {
+    if (K % 2 || !aligned16(A)) {                 // 16-byte loads impossible
+        gemm_nt_reduce(s, M, N, K, A, B, b_f32, nb, bstride, w, C, ws);
+        return;
+    }
     // fold the copies / weights into one FP64 operand when needed
     const double* Bd = (const double*)B;
     const size_t nbk = (size_t)N * K;
@@ -576,13 +603,13 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
     int kchunk = (K + nsplit - 1) / nsplit;
     kchunk = (kchunk + T_K - 1) / T_K * T_K;
     nsplit = (K + kchunk - 1) / kchunk;
-    const bool fold = b_f32 || nb != 1 || w != nullptr;
+    const bool fold = b_f32 || nb != 1 || w != nullptr || !aligned16(B);
     const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
     char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * nbk : 0));
     double* Cp = (double*)base;
     if (fold) {
         double* Bf = (double*)(base + part_bytes);
-        const bool vec = K % 4 == 0 && bstride % 4 == 0 && ((uintptr_t)B % 16) == 0;
+        const bool vec = K % 4 == 0 && bstride % 4 == 0 && aligned16(B);
         const unsigned blocks = (unsigned)((vec ? (nbk / 4 + 255) / 256 : (nbk + 255) / 256));
 #define WB_FOLD(T, V) hipLaunchKernelGGL((fold_copies_kernel<T, V>), dim3(blocks), dim3(256), 0, s, \
                                          nbk, K, (const T*)B, nb, bstride, w, Bf)

@@ -33,7 +33,8 @@ def _system(basis='def2-svp', level=0, npts=2500, seed=0, atoms=None):
 
 
 @pytest.mark.parametrize('kind', [0, 1, 2, 3])
-@pytest.mark.parametrize('shape', [(16, 64, 4), (48, 1536, 48), (100, 70, 333), (648, 640, 648)])
+@pytest.mark.parametrize('shape', [(16, 64, 4), (48, 1536, 48), (100, 70, 333), (648, 640, 648),
+                                   (130, 2002, 46), (97, 129, 4098), (200, 648, 20480)])
 def test_fp64_mfma_gemm(ctx, kind, shape):
     M, N, K = shape
     rng = numpy.random.default_rng(M + N + K)
