@@ -332,6 +332,10 @@ static void upload_basis(Basis& B, hipStream_t s)
         shao[B.nshell + i] = 2 * B.shells[i].l + 1;
     }
     upload(B.d_shao, shao.data(), sizeof(int) * shao.size(), s);
+    std::vector<int> row0(B.nao);
+    for (int i = 0; i < B.nshell; ++i)
+        for (int m = 0; m < 2 * B.shells[i].l + 1; ++m) row0[B.shells[i].ao0 + m] = B.shells[i].ao0;
+    upload(B.d_row0, row0.data(), sizeof(int) * row0.size(), s);
     for (auto& C : B.cls) {
         if (!C.npairs) continue;
         upload(C.d_pairs, C.pairs.data(), sizeof(PairRec) * C.npairs, s);
@@ -543,40 +547,55 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     }
     c.tasks_total = (long long)ntask;
     const int ncopy = c.ncopy;
-    const size_t gstride = (size_t)(nao + MD_SPARE) * Ppad;
+    // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),
+    // one zero row (read for a shell's first update in a region)
+    const int zrow = nao + MD_SPARE;
+    const size_t gstride = (size_t)(zrow + 1) * Ppad;
     const size_t opstride = gstride * ncopy;              // elements between operators' G
     const bool g64 = !f32 || c.g64;
     const size_t gsz = g64 ? 8 : 4;
     char* Gb = (char*)c.ws_G.ensure(gsz * opstride * nop);
     const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
     c.mark(PH_SCREEN);
-    // G needs no separate clearing pass: the first class launched for an
-    // operator clears each wave's rows (all nao + MD_SPARE) before it adds
-    int zrows = nao + MD_SPARE;
+    // G is never cleared: written-shell masks per (copy, sub-block) region
+    // send each shell's first update to the zero row (rows < 2048; larger
+    // bases clear G in the first class launch instead)
+    const bool tracked = zrow <= 32 * MD_MASK_WORDS;
+    const size_t mopstride = (size_t)ncopy * G.nsub * MD_MASK_WORDS;
+    unsigned* masks = tracked ? (unsigned*)c.ws_mask.ensure(sizeof(unsigned) * mopstride * nop)
+                              : nullptr;
+    bool first = true;
     for (int k = 0; k < NCLASS; ++k) {
         ClassData& C = B.cls[k];
         if (!C.npairs) continue;
         const int* cnt = counts + (size_t)k * G.nsub;
         const int* ids_k = ids + cls_off[k];
+        const int zrows = first ? zrow : 0;
         if (nop == 2 && md_fuse(C.la, C.lb)) {
             // both operators in one pass over the tasks
             const void* rc = op_consts(C, {omegas[0], omegas[1]}, f32, s);
+            MdMasks mk{masks, mopstride, zrow, first};
             launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, rc, boys,
-                      zrows);
+                      zrows, mk);
         } else {
             for (int op = 0; op < nop; ++op) {
                 const void* rc = op_consts(C, {omegas[op]}, f32, s);
+                MdMasks mk{tracked ? masks + mopstride * op : nullptr, mopstride, zrow, first};
                 launch_md(s, C, G, f32, g64, 1, F, Gb + gsz * opstride * op, gstride, opstride,
-                          ncopy, cnt, ids_k, rc, boys, zrows);
+                          ncopy, cnt, ids_k, rc, boys, zrows, mk);
             }
         }
-        zrows = 0;
+        first = false;
     }
-    if (zrows) zero_bytes(s, Gb, gsz * opstride * nop);   // no shell pair at all
+    if (first) {                                    // no shell pair at all
+        if (tracked) zero_bytes(s, masks, sizeof(unsigned) * mopstride * nop);
+        else zero_bytes(s, Gb, gsz * opstride * nop);
+    }
     c.mark(PH_MD);
     for (int op = 0; op < nop; ++op) {
         gemm_nt_fast(s, nao, nao, Ppad, X, Gb + gsz * opstride * op, !g64, ncopy, gstride,
-                     G.d_w.as<double>(), d_K + (size_t)op * nao * nao, c.ws_Kp);
+                     G.d_w.as<double>(), d_K + (size_t)op * nao * nao, c.ws_Kp,
+                     tracked ? masks + mopstride * op : nullptr, B.d_row0.as<int>(), G.nsub);
         c.mark(PH_KGEMM);
     }
     collect(c);

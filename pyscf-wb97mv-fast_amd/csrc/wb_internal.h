@@ -95,6 +95,7 @@ struct Basis {
     std::vector<ShellInfo> shells;
     std::vector<double> exps, coefs;
     DevBuf d_shells, d_exps, d_coefs, d_shao;   // d_shao: ao0[nshell], nao[nshell]
+    DevBuf d_row0;                              // int [nao]: first AO of the AO's shell
     ClassData cls[NCLASS];
     double pair_cut = 1e-14;
     long long npairs_total = 0;
@@ -119,7 +120,7 @@ struct Ctx {
     DevBuf d_b32, d_b64;               // Boys tables
     // workspaces (grow-only)
     DevBuf ws_X, ws_Xv, ws_F, ws_G, ws_tasks, ws_counts, ws_fmax, ws_Kp, ws_K, ws_D;
-    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io;
+    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io, ws_mask;
     // statistics of the last K build
     long long tasks_total = 0, tasks_kept = 0;
     float ms_last = 0.f;
@@ -153,12 +154,16 @@ void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A,
                     const void* B, bool b_f32, int nb, size_t bstride,
                     const double* w, double* C, DevBuf& ws);
 
-// 128 x 128 tile versions (the production paths); gemm_nt_fast folds the
-// copies and weights of B into one FP64 operand first when needed
+// MFMA tile versions (the production paths); gemm_nt_fast folds the copies
+// and weights of B into one FP64 operand first when needed.  masks (with
+// row0, the first row of each row's shell, and nsub): the exchange kernels'
+// written-shell masks of the copies (MdMasks layout, one operator): rows of a
+// copy's sub-block region that were never written are skipped (K = nsub * 64)
 void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B,
                   void* C, bool out_f32);
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B,
-                  bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws);
+                  bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
+                  const unsigned* masks = nullptr, const int* row0 = nullptr, int nsub = 0);
 
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
                  int Ppad, int nsub, const int* sh_ao0, const int* sh_nao, int nshell,
@@ -171,12 +176,24 @@ void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
                       const void* b32, const void* b64, double* out);
 // recs: the class's primitive-pair records for nop operators (md_rec_bytes);
 // nop = 2 only where md_fuse; G of operator o at G + o opstride (elements).
-// zero_rows > 0: each wave first clears rows [0, zero_rows) of its sub-block
-// in its copy of the nop operators' G (the first launch of a K build)
+// Written-shell masks of the G regions (MD_MASK_WORDS words per copy and
+// sub-block, [copy][sub-block][word], operator o at m + o opstride; a shell
+// marked at its first row): a region's first read-modify-write of a shell
+// reads the zero row zrow, so G is never cleared; first: the first class
+// launched in a K build (starts from empty masks and writes the zero row).
+// m = nullptr: no masks, and zero_rows > 0 makes each wave clear rows
+// [0, zero_rows) of its region first.
+constexpr int MD_MASK_WORDS = 64;                    // rows < 2048
+struct MdMasks {
+    unsigned* m = nullptr;
+    size_t opstride = 0;
+    int zrow = 0;
+    bool first = false;
+};
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
                const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
                const int* counts, const int* ids, const void* recs, const void* boys,
-               int zero_rows);
+               int zero_rows, const MdMasks& mk);
 
 // VV10
 void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T,

@@ -108,7 +108,8 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
                  const char* __restrict__ recs,
                  const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk,
-                 int zero_rows)
+                 int zero_rows, unsigned* __restrict__ masks, size_t mopstride, int zrow,
+                 int first)
 {
     constexpr int L = LA + LB;
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
@@ -136,18 +137,48 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     const int s = worker % nsub, j = worker / nsub;
     gacc* const Gc = G + (size_t)j * gstride;         // operator o: Gc + o opstride
     const int g = s * SUB + lane;
-    // the first class launched of a K build clears this wave's part of its
-    // operators' G copies (rows x 64 points, owned by this wave alone in
-    // every class launch) instead of a separate fill pass over all copies
-    if (zero_rows > 0) {
+    // This wave alone owns rows x its 64 points of copy j in every class
+    // launch.  With masks, G is never cleared: a bitmask of the shells this
+    // region has received (lane w holds rows 32w .. 32w + 31, a shell marked
+    // at its first row; carried between the class launches in masks) sends
+    // the first read-modify-write of a shell to the zero row zrow instead.
+    // Without masks, the first class launched clears the region.
+    const bool tracked = masks != nullptr;
+    unsigned* const mw = tracked ? masks + ((size_t)j * nsub + s) * 64 : nullptr;
+    unsigned trow = 0;
+    if (tracked) {
+        if (first) {
+#pragma unroll
+            for (int o = 0; o < NOP; ++o) Gc[o * opstride + (size_t)zrow * Ppad + g] = (gacc)0;
+        } else {
+            trow = mw[lane];
+        }
+    } else if (zero_rows > 0) {
 #pragma unroll
         for (int o = 0; o < NOP; ++o)
             for (int r = 0; r < zero_rows; ++r) Gc[o * opstride + (size_t)r * Ppad + g] = (gacc)0;
     }
+    auto store_mask = [&]() {
+#pragma unroll
+        for (int o = 0; o < NOP; ++o) mw[o * mopstride + lane] = trow;
+    };
+    // r: a shell's first row (wave-uniform); the row a read of its row r + m
+    // takes: itself once written, else the zero row
+    auto src_row = [&](int r, int m) -> int {
+        if (!tracked) return r + m;
+        const unsigned wv = (unsigned)__builtin_amdgcn_readlane((int)trow, r >> 5);
+        return (wv >> (r & 31) & 1u) ? r + m : zrow;
+    };
+    auto mark = [&](int r) {
+        if (tracked) trow |= (lane == (r >> 5)) ? (1u << (r & 31)) : 0u;
+    };
     const int cnt = counts[s];
     const int t0 = (int)(((long long)cnt * j) / nchunk);
     const int t1 = (int)(((long long)cnt * (j + 1)) / nchunk);
-    if (t0 >= t1) return;
+    if (t0 >= t1) {
+        if (tracked && first) store_mask();
+        return;
+    }
     const int* tl = ids + (size_t)s * cap;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
     // wait for the point's coordinates here, once: otherwise the first use
@@ -180,7 +211,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             for (int q = 0; q < BB; ++q)
 #pragma unroll
                 for (int m = 0; m < NSK; ++m)
-                    old[o][q][m] = Gc[o * opstride + (size_t)(bo[q] + m) * Ppad + g];
+                    old[o][q][m] = Gc[o * opstride + (size_t)src_row(bo[q], m) * Ppad + g];
 #pragma unroll
         for (int o = 0; o < NOP; ++o)
 #pragma unroll
@@ -191,6 +222,8 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                 for (int m = 0; m < NSK; ++m)
                     Gc[o * opstride + (size_t)(bo[q] + m) * Ppad + g] = old[o][q][m] + gs[m];
             }
+#pragma unroll
+        for (int q = 0; q < BB; ++q) mark(bo[q]);
     };
     // software pipeline: while an item computes, the next item's F rows and
     // (inside one block run, where nothing else writes them before its own
@@ -228,12 +261,14 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             // item shells within a run are distinct and a run change
             // flushes the block first: only a same-run prefetch is valid
             gi_ok = gn->bblk == bblk;
+            // (the shell's written state cannot change before its own
+            // update: nothing else in the run writes it)
             if (gi_ok) {
 #pragma unroll
                 for (int o = 0; o < NOP; ++o)
 #pragma unroll
                     for (int m = 0; m < NSI; ++m)
-                        gi_n[o][m] = Gc[o * opstride + (size_t)(x0n + m) * Ppad + g];
+                        gi_n[o][m] = Gc[o * opstride + (size_t)src_row(x0n, m) * Ppad + g];
             }
         }
         if (bblk != curB) {
@@ -317,7 +352,8 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 #pragma unroll
                 for (int o = 0; o < NOP; ++o)
 #pragma unroll
-                    for (int m = 0; m < NSI; ++m) gi[o][m] = Gc[o * opstride + (size_t)(x0 + m) * Ppad + g];
+                    for (int m = 0; m < NSI; ++m)
+                        gi[o][m] = Gc[o * opstride + (size_t)src_row(x0, m) * Ppad + g];
             }
 #pragma unroll
             for (int o = 0; o < NOP; ++o) {
@@ -327,41 +363,45 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                 for (int m = 0; m < NSI; ++m)
                     Gc[o * opstride + (size_t)(x0 + m) * Ppad + g] = gi[o][m] + gs[m];
             }
+            mark(x0);
         }
         item = item_n;
     }
     if (curB >= 0) flush_k();
+    if (tracked) store_mask();
 }
 
 template <int LA, int LB, int NOP, typename real, typename gacc>
 static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
                         void* G, size_t gstride, size_t opstride, int ncopy, const int* counts,
-                        const int* ids, const void* recs, const void* boys, int zero_rows)
+                        const int* ids, const void* recs, const void* boys, int zero_rows,
+                        const MdMasks& mk)
 {
     const int workers = g.nsub * ncopy;
     const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
     hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES),
                        0, s, g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, opstride,
                        counts, ids, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
-                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy, zero_rows);
+                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy, zero_rows,
+                       mk.m, mk.opstride, mk.zrow, mk.first ? 1 : 0);
     WB_HIP(hipGetLastError());
 }
 
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
                const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
                const int* counts, const int* ids, const void* recs, const void* boys,
-               int zero_rows)
+               int zero_rows, const MdMasks& mk)
 {
     if (c.npairs == 0) return;
     if (nop != 1 && !(nop == 2 && md_fuse(c.la, c.lb)))
         throw Error(WB_INVALID, "operator count not available for this shell-pair class");
 #define WB_LAUNCH(XA, XB, NOPV)                                                                      \
     if (f32 && !g64)                                                                                 \
-        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows);  \
+        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows, mk);  \
     else if (f32)                                                                                    \
-        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows); \
+        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows, mk); \
     else                                                                                             \
-        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows);
+        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows, mk);
 #define WB_CASE(XA, XB)                                                                    \
     if (c.la == XA && c.lb == XB) {                                                        \
         if constexpr (md_fuse(XA, XB)) {                                                   \

@@ -560,6 +560,39 @@ void fold_copies_kernel(size_t n, int K, const BT* __restrict__ src, int nb, siz
     }
 }
 
+// The same fold over the exchange kernels' G copies, where a copy's rows
+// in a sub-block region hold data only once written (masks, MdMasks
+// layout; row0[r] = first row of row r's shell): never-written rows are
+// skipped, not read.  Four consecutive points per thread (one sub-block).
+template <typename BT>
+__global__ __launch_bounds__(256)
+void fold_masked_kernel(size_t n, int K, const BT* __restrict__ src, int nb, size_t stride,
+                        const double* __restrict__ w, double* __restrict__ out,
+                        const unsigned* __restrict__ masks, const int* __restrict__ row0,
+                        int nsub)
+{
+    const size_t i = 4 * ((size_t)blockIdx.x * 256 + threadIdx.x);
+    if (i >= n) return;
+    const int r = (int)(i / K), p = (int)(i - (size_t)r * K);
+    const int f = row0[r], sb = p / SUB;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    for (int c = 0; c < nb; ++c) {
+        const unsigned word = masks[((size_t)c * nsub + sb) * MD_MASK_WORDS + (f >> 5)];
+        if (!(word >> (f & 31) & 1u)) continue;
+        const BT* q = src + c * stride + i;
+        if constexpr (sizeof(BT) == 4) {
+            const float4 x = *(const float4*)q;
+            v0 += (double)x.x; v1 += (double)x.y; v2 += (double)x.z; v3 += (double)x.w;
+        } else {
+            const double2 x0 = *(const double2*)q, x1 = *(const double2*)(q + 2);
+            v0 += x0.x; v1 += x0.y; v2 += x1.x; v3 += x1.y;
+        }
+    }
+    const double2 w0 = *(const double2*)(w + p), w1 = *(const double2*)(w + p + 2);
+    *(double2*)(out + i) = make_double2(v0 * w0.x, v1 * w0.y);
+    *(double2*)(out + i + 2) = make_double2(v2 * w1.x, v3 * w1.y);
+}
+
 // 3 (96) or 4 (128) MFMA tiles per wave side: the one with less padding
 static int wave_tiles(int D)
 {
@@ -588,9 +621,13 @@ void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const dou
 }
 
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B, bool b_f32,
-                  int nb, size_t bstride, const double* w, double* C, DevBuf& ws)
+                  int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
+                  const unsigned* masks, const int* row0, int nsub)
 {
+    if (masks && (K != nsub * SUB || !w || bstride % 4 || !aligned16(B) || !aligned16(w)))
+        throw Error(WB_INVALID, "masked fold: B must be sub-block copies of 16-byte rows");
     if (K % 2 || !aligned16(A)) {                 // 16-byte loads impossible
+        if (masks) throw Error(WB_INVALID, "masked fold: unaligned operand");
         gemm_nt_reduce(s, M, N, K, A, B, b_f32, nb, bstride, w, C, ws);
         return;
     }
@@ -603,20 +640,32 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
     int kchunk = (K + nsplit - 1) / nsplit;
     kchunk = (kchunk + T_K - 1) / T_K * T_K;
     nsplit = (K + kchunk - 1) / kchunk;
-    const bool fold = b_f32 || nb != 1 || w != nullptr || !aligned16(B);
+    const bool fold = b_f32 || nb != 1 || w != nullptr || !aligned16(B) || masks;
     const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
     char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * nbk : 0));
     double* Cp = (double*)base;
     if (fold) {
         double* Bf = (double*)(base + part_bytes);
+        if (masks) {
+            const unsigned blocks = (unsigned)((nbk / 4 + 255) / 256);
+            if (b_f32)
+                hipLaunchKernelGGL(fold_masked_kernel<float>, dim3(blocks), dim3(256), 0, s, nbk, K,
+                                   (const float*)B, nb, bstride, w, Bf, masks, row0, nsub);
+            else
+                hipLaunchKernelGGL(fold_masked_kernel<double>, dim3(blocks), dim3(256), 0, s, nbk, K,
+                                   (const double*)B, nb, bstride, w, Bf, masks, row0, nsub);
+            WB_HIP(hipGetLastError());
+        }
         const bool vec = K % 4 == 0 && bstride % 4 == 0 && aligned16(B);
         const unsigned blocks = (unsigned)((vec ? (nbk / 4 + 255) / 256 : (nbk + 255) / 256));
 #define WB_FOLD(T, V) hipLaunchKernelGGL((fold_copies_kernel<T, V>), dim3(blocks), dim3(256), 0, s, \
                                          nbk, K, (const T*)B, nb, bstride, w, Bf)
-        if (b_f32) { if (vec) WB_FOLD(float, true); else WB_FOLD(float, false); }
-        else { if (vec) WB_FOLD(double, true); else WB_FOLD(double, false); }
+        if (!masks) {
+            if (b_f32) { if (vec) WB_FOLD(float, true); else WB_FOLD(float, false); }
+            else { if (vec) WB_FOLD(double, true); else WB_FOLD(double, false); }
+            WB_HIP(hipGetLastError());
+        }
 #undef WB_FOLD
-        WB_HIP(hipGetLastError());
         Bd = Bf;
     }
     dim3 grid(ntile * nsplit);
