@@ -622,21 +622,24 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     hipStream_t s = c.stream;
     c.start();
     const size_t plane = (size_t)nao * Ppad;
-    double* X4 = (double*)c.ws_Xv.ensure(sizeof(double) * 4 * plane);
-    launch_ao_eval(s, B, G, 1, X4);
+    // AO values only: the density and the potential operand evaluate the
+    // gradients on the fly instead of storing (and twice reading) 3 planes
+    double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane);
+    launch_ao_eval(s, B, G, 0, X);
     c.mark(PH_NLC_AO);
     double* T = (double*)c.ws_T.ensure(sizeof(double) * plane);
-    gemm_nn_fast(s, nao, Ppad, nao, d_dm, X4, T, false);   // T = D X (D symmetric)
+    gemm_nn_fast(s, nao, Ppad, nao, d_dm, X, T, false);   // T = D X (D symmetric)
     double* rho = (double*)c.ws_rho.ensure(sizeof(double) * 4 * Ppad);
-    launch_rho(s, nao, Ppad, G.npts, T, X4, rho);
+    double* part = (double*)c.ws_red.ensure(ao_rho_bytes(B, Ppad));
+    launch_ao_rho(s, B, G, T, part, rho);
     c.mark(PH_NLC_RHO);
     double* wv = (double*)c.ws_vv.ensure(sizeof(double) * 5 * Ppad);
-    double* Bm = T;    // T is dead after rho: reuse for B
-    vv10_flow(s, nao, G.nsub, Ppad, G.d_xyz.as<double>(), G.d_w.as<double>(), G.origin, X4, rho,
-              b, Cc, tol, f64, c.ws_B, wv, wv + 4 * Ppad, Bm, d_out2,
-              c.profile ? &c : nullptr);
+    vv10_flow(s, G.nsub, Ppad, G.d_xyz.as<double>(), G.d_w.as<double>(), G.origin, rho, b, Cc,
+              tol, f64, c.ws_B, wv, wv + 4 * Ppad, d_out2, c.profile ? &c : nullptr);
+    double* Bm = T;    // T is dead after the density: reuse for B
+    launch_ao_proj(s, B, G, wv, Bm);
     double* V = (double*)c.ws_K.ensure(sizeof(double) * nao * nao);
-    gemm_nt_fast(s, nao, nao, Ppad, X4, Bm, false, 1, 0, nullptr, V, c.ws_Kp);
+    gemm_nt_fast(s, nao, nao, Ppad, X, Bm, false, 1, 0, nullptr, V, c.ws_Kp);
     symadd(s, nao, V, d_vmat);
     c.mark(PH_NLC_FOCK);
     collect(c);

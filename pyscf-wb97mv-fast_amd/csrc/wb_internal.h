@@ -195,9 +195,18 @@ void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool 
                const int* counts, const int* ids, const void* recs, const void* boys,
                int zero_rows, const MdMasks& mk);
 
+// GGA density on the grid without stored AO gradients: rho[0][g] =
+// sum_mu T[mu][g] X[mu][g], rho[1+d][g] = 2 sum_mu T[mu][g] dX_d[mu][g]
+// (T = D X), the AO values and gradients evaluated on the fly; part:
+// ao_rho_bytes scratch (partials per shell group, summed in fixed order)
+size_t ao_rho_bytes(const Basis& b, int Ppad);
+void launch_ao_rho(hipStream_t s, const Basis& b, const Grid& g, const double* T, double* part,
+                   double* rho /* [4][Ppad] */);
+// B[mu][g] = v0 X + v1 dX/dx + v2 dX/dy + v3 dX/dz, vg = [4][Ppad] (the GGA
+// potential factors), AO values and gradients evaluated on the fly
+void launch_ao_proj(hipStream_t s, const Basis& b, const Grid& g, const double* vg, double* Bout);
+
 // VV10
-void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T,
-                const double* X, double* rho /* [4][Ppad] */);
 // VV10 double sum over n = *nptr <= ncap compacted points.  pts: (x, y, z,
 // W0) and pk: (kappa, q) per point, float or double (f64); tiles: per
 // 64-point tile two float4 (centre xyz, radius | W0min, kmin, qsum, 0); ws:
@@ -211,12 +220,11 @@ void zero_bytes(hipStream_t s, void* p, size_t bytes);
 void transpose(hipStream_t s, int n, const double* a, double* b);
 void symadd(hipStream_t s, int n, const double* a, double* c);
 // The VV10 part of nr_nlc_vxc after the density: pointwise terms, compaction,
-// double sum, potential -> B[nu][g] (vmat = X^T B + (X^T B)^T) and
-// sums = (nelec, excsum).
+// double sum, potential factors wv = [4][Ppad] (the operand B = launch_ao_proj
+// of them; vmat = X^T B + (X^T B)^T) and sums = (nelec, excsum).
 struct Ctx;
-void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, const double* w,
-               const double* origin, const double* X4, const double* rho, double b, double C,
-               double tol, bool f64, DevBuf& ws, double* wv, double* e, double* Bout,
-               double* sums, Ctx* prof);
+void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
+               const double* origin, const double* rho, double b, double C, double tol, bool f64,
+               DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof);
 
 }  // namespace wb

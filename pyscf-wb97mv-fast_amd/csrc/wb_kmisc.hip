@@ -10,12 +10,13 @@ namespace wb {
 // Follows the evaluation of the reference's AoEvaluator (gpu/ao_eval.py:
 // 255-329): radial sum in FP64 with exp in FP64, Cartesian powers, then
 // cart -> sph (libcint order).
+// One shell at one point: spherical values sv and (DERIV) gradients gs.
 template <int l, int DERIV>
-__device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double rad,
-                                         double drad, double* __restrict__ X, size_t plane,
-                                         int ao0, int Ppad, int g)
+__device__ __forceinline__ void ao_shell_eval(double dx, double dy, double dz, double rad,
+                                              double drad, double (&sv)[2 * l + 1],
+                                              double (&gs)[3][2 * l + 1])
 {
-    constexpr int NC = (l + 1) * (l + 2) / 2, NS = 2 * l + 1;
+    constexpr int NC = (l + 1) * (l + 2) / 2;
     double px[l + 2], py[l + 2], pz[l + 2];
     px[0] = py[0] = pz[0] = 1.0;
 #pragma unroll
@@ -24,7 +25,7 @@ __device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double
         py[k] = py[k - 1] * dy;
         pz[k] = pz[k - 1] * dz;
     }
-    double cv[NC], gx[NC], gy[NC], gz[NC];
+    double cv[NC], gc[3][NC];
     int c = 0;
 #pragma unroll
     for (int lx = l; lx >= 0; --lx) {
@@ -34,44 +35,83 @@ __device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double
             const double poly = px[lx] * py[ly] * pz[lz];
             cv[c] = poly * rad;
             if (DERIV) {
-                gx[c] = py[ly] * pz[lz] * ((lx ? lx * px[lx - 1] : 0.0) * rad + px[lx + 1] * drad);
-                gy[c] = px[lx] * pz[lz] * ((ly ? ly * py[ly - 1] : 0.0) * rad + py[ly + 1] * drad);
-                gz[c] = px[lx] * py[ly] * ((lz ? lz * pz[lz - 1] : 0.0) * rad + pz[lz + 1] * drad);
+                gc[0][c] = py[ly] * pz[lz] * ((lx ? lx * px[lx - 1] : 0.0) * rad + px[lx + 1] * drad);
+                gc[1][c] = px[lx] * pz[lz] * ((ly ? ly * py[ly - 1] : 0.0) * rad + py[ly + 1] * drad);
+                gc[2][c] = px[lx] * py[ly] * ((lz ? lz * pz[lz - 1] : 0.0) * rad + pz[lz + 1] * drad);
             }
             ++c;
         }
     }
-    double sv[NS];
     if constexpr (l == 0) md_gsph_0<double>(cv, sv);
     else if constexpr (l == 1) md_gsph_1<double>(cv, sv);
     else if constexpr (l == 2) md_gsph_2<double>(cv, sv);
     else md_gsph_3<double>(cv, sv);
-#pragma unroll
-    for (int m = 0; m < NS; ++m) X[(size_t)(ao0 + m) * Ppad + g] = sv[m];
     if (DERIV) {
-        const double* src[3] = {gx, gy, gz};
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            if constexpr (l == 0) md_gsph_0<double>(src[ax], sv);
-            else if constexpr (l == 1) md_gsph_1<double>(src[ax], sv);
-            else if constexpr (l == 2) md_gsph_2<double>(src[ax], sv);
-            else md_gsph_3<double>(src[ax], sv);
-#pragma unroll
-            for (int m = 0; m < NS; ++m)
-                X[(ax + 1) * plane + (size_t)(ao0 + m) * Ppad + g] = sv[m];
+            if constexpr (l == 0) md_gsph_0<double>(gc[ax], gs[ax]);
+            else if constexpr (l == 1) md_gsph_1<double>(gc[ax], gs[ax]);
+            else if constexpr (l == 2) md_gsph_2<double>(gc[ax], gs[ax]);
+            else md_gsph_3<double>(gc[ax], gs[ax]);
         }
     }
 }
 
-template <int DERIV>
+// What a pass does with each shell's values at its point g:
+//   AO_STORE0 / AO_STORE1: store values (and the three gradient planes);
+//   AO_RHO: add sum_m T[mu][g] (value, gradient) to the density partials;
+//   AO_PROJ: store B[mu][g] = v0 X + v1 dX/dx + v2 dX/dy + v3 dX/dz (v: the
+//   GGA potential factors at g), the operand of the potential matrix.
+enum AoMode { AO_STORE0 = 0, AO_STORE1, AO_RHO, AO_PROJ };
+
+template <int l, int MODE>
+__device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double rad,
+                                         double drad, double* __restrict__ X, size_t plane,
+                                         int ao0, int Ppad, int g, const double* __restrict__ T,
+                                         const double (&v)[4], double (&acc)[4])
+{
+    constexpr int NS = 2 * l + 1;
+    constexpr int DERIV = MODE != AO_STORE0;
+    double sv[NS], gs[3][NS];
+    ao_shell_eval<l, DERIV>(dx, dy, dz, rad, drad, sv, gs);
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        const size_t o = (size_t)(ao0 + m) * Ppad + g;
+        if constexpr (MODE == AO_STORE0 || MODE == AO_STORE1) {
+            X[o] = sv[m];
+            if constexpr (MODE == AO_STORE1) {
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) X[(ax + 1) * plane + o] = gs[ax][m];
+            }
+        } else if constexpr (MODE == AO_RHO) {
+            const double t = T[o];
+            acc[0] = fma(t, sv[m], acc[0]);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) acc[1 + ax] = fma(t, gs[ax][m], acc[1 + ax]);
+        } else {
+            X[o] = v[0] * sv[m] + v[1] * gs[0][m] + v[2] * gs[1][m] + v[3] * gs[2][m];
+        }
+    }
+}
+
+// grid: (points / 256, shell groups of spb); AO_RHO writes the group's
+// partial sums part[group][4][Ppad]
+template <int MODE>
 __global__ __launch_bounds__(256)
 void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
                const double* __restrict__ exps, const double* __restrict__ coefs,
-               const double* __restrict__ xyz, int Ppad, double* __restrict__ X, size_t plane)
+               const double* __restrict__ xyz, int Ppad, double* __restrict__ X, size_t plane,
+               const double* __restrict__ T, const double* __restrict__ vg,
+               double* __restrict__ part)
 {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= Ppad) return;
     const double x = xyz[g], y = xyz[Ppad + g], z = xyz[2 * Ppad + g];
+    double v[4] = {0, 0, 0, 0}, acc[4] = {0, 0, 0, 0};
+    if constexpr (MODE == AO_PROJ) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = vg[(size_t)k * Ppad + g];
+    }
     const int s0 = blockIdx.y * spb, s1 = min(nshell, s0 + spb);
     for (int is = s0; is < s1; ++is) {
         const ShellInfo sh = shells[is];
@@ -88,29 +128,75 @@ void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
             }
         }
         switch (sh.l) {
-        case 0: ao_shell<0, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
-        case 1: ao_shell<1, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
-        case 2: ao_shell<2, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
-        default: ao_shell<3, DERIV>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g); break;
+        case 0: ao_shell<0, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        case 1: ao_shell<1, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        case 2: ao_shell<2, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        default: ao_shell<3, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
         }
     }
+    if constexpr (MODE == AO_RHO) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[((size_t)blockIdx.y * 4 + k) * Ppad + g] = acc[k];
+    }
+}
+
+template <int MODE>
+static void launch_ao_pass(hipStream_t s, const Basis& b, const double* xyz, int Ppad, int spb,
+                           double* X, const double* T, const double* vg, double* part)
+{
+    dim3 grid((Ppad + 255) / 256, (b.nshell + spb - 1) / spb);
+    hipLaunchKernelGGL(ao_kernel<MODE>, grid, dim3(256), 0, s, b.d_shells.as<ShellInfo>(),
+                       b.nshell, spb, b.d_exps.as<double>(), b.d_coefs.as<double>(), xyz, Ppad, X,
+                       (size_t)b.nao * Ppad, T, vg, part);
+    WB_HIP(hipGetLastError());
 }
 
 void launch_ao_eval_pts(hipStream_t s, const Basis& b, const double* xyz, int Ppad, int deriv,
                         double* X)
 {
-    const int spb = 16;
-    dim3 grid((Ppad + 255) / 256, (b.nshell + spb - 1) / spb);
-    const size_t plane = (size_t)b.nao * Ppad;
-    if (deriv)
-        hipLaunchKernelGGL(ao_kernel<1>, grid, dim3(256), 0, s, b.d_shells.as<ShellInfo>(),
-                           b.nshell, spb, b.d_exps.as<double>(), b.d_coefs.as<double>(), xyz,
-                           Ppad, X, plane);
-    else
-        hipLaunchKernelGGL(ao_kernel<0>, grid, dim3(256), 0, s, b.d_shells.as<ShellInfo>(),
-                           b.nshell, spb, b.d_exps.as<double>(), b.d_coefs.as<double>(), xyz,
-                           Ppad, X, plane);
+    if (deriv) launch_ao_pass<AO_STORE1>(s, b, xyz, Ppad, 16, X, nullptr, nullptr, nullptr);
+    else launch_ao_pass<AO_STORE0>(s, b, xyz, Ppad, 16, X, nullptr, nullptr, nullptr);
+}
+
+// rho[0][g] = sum_mu T[mu][g] X[mu][g], rho[1+d][g] = 2 sum_mu T[mu][g] dX_d[mu][g]
+// from the partials of the shell groups, summed in group order
+__global__ __launch_bounds__(256)
+void rho_reduce_kernel(int Ppad, int ngrp, const double* __restrict__ part,
+                       double* __restrict__ rho)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= Ppad) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double r = 0.0;
+        for (int q = 0; q < ngrp; ++q) r += part[((size_t)q * 4 + k) * Ppad + g];
+        rho[(size_t)k * Ppad + g] = k ? 2.0 * r : r;
+    }
+}
+
+// shell groups of the fused passes: about 8 per point block, for enough waves
+static int ao_group(int nshell) { return std::max(1, (nshell + 7) / 8); }
+
+size_t ao_rho_bytes(const Basis& b, int Ppad)
+{
+    const int spb = ao_group(b.nshell);
+    return sizeof(double) * 4 * (size_t)Ppad * ((b.nshell + spb - 1) / spb);
+}
+
+void launch_ao_rho(hipStream_t s, const Basis& b, const Grid& g, const double* T, double* part,
+                   double* rho)
+{
+    const int spb = ao_group(b.nshell), ngrp = (b.nshell + spb - 1) / spb;
+    launch_ao_pass<AO_RHO>(s, b, g.d_xyz.as<double>(), g.Ppad, spb, nullptr, T, nullptr, part);
+    hipLaunchKernelGGL(rho_reduce_kernel, dim3((g.Ppad + 255) / 256), dim3(256), 0, s, g.Ppad,
+                       ngrp, part, rho);
     WB_HIP(hipGetLastError());
+}
+
+void launch_ao_proj(hipStream_t s, const Basis& b, const Grid& g, const double* vg, double* Bout)
+{
+    launch_ao_pass<AO_PROJ>(s, b, g.d_xyz.as<double>(), g.Ppad, ao_group(b.nshell), Bout,
+                            nullptr, vg, nullptr);
 }
 
 void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X)
@@ -682,39 +768,6 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
     WB_HIP(hipGetLastError());
 }
 
-// ---------------------------------------------------------- GGA density ---
-// rho[0][g] = sum_mu T[mu][g] X[mu][g], rho[1+d][g] = 2 sum_mu T[mu][g] dX_d[mu][g]
-__global__ __launch_bounds__(256)
-void rho_kernel(int nao, int Ppad, const double* __restrict__ T, const double* __restrict__ X,
-                double* __restrict__ rho)
-{
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= Ppad) return;
-    const size_t plane = (size_t)nao * Ppad;
-    double r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-    for (int m = 0; m < nao; ++m) {
-        const size_t o = (size_t)m * Ppad + g;
-        const double tv = T[o];
-        r0 = fma(tv, X[o], r0);
-        r1 = fma(tv, X[plane + o], r1);
-        r2 = fma(tv, X[2 * plane + o], r2);
-        r3 = fma(tv, X[3 * plane + o], r3);
-    }
-    rho[g] = r0;
-    rho[Ppad + g] = 2.0 * r1;
-    rho[2 * Ppad + g] = 2.0 * r2;
-    rho[3 * Ppad + g] = 2.0 * r3;
-}
-
-void launch_rho(hipStream_t s, int nao, int Ppad, int npts, const double* T, const double* X,
-                double* rho)
-{
-    (void)npts;
-    hipLaunchKernelGGL(rho_kernel, dim3((Ppad + 255) / 256), dim3(256), 0, s, nao, Ppad, T, X,
-                       rho);
-    WB_HIP(hipGetLastError());
-}
-
 // --------------------------------------------------------- VV10 sum -------
 // Tile pruning (tol > 0): a wave skips a 64-point tile when, from the tile
 // statistics and its own points' (wave reductions),
@@ -1171,18 +1224,6 @@ void vv_pot_kernel(const int* __restrict__ nptr, const int* __restrict__ idx, in
 }
 
 // B[nu][g] = sum_c wv[c][g] X[c][nu][g]
-__global__ __launch_bounds__(256)
-void vv_b_kernel(int nao, int Ppad, const double* __restrict__ wv, const double* __restrict__ X,
-                 double* __restrict__ B)
-{
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    const int mu = blockIdx.y;
-    if (g >= Ppad) return;
-    const size_t plane = (size_t)nao * Ppad, o = (size_t)mu * Ppad + g;
-    B[o] = wv[g] * X[o] + wv[Ppad + g] * X[plane + o] + wv[2 * Ppad + g] * X[2 * plane + o] +
-           wv[3 * Ppad + g] * X[3 * plane + o];
-}
-
 // out[0] = sum rho w, out[1] = sum e  (one block, fixed order)
 __global__ __launch_bounds__(1024)
 void vv_sums_kernel(int Ppad, const double* __restrict__ rho, const double* __restrict__ w,
@@ -1266,10 +1307,9 @@ void symadd(hipStream_t s, int n, const double* a, double* c)
     WB_HIP(hipGetLastError());
 }
 
-void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, const double* w,
-               const double* origin, const double* X4, const double* rho, double b, double C,
-               double tol, bool f64, DevBuf& ws, double* wv, double* e, double* Bout,
-               double* sums, Ctx* prof)
+void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
+               const double* origin, const double* rho, double b, double C, double tol, bool f64,
+               DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof)
 {
     // workspace layout
     const size_t rs = f64 ? 8 : 4;
@@ -1322,8 +1362,6 @@ void vv10_flow(hipStream_t s, int nao, int nsub, int Ppad, const double* xyz, co
     zero_bytes(s, e, sizeof(double) * Ppad);
     hipLaunchKernelGGL(vv_pot_kernel, dim3((Ppad + 255) / 256), dim3(256), 0, s, nptr, idx, Ppad,
                        F, U, W, aux, rho, w, beta, wv, e);
-    hipLaunchKernelGGL(vv_b_kernel, dim3((Ppad + 255) / 256, nao), dim3(256), 0, s, nao, Ppad, wv,
-                       X4, Bout);
     hipLaunchKernelGGL(vv_sums_kernel, dim3(1), dim3(1024), 0, s, Ppad, rho, w, e, sums);
     WB_HIP(hipGetLastError());
 }
