@@ -171,6 +171,28 @@ def main():
                           'phases_ms': dict(zip(PHASES, [round(float(v), 4) for v in ph])),
                           'md_algorithmic_flops_per_operator': flops}), flush=True)
         return
+    # the same mixed cycle replayed as one HIP graph (wb_cycle_graph_*)
+    graph = lib.CycleGraph(ctx, basis, grid, d_dm.data_ptr(), om, K_TOL, grid, d_dm.data_ptr(),
+                           NLC, args.vv10_tol, 'mixed', d_K.data_ptr(), d_e.data_ptr(),
+                           d_V.data_ptr())
+    for _ in range(args.warmup):
+        graph.launch()
+    ctx.sync()
+    gtimes = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        graph.launch()
+        ctx.sync()
+        gtimes.append(time.perf_counter() - t)
+    gtotal = time.perf_counter() - t0
+    gsame = (numpy.array_equal(d_K.cpu().numpy(), res32[0]) and
+             numpy.array_equal(d_e.cpu().numpy(), res32[1]) and
+             numpy.array_equal(d_V.cpu().numpy(), res32[2]))
+    graph.close()
+    log('mixed, one HIP graph per cycle: %.2f ms/cycle (outputs identical to the eager '
+        'cycle: %s)' % (1e3 * gtotal / args.steps, gsame))
+
     # all-FP64 form of the same kernels
     # (exact VV10: the reference for the mixed form's error, pruning included)
     total64, times64, ph64, res64 = run(lib.PREC['fp64'], args.fp64_steps, 1, profile=True,
@@ -201,6 +223,10 @@ def main():
                    'guess': 'extended Hueckel'},
         'latency_ms': {'p50': round(1e3 * float(numpy.percentile(times, 50)), 3),
                        'p99': round(1e3 * float(numpy.percentile(times, 99)), 3)},
+        'graph': {'ms_per_step': round(1e3 * gtotal / args.steps, 3),
+                  'p50': round(1e3 * float(numpy.percentile(gtimes, 50)), 3),
+                  'p99': round(1e3 * float(numpy.percentile(gtimes, 99)), 3),
+                  'identical_to_eager': bool(gsame)},
         'fp64': {'value': round(value64, 4), 'ms_per_step': round(1e3 * total64 / args.fp64_steps, 3),
                  'speedup_mixed_vs_fp64': round(value / value64, 3)},
         'accuracy_vs_fp64': {'dE_K_1/r': dEk[0], 'dE_K_erf': dEk[1], 'dE_x': dEx,

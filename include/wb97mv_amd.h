@@ -49,6 +49,7 @@ extern "C" {
 typedef void* wb_ctx_t;
 typedef void* wb_basis_t;
 typedef void* wb_grid_t;
+typedef void* wb_graph_t;
 
 const char* wb_last_error(void);
 int wb_version(void);
@@ -119,6 +120,25 @@ int wb_nlc_vxc(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* dm,
 int wb_nlc_vxc_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_dm,
                    double b, double C, double tile_tol, int precision, double* d_out2,
                    double* d_vmat);
+
+/* One SCF cycle's device work as a HIP graph (no counterpart in the
+ * reference, which launches every kernel per call; for callers that replay
+ * cycles on fixed buffers, such as a device-resident SCF loop, and skip the
+ * per-kernel launch cost): the K build of wb_k_build_dev
+ * (when kgrid is not null: d_qd -> d_k) followed by the VV10 term of
+ * wb_nlc_vxc_dev (when nlcgrid is not null: d_dm -> d_out2, d_vmat), with
+ * the context's copy count and G precision.  Records by running the cycle
+ * once (outputs written), then capturing it on the context's stream; the
+ * graph owns its workspaces.  The device buffers are fixed at recording:
+ * write each cycle's qd / dm into the same buffers, then launch.  basis and
+ * the grids must outlive the graph. */
+int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const double* d_qd,
+                          int nop, const double* omegas, double k_tol, wb_grid_t nlcgrid,
+                          const double* d_dm, double b, double C, double vv_tol, int precision,
+                          double* d_k, double* d_out2, double* d_vmat, wb_graph_t* out);
+/* enqueue one replay on the context's stream (asynchronous: wb_sync waits) */
+int wb_cycle_graph_launch(wb_graph_t graph);
+int wb_cycle_graph_destroy(wb_graph_t graph);
 
 /* VV10 kernel sums on given points (coords n x 3): F = -1.5 sum T, U, W as
  * libdft's VXC_vv10nlc returns them (gpu/vv10.py:167-240). */

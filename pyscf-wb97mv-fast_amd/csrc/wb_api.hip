@@ -14,6 +14,7 @@
 #include <numeric>
 #include "wb_boys.h"
 #include "wb_internal.h"
+#include <memory>
 
 namespace wb {
 
@@ -844,6 +845,94 @@ int wb_nlc_vxc_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double*
             throw Error(WB_INVALID, "unknown precision");
         WB_HIP(hipSetDevice(c->device));
         nlc_dev(*c, *(Basis*)basis, *(Grid*)grid, d_dm, b, C, tile_tol, precision, d_out2, d_vmat);
+    })
+}
+
+// ------------------------------------------------------- cycle graphs ---
+// One SCF cycle's device work recorded once as a HIP graph.  The graph owns
+// its workspaces (a private context on the owner's stream): an eager call on
+// the owner context may regrow the owner's buffers without invalidating the
+// recorded addresses.  Nothing in the recorded path allocates, copies from
+// the host or synchronizes; buffers are cleared by fill kernels.
+struct CycleGraph {
+    Ctx w;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~CycleGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const double* d_qd,
+                          int nop, const double* omegas, double k_tol, wb_grid_t nlcgrid,
+                          const double* d_dm, double b, double C, double vv_tol, int precision,
+                          double* d_k, double* d_out2, double* d_vmat, wb_graph_t* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!out || !basis) throw Error(WB_INVALID, "null argument");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        if (kgrid && (nop < 1 || nop > 8 || !omegas || !d_qd || !d_k))
+            throw Error(WB_INVALID, "K build: nop out of range or null buffer");
+        if (nlcgrid && (!d_dm || !d_out2 || !d_vmat)) throw Error(WB_INVALID, "VV10: null buffer");
+        if (!kgrid && !nlcgrid) throw Error(WB_INVALID, "nothing to record");
+        WB_HIP(hipSetDevice(c->device));
+        auto g = std::make_unique<CycleGraph>();
+        Ctx& w = g->w;
+        w.device = c->device;
+        w.stream = c->stream;
+        w.ncopy = c->ncopy;
+        w.g64 = c->g64;
+        std::vector<float> t32;
+        std::vector<double> t64;
+        boys_tables(t32, t64);
+        upload(w.d_b32, t32.data(), sizeof(float) * t32.size(), w.stream);
+        upload(w.d_b64, t64.data(), sizeof(double) * t64.size(), w.stream);
+        std::vector<double> om(omegas, omegas + (kgrid ? nop : 0));
+        Basis& B = *(Basis*)basis;
+        auto body = [&]() {
+            if (kgrid) k_build_dev(w, B, *(Grid*)kgrid, d_qd, nop, om.data(), k_tol, precision, d_k);
+            if (nlcgrid)
+                nlc_dev(w, B, *(Grid*)nlcgrid, d_dm, b, C, vv_tol, precision, d_out2, d_vmat);
+        };
+        body();                 // sizes every workspace and fills the constant caches
+        WB_HIP(hipStreamSynchronize(w.stream));
+        WB_HIP(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t part = nullptr;
+            (void)hipStreamEndCapture(w.stream, &part);
+            if (part) (void)hipGraphDestroy(part);
+            throw;
+        }
+        WB_HIP(hipStreamEndCapture(w.stream, &g->graph));
+        WB_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        *out = (wb_graph_t)g.release();
+    })
+}
+
+int wb_cycle_graph_launch(wb_graph_t graph)
+{
+    WB_TRY({
+        auto* g = (CycleGraph*)graph;
+        if (!g) throw Error(WB_INVALID, "null graph");
+        WB_HIP(hipSetDevice(g->w.device));
+        WB_HIP(hipGraphLaunch(g->exec, g->w.stream));
+    })
+}
+
+int wb_cycle_graph_destroy(wb_graph_t graph)
+{
+    WB_TRY({
+        auto* g = (CycleGraph*)graph;
+        if (!g) return WB_OK;
+        WB_HIP(hipSetDevice(g->w.device));
+        WB_HIP(hipStreamSynchronize(g->w.stream));
+        delete g;
     })
 }
 

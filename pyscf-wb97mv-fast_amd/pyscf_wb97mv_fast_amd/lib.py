@@ -53,6 +53,10 @@ _SIGS = {
     'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     'wb_boys_eval': (_i, [_vp, _i, _vp, _i, _i, _vp]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'wb_cycle_graph_create': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i,
+                                   _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    'wb_cycle_graph_launch': (_i, [_vp]),
+    'wb_cycle_graph_destroy': (_i, [_vp]),
 }
 EXPORTS = tuple(_SIGS)
 
@@ -65,6 +69,12 @@ def load(path=LIB_PATH):
             if not os.path.exists(path):
                 raise Unsupported('libwb97mv_amd.so is not built (%s); run '
                                   '__graft_entry__.build()' % path)
+            try:
+                # PyTorch ships its own HIP runtime; loaded after this
+                # library it finds no device, so it goes first when present
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             lib = ctypes.CDLL(path)
             for name, (res, args) in _SIGS.items():
                 fn = getattr(lib, name)
@@ -171,5 +181,43 @@ class DeviceGrid:
         try:
             if getattr(self, 'h', None):
                 load().wb_grid_destroy(self.h)
+        except Exception:      # noqa: BLE001
+            pass
+
+
+class CycleGraph:
+    """One SCF cycle recorded as a HIP graph (wb_cycle_graph_create): the K
+    build (kgrid: d_qd -> d_k, len(omegas) operators) and/or the VV10 term
+    (nlcgrid: d_dm -> d_out2, d_vmat).  The d_* arguments are device
+    addresses (ints, e.g. ``tensor.data_ptr()``) fixed at recording; write
+    each cycle's inputs into the same buffers and call :meth:`launch`.  The
+    basis and grids must stay alive as long as the graph."""
+
+    def __init__(self, ctx, basis, kgrid=None, d_qd=0, omegas=(), k_tol=1e-11, nlcgrid=None,
+                 d_dm=0, nlc=(6.0, 0.01), vv_tol=0.0, precision='mixed', d_k=0, d_out2=0,
+                 d_vmat=0):
+        lib = load()
+        self.ctx, self.basis, self.kgrid, self.nlcgrid = ctx, basis, kgrid, nlcgrid
+        om = f64(omegas) if len(omegas) else numpy.zeros(1)
+        h = ctypes.c_void_p()
+        check(lib.wb_cycle_graph_create(
+            ctx.h, basis.h, kgrid.h if kgrid is not None else None, d_qd or None,
+            len(omegas), ptr(om), float(k_tol), nlcgrid.h if nlcgrid is not None else None,
+            d_dm or None, float(nlc[0]), float(nlc[1]), float(vv_tol), PREC[precision],
+            d_k or None, d_out2 or None, d_vmat or None, ctypes.byref(h)))
+        self.h = h
+
+    def launch(self):
+        """Enqueue one replay on the context's stream (ctx.sync() waits)."""
+        check(load().wb_cycle_graph_launch(self.h))
+
+    def close(self):
+        if getattr(self, 'h', None):
+            load().wb_cycle_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:      # noqa: BLE001
             pass
