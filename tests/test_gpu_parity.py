@@ -135,6 +135,28 @@ def test_k_drug_like_tzvp(ctx, precision):
         assert err < (1e-10 if precision == 'fp64' else 1e-6)
 
 
+def test_k_large_basis_clears_g(ctx):
+    """(H2O)86 in def2-SVP has 2064 AOs: past the 2048 rows that the
+    written-shell masks of the G copies cover, the first class launched
+    clears G instead.  K for both operators on one 64-point sub-block
+    (random symmetric density, no screening) matches the oracle."""
+    import types
+    mol = mole.Mole(geometry.water_cluster(86, seed=4), 'def2-svp')
+    assert mol.nao + 7 > 2048
+    rng = numpy.random.default_rng(0)
+    xyz = mol.atom_coords()
+    pts = xyz[rng.choice(len(xyz), 64)] + rng.normal(scale=0.8, size=(64, 3))
+    g = types.SimpleNamespace(coords=pts, weights=rng.uniform(0.01, 0.1, 64))
+    a = rng.normal(scale=0.01, size=(mol.nao, mol.nao))
+    dm = a + a.T
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
+    Kb = kb.k_raw(g, dm, [0.0, 0.3])
+    ob = oracle.OracleBasis.from_mol(mol)
+    for K, omega in zip(Kb, (0.0, 0.3)):
+        ref = oracle.k_build(ob, g.coords, g.weights, dm, omega, pair_cut=1e-14)
+        assert numpy.allclose(K, ref, rtol=0, atol=1e-10 * numpy.abs(ref).max()), omega
+
+
 def test_k_screening(ctx):
     mol, g, dm = _system(atoms=geometry.water_cluster(8, seed=1), level=0, npts=6000)
     kb0 = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
