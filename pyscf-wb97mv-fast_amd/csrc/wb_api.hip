@@ -204,8 +204,13 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         double Pd[3] = {0, 0, 0};
         std::vector<double> Ps;
         const double rs = rowsum_c2s_max(la) * rowsum_c2s_max(lb);
+        // primitive pairs, most important first: (far-field bound, index)
+        struct Prim { double key, P[3], p, pref, beta; double E[3][4][4][7]; };
+        std::vector<Prim> prims;
+        prims.reserve((size_t)A.nprim * Bs.nprim);
         for (int i = 0; i < A.nprim; ++i)
             for (int j = 0; j < Bs.nprim; ++j) {
+                Prim pm{};
                 const double ea = exps[A.prim0 + i], eb = exps[Bs.prim0 + j];
                 const double p = ea + eb, mu = ea * eb / p;
                 const double K = std::exp(-mu * ab2);
@@ -214,16 +219,10 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
                                      (ea * A.cz + eb * Bs.cz) / p};
                 const double PA[3] = {P[0] - A.cx, P[1] - A.cy, P[2] - A.cz};
                 const double PB[3] = {P[0] - Bs.cx, P[1] - Bs.cy, P[2] - Bs.cz};
-                C.P.insert(C.P.end(), P, P + 3);
-                C.p.push_back(p);
-                C.pref.push_back(cc * K * 2.0 * M_PI / p);
-                double E[3][4][4][7];
-                for (int d = 0; d < 3; ++d) hermite_axis(la, lb, p, PA[d], PB[d], E[d]);
-                const int L = la + lb;
-                for (int d = 0; d < 3; ++d)
-                    for (int ii = 0; ii <= la; ++ii)
-                        for (int jj = 0; jj <= lb; ++jj)
-                            for (int t = 0; t <= L; ++t) C.E.push_back(E[d][ii][jj][t]);
+                for (int d = 0; d < 3; ++d) pm.P[d] = P[d];
+                pm.p = p;
+                pm.pref = cc * K * 2.0 * M_PI / p;
+                for (int d = 0; d < 3; ++d) hermite_axis(la, lb, p, PA[d], PB[d], pm.E[d]);
                 // screening: |chi_a chi_b| <= |cc| K M exp(-0.75 p |r-P|^2),
                 // M = max over Cartesian components of prod_axes
                 //     sup_s (s+|PA|)^i (s+|PB|)^j exp(-0.25 p s^2)
@@ -236,10 +235,29 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
                                                 std::fabs(PB[d]), 0.25 * p);
                         M = std::max(M, m);
                     }
-                beta += std::fabs(cc) * K * M * 2.0 * M_PI / (0.75 * p) * rs;
-                if (p < pmin) { pmin = p; Pd[0] = P[0]; Pd[1] = P[1]; Pd[2] = P[2]; }
-                Ps.insert(Ps.end(), P, P + 3);
+                pm.beta = std::fabs(cc) * K * M * 2.0 * M_PI / (0.75 * p) * rs;
+                // far from the pair the bound falls as beta / sqrt(p) / distance
+                pm.key = pm.beta / std::sqrt(p);
+                prims.push_back(pm);
             }
+        std::stable_sort(prims.begin(), prims.end(),
+                         [](const Prim& x, const Prim& y) { return x.key > y.key; });
+        for (const Prim& pm : prims) {
+            C.P.insert(C.P.end(), pm.P, pm.P + 3);
+            C.p.push_back(pm.p);
+            C.pref.push_back(pm.pref);
+            const int L = la + lb;
+            for (int d = 0; d < 3; ++d)
+                for (int ii = 0; ii <= la; ++ii)
+                    for (int jj = 0; jj <= lb; ++jj)
+                        for (int t = 0; t <= L; ++t) C.E.push_back(pm.E[d][ii][jj][t]);
+            C.pbc.push_back(make_float4((float)pm.P[0], (float)pm.P[1], (float)pm.P[2],
+                                        (float)(0.8862269254527580 * pm.beta /
+                                                std::sqrt(0.75 * pm.p) * (1 + 1e-5))));
+            beta += pm.beta;
+            if (pm.p < pmin) { pmin = pm.p; Pd[0] = pm.P[0]; Pd[1] = pm.P[1]; Pd[2] = pm.P[2]; }
+            Ps.insert(Ps.end(), pm.P, pm.P + 3);
+        }
         double delta = 0.0;
         for (size_t m = 0; m < Ps.size() / 3; ++m) {
             const double dx = Ps[3 * m] - Pd[0], dy = Ps[3 * m + 1] - Pd[1], dz = Ps[3 * m + 2] - Pd[2];
@@ -343,6 +361,7 @@ static void upload_basis(Basis& B, hipStream_t s)
         upload(C.d_scr, C.scr.data(), sizeof(PairScreen) * C.npairs, s);
         upload(C.d_groups, C.groups.data(), sizeof(PairGroup) * C.ngroups, s);
         upload(C.d_bbao, C.bbao.data(), sizeof(int) * C.bbao.size(), s);
+        upload(C.d_pbc, C.pbc.data(), sizeof(float4) * C.pbc.size(), s);
     }
     WB_HIP(hipStreamSynchronize(s));
 }
@@ -538,12 +557,13 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     auto* work = (unsigned long long*)c.ws_work.ensure(sizeof(unsigned long long) * 2 * NCLASS);
     zero_bytes(s, work, sizeof(unsigned long long) * 2 * NCLASS);
     int* ids = (int*)c.ws_tasks.ensure(sizeof(int) * std::max<size_t>(nitem, 1));
+    unsigned* skips = (unsigned*)c.ws_skips.ensure(sizeof(unsigned) * std::max<size_t>(nitem, 1));
     size_t toff = 0;
     std::vector<size_t> cls_off(NCLASS);
     for (int k = 0; k < NCLASS; ++k) {
         cls_off[k] = toff;
         launch_tasks(s, B.cls[k], G, fmax, B.nshell, (float)tol, counts + (size_t)k * G.nsub,
-                     ids + toff, work + 2 * k);
+                     ids + toff, skips + toff, work + 2 * k);
         toff += (size_t)B.cls[k].ngroups * G.nsub;
     }
     c.tasks_total = (long long)ntask;
@@ -571,19 +591,20 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
         if (!C.npairs) continue;
         const int* cnt = counts + (size_t)k * G.nsub;
         const int* ids_k = ids + cls_off[k];
+        const unsigned* skips_k = skips + cls_off[k];
         const int zrows = first ? zrow : 0;
         if (nop == 2 && md_fuse(C.la, C.lb)) {
             // both operators in one pass over the tasks
             const void* rc = op_consts(C, {omegas[0], omegas[1]}, f32, s);
             MdMasks mk{masks, mopstride, zrow, first};
-            launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, rc, boys,
-                      zrows, mk);
+            launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, skips_k,
+                      rc, boys, zrows, mk);
         } else {
             for (int op = 0; op < nop; ++op) {
                 const void* rc = op_consts(C, {omegas[op]}, f32, s);
                 MdMasks mk{tracked ? masks + mopstride * op : nullptr, mopstride, zrow, first};
                 launch_md(s, C, G, f32, g64, 1, F, Gb + gsz * opstride * op, gstride, opstride,
-                          ncopy, cnt, ids_k, rc, boys, zrows, mk);
+                          ncopy, cnt, ids_k, skips_k, rc, boys, zrows, mk);
             }
         }
         first = false;

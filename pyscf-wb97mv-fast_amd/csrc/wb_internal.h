@@ -80,11 +80,15 @@ struct ClassData {
     std::vector<double> pref;         // (nprims,) c_a c_b K_AB 2 pi / p
     std::vector<double> E;            // (nprims, erl) Hermite E records
     int erl = 0;                      // E record length
+    // per primitive pair (within a shell pair, most important first): centre
+    // and far-field bound factor beta sqrt(pi) / (2 sqrt(0.75 p)) (rounded
+    // up) of the per-task primitive screening
+    std::vector<float4> pbc;
     // pairs are ordered by (B block, A, B); groups: runs of one (B block, A)
     std::vector<PairGroup> groups;
     std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, nao (spare rows) if none
     int ngroups = 0;
-    DevBuf d_pairs, d_scr, d_groups, d_bbao;
+    DevBuf d_pairs, d_scr, d_groups, d_bbao, d_pbc;
     // operator set (omegas) -> primitive-pair records (FP32, FP64), built
     // on first use
     std::map<std::vector<double>, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;
@@ -120,7 +124,7 @@ struct Ctx {
     DevBuf d_b32, d_b64;               // Boys tables
     // workspaces (grow-only)
     DevBuf ws_X, ws_Xv, ws_F, ws_G, ws_tasks, ws_counts, ws_fmax, ws_Kp, ws_K, ws_D;
-    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io, ws_mask;
+    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io, ws_mask, ws_skips;
     // statistics of the last K build
     long long tasks_total = 0, tasks_kept = 0;
     float ms_last = 0.f;
@@ -168,8 +172,11 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
                  int Ppad, int nsub, const int* sh_ao0, const int* sh_nao, int nshell,
                  float* fmax);
+// kept items (group << 8 | kept-position mask) per sub-block, and per item
+// the trailing primitive pairs each kept task skips (4 bits per position)
 void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
-                  int nshell, float tol, int* counts, int* ids, unsigned long long* work);
+                  int nshell, float tol, int* counts, int* ids, unsigned* skips,
+                  unsigned long long* work);
 // f32: FP32 integrals (else FP64); g64: accumulate G in FP64 (always with FP64 integrals)
 // F_0..F_L at n values T (device arrays), out [L+1][n]
 void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
@@ -192,8 +199,8 @@ struct MdMasks {
 };
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
                const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
-               const int* counts, const int* ids, const void* recs, const void* boys,
-               int zero_rows, const MdMasks& mk);
+               const int* counts, const int* ids, const unsigned* skips, const void* recs,
+               const void* boys, int zero_rows, const MdMasks& mk);
 
 // GGA density on the grid without stored AO gradients: rho[0][g] =
 // sum_mu T[mu][g] X[mu][g], rho[1+d][g] = 2 sum_mu T[mu][g] dX_d[mu][g]

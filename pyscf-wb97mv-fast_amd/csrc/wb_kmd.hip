@@ -104,7 +104,8 @@ template <int LA, int LB, int NOP, typename real, typename gacc>
 __global__ __launch_bounds__(64 * MD_WAVES)
 void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  gacc* __restrict__ G, size_t gstride, size_t opstride,
-                 const int* __restrict__ counts, const int* __restrict__ ids, int cap,
+                 const int* __restrict__ counts, const int* __restrict__ ids,
+                 const unsigned* __restrict__ skips, int cap,
                  const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
                  const char* __restrict__ recs,
                  const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk,
@@ -180,6 +181,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
         return;
     }
     const int* tl = ids + (size_t)s * cap;
+    const unsigned* sl = skips + (size_t)s * cap;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
     // wait for the point's coordinates here, once: otherwise the first use
     // inside the item loop waits on the vector-memory counter in program
@@ -240,6 +242,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     for (int t = t0; t < t1; ++t) {
         const PairGroup* gr = groups + (item >> 8);
         const unsigned mask = (unsigned)item & 0xffu;
+        const unsigned skw = sl[t];                          // trailing primitive pairs skipped
         const int bblk = gr->bblk;
         const int x0 = gr->x0;
         real fi[NSI];
@@ -308,7 +311,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 #pragma unroll
         for (int q = 0; q < BB; ++q) {
             if (!(mask >> q & 1u)) continue;                 // uniform
-            const int pp0 = pw[q] >> 8, npp = (pw[q] >> 2) & 63;
+            const int pp0 = pw[q] >> 8, npp = ((pw[q] >> 2) & 63) - (int)(skw >> (4 * q) & 15u);
             const bool onec = pw[q] & 2, diag = pw[q] & 1;
             vacc Vs[NOP * NV];
 #pragma unroll
@@ -374,14 +377,14 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 template <int LA, int LB, int NOP, typename real, typename gacc>
 static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
                         void* G, size_t gstride, size_t opstride, int ncopy, const int* counts,
-                        const int* ids, const void* recs, const void* boys, int zero_rows,
-                        const MdMasks& mk)
+                        const int* ids, const unsigned* skips, const void* recs,
+                        const void* boys, int zero_rows, const MdMasks& mk)
 {
     const int workers = g.nsub * ncopy;
     const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
     hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES),
                        0, s, g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, opstride,
-                       counts, ids, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
+                       counts, ids, skips, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
                        (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy, zero_rows,
                        mk.m, mk.opstride, mk.zrow, mk.first ? 1 : 0);
     WB_HIP(hipGetLastError());
@@ -389,19 +392,19 @@ static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const 
 
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
                const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
-               const int* counts, const int* ids, const void* recs, const void* boys,
-               int zero_rows, const MdMasks& mk)
+               const int* counts, const int* ids, const unsigned* skips, const void* recs,
+               const void* boys, int zero_rows, const MdMasks& mk)
 {
     if (c.npairs == 0) return;
     if (nop != 1 && !(nop == 2 && md_fuse(c.la, c.lb)))
         throw Error(WB_INVALID, "operator count not available for this shell-pair class");
 #define WB_LAUNCH(XA, XB, NOPV)                                                                      \
     if (f32 && !g64)                                                                                 \
-        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows, mk);  \
+        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, skips, recs, boys, zero_rows, mk);  \
     else if (f32)                                                                                    \
-        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows, mk); \
+        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, skips, recs, boys, zero_rows, mk); \
     else                                                                                             \
-        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, recs, boys, zero_rows, mk);
+        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, skips, recs, boys, zero_rows, mk);
 #define WB_CASE(XA, XB)                                                                    \
     if (c.la == XA && c.lb == XB) {                                                        \
         if constexpr (md_fuse(XA, XB)) {                                                   \
@@ -490,18 +493,26 @@ void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int na
     WB_HIP(hipGetLastError());
 }
 
-// One wave per (sub-block): the kept pair ids of one class, in pair order.
+// One wave per (sub-block): the kept groups of one class, in group order,
+// and per kept item the trailing primitive pairs each of its tasks skips
+// (skips: 4 bits per block position).
 // est = beta * F0bound(q d_min^2) * max(fmax[A], fmax[B]); F0(x) <= min(1,
-// sqrt(pi/(4x))).  tol <= 0 keeps every pair.
-// work[0] += kept tasks, work[1] += their primitive pairs (integer atomics:
-// the totals do not depend on the order)
+// sqrt(pi/(4x))).  A task is kept when est >= tol (the reference's
+// shell-pair screening, gpu/sgx_k.py:866-924).  The pair's primitive pairs
+// are stored most important first; pbc[m] = (centre, c_m) with c_m >=
+// beta_m sqrt(pi) / (2 sqrt(q_m)), so primitive pair m contributes at most
+// c_m fmax / d_m at a distance d_m > 0 from the sub-block: a kept task skips
+// the longest tail whose bounds sum to at most tol / 2 (never its first
+// primitive pair, at most 15).  tol <= 0 keeps every pair and primitive.
+// work[0] += kept tasks, work[1] += their evaluated primitive pairs (integer
+// atomics: the totals do not depend on the order)
 __global__ __launch_bounds__(256)
 void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__ pairs,
                  const PairGroup* __restrict__ groups, int ngroups, int bb,
                  const float4* __restrict__ sub, int nsub,
                  const float* __restrict__ fmax, int nshell, float tol,
-                 int* __restrict__ counts, int* __restrict__ ids,
-                 unsigned long long* __restrict__ work)
+                 const float4* __restrict__ pbc, int* __restrict__ counts, int* __restrict__ ids,
+                 unsigned* __restrict__ skips, unsigned long long* __restrict__ work)
 {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = blockIdx.x * 4 + wave;
@@ -510,33 +521,50 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
     const float4 sb = sub[s];
     const float* fm = fmax + (size_t)s * nshell;
     int* out = ids + (size_t)s * ngroups;
+    unsigned* sout = skips + (size_t)s * ngroups;
     int n = 0;
     unsigned long long ntask = 0, npp = 0;
     for (int k0 = 0; k0 < ngroups; k0 += 64) {
         const int k = k0 + lane;
-        unsigned mask = 0;
+        unsigned mask = 0, skw = 0;
         if (k < ngroups) {
             const PairGroup gr = groups[k];
 #pragma unroll
             for (int q = 0; q < MD_BBMAX; ++q) {
                 const int p = gr.pidx[q];
                 if (q >= bb || p < 0) continue;
+                const PairRec pr = pairs[p];
                 bool keep = true;
+                int skip = 0;
                 if (tol > 0.f) {
                     const PairScreen ps = scr[p];
-                    const PairRec pr = pairs[p];
+                    const float fmx = fmaxf(fm[pr.shA], fm[pr.shB]);
                     const float dx = ps.cx - sb.x, dy = ps.cy - sb.y, dz = ps.cz - sb.z;
                     const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
                     float d = dist - sb.w - ps.delta;
                     d -= 1e-5f * (dist + sb.w + ps.delta) + 1e-6f;   // FP32 safety
                     float f0 = 1.0f;
                     if (d > 0.f) f0 = fminf(1.0f, 0.8862269254527580f / (sqrtf(ps.q) * d));
-                    const float est = ps.beta * f0 * fmaxf(fm[pr.shA], fm[pr.shB]);
-                    keep = est * 1.001f >= tol;
+                    keep = ps.beta * f0 * fmx * 1.001f >= tol;
+                    if (keep) {
+                        float acc = 0.f;
+                        for (int m = pr.pp0 + pr.npp - 1; m > pr.pp0 && skip < 15; --m) {
+                            const float4 P = pbc[m];
+                            const float ex = P.x - sb.x, ey = P.y - sb.y, ez = P.z - sb.z;
+                            const float r = sqrtf(ex * ex + ey * ey + ez * ez);
+                            float dm = r - sb.w;
+                            dm -= 1e-5f * (r + sb.w) + 1e-6f;        // FP32 safety
+                            if (dm <= 0.f) break;
+                            acc += P.w * fmx / dm;
+                            if (acc * 1.001f > 0.5f * tol) break;
+                            ++skip;
+                        }
+                    }
                 }
                 if (keep) {
                     mask |= 1u << q;
-                    npp += (unsigned long long)pairs[p].npp;
+                    skw |= (unsigned)skip << (4 * q);
+                    npp += (unsigned long long)(pr.npp - skip);
                 }
             }
         }
@@ -545,7 +573,10 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
         const unsigned long long bal = __ballot(keep);
         const int pos = n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-        if (keep) out[pos] = (k << 8) | (int)mask;
+        if (keep) {
+            out[pos] = (k << 8) | (int)mask;
+            sout[pos] = skw;
+        }
         n += __popcll(bal);
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -560,14 +591,16 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
 }
 
 void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
-                  int nshell, float tol, int* counts, int* ids, unsigned long long* work)
+                  int nshell, float tol, int* counts, int* ids, unsigned* skips,
+                  unsigned long long* work)
 {
     if (c.npairs == 0) return;
     const int blocks = (g.nsub + 3) / 4;
     hipLaunchKernelGGL(task_kernel, dim3(blocks), dim3(256), 0, s, c.d_scr.as<PairScreen>(),
                        c.d_pairs.as<PairRec>(), c.d_groups.as<PairGroup>(), c.ngroups,
                        md_bb(c.la, c.lb), g.d_sub.as<float4>(), g.nsub, fmax, nshell, tol,
-                       counts, ids, work);
+                       c.d_pbc.as<float4>(), counts, ids, skips, work);
+
     WB_HIP(hipGetLastError());
 }
 
