@@ -268,6 +268,10 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         ps.beta = (float)(beta * (1 + 1e-5));
         ps.q = (float)(0.75 * pmin * (1 - 1e-6));
         pr.npp = C.nprims + A.nprim * Bs.nprim - pr.pp0;
+        if (pr.shA >= 32768 || pr.shB >= 32768 || pr.pp0 >= (1 << 25) || pr.npp >= 64)
+            throw Error(WB_UNSUPPORTED, "basis too large for the packed screening records");
+        ps.sh = pr.shA | pr.shB << 16;
+        ps.pp = pr.pp0 << 6 | pr.npp;
         C.nprims += A.nprim * Bs.nprim;
         C.pairs.push_back(pr);
         C.scr.push_back(ps);

@@ -68,7 +68,8 @@ struct PairScreen {           // screening data per pair (pair order of its clas
     float delta;              // max distance of any primitive centre from (cx,cy,cz)
     float beta;               // sum of per-primitive bound prefactors (rounded up)
     float q;                  // 0.75 * min p (rounded down)
-    float pad0, pad1;
+    int sh;                   // shA | shB << 16 (the pair's shells)
+    int pp;                   // pp0 << 6 | npp (its primitive pairs)
 };
 
 struct ClassData {

@@ -507,7 +507,7 @@ void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int na
 // work[0] += kept tasks, work[1] += their evaluated primitive pairs (integer
 // atomics: the totals do not depend on the order)
 __global__ __launch_bounds__(256)
-void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__ pairs,
+void task_kernel(const PairScreen* __restrict__ scr,
                  const PairGroup* __restrict__ groups, int ngroups, int bb,
                  const float4* __restrict__ sub, int nsub,
                  const float* __restrict__ fmax, int nshell, float tol,
@@ -533,12 +533,12 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
             for (int q = 0; q < MD_BBMAX; ++q) {
                 const int p = gr.pidx[q];
                 if (q >= bb || p < 0) continue;
-                const PairRec pr = pairs[p];
+                const PairScreen ps = scr[p];
+                const int pp0 = ps.pp >> 6, pnpp = ps.pp & 63;
                 bool keep = true;
                 int skip = 0;
                 if (tol > 0.f) {
-                    const PairScreen ps = scr[p];
-                    const float fmx = fmaxf(fm[pr.shA], fm[pr.shB]);
+                    const float fmx = fmaxf(fm[ps.sh & 0xffff], fm[ps.sh >> 16]);
                     const float dx = ps.cx - sb.x, dy = ps.cy - sb.y, dz = ps.cz - sb.z;
                     const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
                     float d = dist - sb.w - ps.delta;
@@ -548,7 +548,7 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
                     keep = ps.beta * f0 * fmx * 1.001f >= tol;
                     if (keep) {
                         float acc = 0.f;
-                        for (int m = pr.pp0 + pr.npp - 1; m > pr.pp0 && skip < 15; --m) {
+                        for (int m = pp0 + pnpp - 1; m > pp0 && skip < 15; --m) {
                             const float4 P = pbc[m];
                             const float ex = P.x - sb.x, ey = P.y - sb.y, ez = P.z - sb.z;
                             const float r = sqrtf(ex * ex + ey * ey + ez * ez);
@@ -564,7 +564,7 @@ void task_kernel(const PairScreen* __restrict__ scr, const PairRec* __restrict__
                 if (keep) {
                     mask |= 1u << q;
                     skw |= (unsigned)skip << (4 * q);
-                    npp += (unsigned long long)(pr.npp - skip);
+                    npp += (unsigned long long)(pnpp - skip);
                 }
             }
         }
@@ -597,7 +597,7 @@ void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float*
     if (c.npairs == 0) return;
     const int blocks = (g.nsub + 3) / 4;
     hipLaunchKernelGGL(task_kernel, dim3(blocks), dim3(256), 0, s, c.d_scr.as<PairScreen>(),
-                       c.d_pairs.as<PairRec>(), c.d_groups.as<PairGroup>(), c.ngroups,
+                       c.d_groups.as<PairGroup>(), c.ngroups,
                        md_bb(c.la, c.lb), g.d_sub.as<float4>(), g.nsub, fmax, nshell, tol,
                        c.d_pbc.as<float4>(), counts, ids, skips, work);
 
