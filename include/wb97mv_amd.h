@@ -121,6 +121,20 @@ int wb_nlc_vxc_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double*
                    double b, double C, double tile_tol, int precision, double* d_out2,
                    double* d_vmat);
 
+/* Semilocal XC on grid (the device side of nr_rks, gpu/xc.py:130-201; the
+ * functional is evaluated by the caller between the two calls).
+ * wb_xc_rho: rho5 (5 x npts, the grid's point order) = rho, d rho/dx, /dy,
+ * /dz and (mgga != 0) tau = 1/2 sum |grad phi|^2 for the density matrix dm
+ * (nao x nao, symmetric); tau = 0 otherwise.
+ * wb_xc_vmat: vmat (nao x nao) = sum_g w_g [ v0 phi phi^T + v_d (d_d phi phi^T
+ * + phi d_d phi^T) + 1/2 v4 sum_d d_d phi d_d phi^T ] for the chain-ruled
+ * potential v5 (5 x npts: v_rho, 2 v_sigma d rho / dx, /dy, /dz, v_tau; v4
+ * is used only when mgga != 0).  FP64. */
+int wb_xc_rho(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* dm, int mgga,
+              double* rho5);
+int wb_xc_vmat(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* v5, int mgga,
+               double* vmat);
+
 /* One SCF cycle's device work as a HIP graph (no counterpart in the
  * reference, which launches every kernel per call; for callers that replay
  * cycles on fixed buffers, such as a device-resident SCF loop, and skip the

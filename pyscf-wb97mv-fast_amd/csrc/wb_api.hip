@@ -873,6 +873,63 @@ int wb_nlc_vxc_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double*
     })
 }
 
+// ------------------------------------------------------- semilocal XC ---
+// The device side of nr_rks (reference gpu/xc.py:130-201): the densities on
+// the grid and the potential matrix from the chain-ruled potential; the
+// functional itself is evaluated by the caller (PySCF's libxc, as in the
+// reference).  FP64 throughout.
+// rho: [5][Ppad] (rho, d rho / dx, dy, dz, tau) in the grid's internal order
+static void xc_rho_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, bool mgga, double* rho)
+{
+    const int nao = B.nao, Ppad = G.Ppad;
+    hipStream_t s = c.stream;
+    const size_t plane = (size_t)nao * Ppad;
+    double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane * (mgga ? 4 : 1));
+    launch_ao_eval(s, B, G, mgga ? 1 : 0, X);
+    double* T = (double*)c.ws_T.ensure(sizeof(double) * plane);
+    gemm_nn_fast(s, nao, Ppad, nao, d_dm, X, T, false);         // T = D X (D symmetric)
+    double* part = (double*)c.ws_red.ensure(ao_rho_bytes(B, Ppad));
+    launch_ao_rho(s, B, G, T, part, rho);
+    if (mgga) {
+        // tau = 1/2 sum_d sum_mu (D dX_d)_mu dX_d,mu
+        for (int d = 0; d < 3; ++d) {
+            const double* Xd = X + (size_t)(d + 1) * plane;
+            gemm_nn_fast(s, nao, Ppad, nao, d_dm, Xd, T, false);
+            launch_dot_planes(s, nao, Ppad, T, Xd, 0.5, d > 0, rho + 4 * (size_t)Ppad);
+        }
+    } else {
+        zero_bytes(s, rho + 4 * (size_t)Ppad, sizeof(double) * Ppad);
+    }
+}
+
+// vmat = sum_g w [ v0 X X^T + v_d (dX_d X^T + X dX_d^T) + 1/2 v4 sum_d dX_d dX_d^T ],
+// v: [5][Ppad] chain-ruled potential (internal order, padding 0)
+static void xc_vmat_dev(Ctx& c, Basis& B, Grid& G, const double* v, bool mgga, double* d_vmat)
+{
+    const int nao = B.nao, Ppad = G.Ppad;
+    hipStream_t s = c.stream;
+    const size_t plane = (size_t)nao * Ppad;
+    double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane * (mgga ? 4 : 1));
+    launch_ao_eval(s, B, G, mgga ? 1 : 0, X);
+    // B = (w v0 / 2) X + w v_d dX_d (weights and the 1/2 of the symmetric sum)
+    double* wv = (double*)c.ws_vv.ensure(sizeof(double) * 5 * Ppad);
+    launch_xc_weights(s, Ppad, G.d_w.as<double>(), v, wv);
+    double* Bm = (double*)c.ws_T.ensure(sizeof(double) * plane);
+    launch_ao_proj(s, B, G, wv, Bm);
+    double* V = (double*)c.ws_K.ensure(sizeof(double) * (size_t)nao * nao * (mgga ? 2 : 1));
+    gemm_nt_fast(s, nao, nao, Ppad, X, Bm, false, 1, 0, nullptr, V, c.ws_Kp);
+    symadd(s, nao, V, d_vmat);                                  // vmat = V + V^T
+    if (mgga) {
+        // + sum_d dX_d diag(w v4 / 2) dX_d^T (added once: already symmetric)
+        double* Vt = V + (size_t)nao * nao;
+        for (int d = 0; d < 3; ++d) {
+            const double* Xd = X + (size_t)(d + 1) * plane;
+            gemm_nt_fast(s, nao, nao, Ppad, Xd, Xd, false, 1, 0, wv + 4 * (size_t)Ppad, Vt, c.ws_Kp);
+            launch_axpy(s, (size_t)nao * nao, 1.0, Vt, d_vmat);
+        }
+    }
+}
+
 // ------------------------------------------------------- cycle graphs ---
 // One SCF cycle's device work recorded once as a HIP graph.  The graph owns
 // its workspaces (a private context on the owner's stream): an eager call on
@@ -958,6 +1015,54 @@ int wb_cycle_graph_destroy(wb_graph_t graph)
         WB_HIP(hipSetDevice(g->w.device));
         WB_HIP(hipStreamSynchronize(g->w.stream));
         delete g;
+    })
+}
+
+int wb_xc_rho(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* dm, int mgga,
+              double* rho5)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* bs = (Basis*)basis;
+        auto* G = (Grid*)grid;
+        if (!c || !bs || !G || !dm || !rho5) throw Error(WB_INVALID, "null argument");
+        WB_HIP(hipSetDevice(c->device));
+        const size_t nn = (size_t)bs->nao * bs->nao;
+        double* din = (double*)c->ws_io.ensure(sizeof(double) * (nn + 5 * (size_t)G->Ppad));
+        double* rho = din + nn;
+        WB_HIP(hipMemcpyAsync(din, dm, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+        xc_rho_dev(*c, *bs, *G, din, mgga != 0, rho);
+        std::vector<double> h(5 * (size_t)G->Ppad);
+        WB_HIP(hipMemcpyAsync(h.data(), rho, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+        const int n = G->npts;
+        for (int k = 0; k < 5; ++k)
+            for (int j = 0; j < n; ++j) rho5[(size_t)k * n + G->perm[j]] = h[(size_t)k * G->Ppad + j];
+    })
+}
+
+int wb_xc_vmat(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* v5, int mgga,
+               double* vmat)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* bs = (Basis*)basis;
+        auto* G = (Grid*)grid;
+        if (!c || !bs || !G || !v5 || !vmat) throw Error(WB_INVALID, "null argument");
+        WB_HIP(hipSetDevice(c->device));
+        const size_t nn = (size_t)bs->nao * bs->nao;
+        const int n = G->npts;
+        std::vector<double> h(5 * (size_t)G->Ppad, 0.0);
+        for (int k = 0; k < 5; ++k)
+            for (int j = 0; j < n; ++j) h[(size_t)k * G->Ppad + j] = v5[(size_t)k * n + G->perm[j]];
+        double* din = (double*)c->ws_io.ensure(sizeof(double) * (nn + h.size()));
+        double* dv = din + nn;
+        WB_HIP(hipMemcpyAsync(dv, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        xc_vmat_dev(*c, *bs, *G, dv, mgga != 0, din);
+        WB_HIP(hipMemcpyAsync(vmat, din, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
     })
 }
 

@@ -214,6 +214,14 @@ void launch_ao_rho(hipStream_t s, const Basis& b, const Grid& g, const double* T
 // potential factors), AO values and gradients evaluated on the fly
 void launch_ao_proj(hipStream_t s, const Basis& b, const Grid& g, const double* vg, double* Bout);
 
+// out[g] (+)= scale * sum_mu A[mu][g] B[mu][g], A and B [nao][Ppad]
+void launch_dot_planes(hipStream_t s, int nao, int Ppad, const double* A, const double* B,
+                       double scale, bool accumulate, double* out);
+// y += a x
+void launch_axpy(hipStream_t s, size_t n, double a, const double* x, double* y);
+// wv = (w v0 / 2, w v1, w v2, w v3, w v4 / 2), v and wv [5][Ppad]
+void launch_xc_weights(hipStream_t s, int Ppad, const double* w, const double* v, double* wv);
+
 // VV10
 // VV10 double sum over n = *nptr <= ncap compacted points.  pts: (x, y, z,
 // W0) and pk: (kappa, q) per point, float or double (f64); tiles: per

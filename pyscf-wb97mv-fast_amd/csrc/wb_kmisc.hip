@@ -174,6 +174,31 @@ void rho_reduce_kernel(int Ppad, int ngrp, const double* __restrict__ part,
     }
 }
 
+// out[g] (+)= scale * sum_mu A[mu][g] B[mu][g] (the kinetic energy density
+// from T_d = D dX_d and dX_d, one axis per call)
+__global__ __launch_bounds__(256)
+void dot_planes_kernel(int nao, int Ppad, const double* __restrict__ A,
+                       const double* __restrict__ B, double scale, int accumulate,
+                       double* __restrict__ out)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= Ppad) return;
+    double r = 0.0;
+    for (int m = 0; m < nao; ++m) {
+        const size_t o = (size_t)m * Ppad + g;
+        r = fma(A[o], B[o], r);
+    }
+    out[g] = accumulate ? out[g] + scale * r : scale * r;
+}
+
+void launch_dot_planes(hipStream_t s, int nao, int Ppad, const double* A, const double* B,
+                       double scale, bool accumulate, double* out)
+{
+    hipLaunchKernelGGL(dot_planes_kernel, dim3((Ppad + 255) / 256), dim3(256), 0, s, nao, Ppad,
+                       A, B, scale, accumulate ? 1 : 0, out);
+    WB_HIP(hipGetLastError());
+}
+
 // shell groups of the fused passes: about 8 per point block, for enough waves
 static int ao_group(int nshell) { return std::max(1, (nshell + 7) / 8); }
 
@@ -1304,6 +1329,36 @@ void transpose(hipStream_t s, int n, const double* a, double* b)
 void symadd(hipStream_t s, int n, const double* a, double* c)
 {
     hipLaunchKernelGGL(symadd_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, a, c);
+    WB_HIP(hipGetLastError());
+}
+
+// y += a x (n doubles)
+__global__ void axpy_kernel(size_t n, double a, const double* __restrict__ x, double* __restrict__ y)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = fma(a, x[i], y[i]);
+}
+void launch_axpy(hipStream_t s, size_t n, double a, const double* x, double* y)
+{
+    hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, x, y);
+    WB_HIP(hipGetLastError());
+}
+
+// wv = (w v0 / 2, w v1, w v2, w v3, w v4 / 2) from the chain-ruled potential
+// v [5][Ppad]: the operand factors of the semilocal potential matrix
+__global__ void xc_weights_kernel(int Ppad, const double* __restrict__ w,
+                                  const double* __restrict__ v, double* __restrict__ wv)
+{
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= Ppad) return;
+    const double wg = w[g];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        wv[(size_t)k * Ppad + g] = (k == 0 || k == 4 ? 0.5 : 1.0) * wg * v[(size_t)k * Ppad + g];
+}
+void launch_xc_weights(hipStream_t s, int Ppad, const double* w, const double* v, double* wv)
+{
+    hipLaunchKernelGGL(xc_weights_kernel, dim3((Ppad + 255) / 256), dim3(256), 0, s, Ppad, w, v, wv);
     WB_HIP(hipGetLastError());
 }
 

@@ -2,6 +2,7 @@
 
 Same removable hooks as the reference (gpu/install.py:209-336):
 
+    mf._numint.nr_rks            -> xc.gpu_nr_rks            (semilocal XC, semilocal=True)
     mf._numint.nr_nlc_vxc        -> vv10.gpu_nr_nlc_vxc      (VV10)
     pyscf.sgx.sgx_jk.get_k_only  -> sgx_k.get_k_only_gpu     (COSX K, k=True)
 
@@ -14,7 +15,8 @@ RuntimeWarning with the traceback and calls the original PySCF function
 (gpu/install.py:91-96); an out-of-memory error first retries with half the
 number of G partial copies (down to one), the memory lever of this design, then
 falls back (gpu/install.py:99-109).  ``restore_all()`` on the returned
-HookSet removes every patch.  The semilocal XC stays on PySCF's CPU path.
+HookSet removes every patch.  semilocal=False leaves the semilocal XC on
+PySCF's CPU path (the reference's switch of the same name).
 """
 import traceback
 import warnings
@@ -22,7 +24,7 @@ import warnings
 from .. import lib
 from ..core.hooks import HookSet
 from ..errors import Unsupported, is_oom
-from . import sgx_k, vv10
+from . import sgx_k, vv10, xc
 
 WARN_PREFIX = 'pyscf_wb97mv_fast_amd GPU'
 
@@ -95,10 +97,11 @@ def gpu_available(device=0):
 
 
 def install_gpu(mf, backend=None, precision='mixed', vv10_tile_tol=0.0, k=False,
-                k_tile_tol=1e-11, device=0, sgx_module=None):
-    """Hook mf._numint.nr_nlc_vxc (and, with k=True, sgx_jk.get_k_only);
-    returns a HookSet.  backend=None installs nothing (with a warning) when
-    the library or the GPU is unavailable; backend='hip' insists."""
+                k_tile_tol=1e-11, device=0, sgx_module=None, semilocal=True):
+    """Hook mf._numint.nr_rks (semilocal=True) and nr_nlc_vxc (and, with
+    k=True, sgx_jk.get_k_only); returns a HookSet.  backend=None installs
+    nothing (with a warning) when the library or the GPU is unavailable;
+    backend='hip' insists."""
     ni = getattr(mf, '_numint', None)
     if ni is None:
         raise TypeError('mf has no _numint; not a DFT object?')
@@ -126,6 +129,21 @@ def install_gpu(mf, backend=None, precision='mixed', vv10_tile_tol=0.0, k=False,
                 return orig(mol, grids, xc_code, dm, relativity, hermi, max_memory, verbose)
         return nr_nlc_vxc
 
+    def rks_factory(orig):
+        def nr_rks(mol, grids, xc_code, dms, relativity=0, hermi=1, max_memory=2000,
+                   verbose=None):
+            try:
+                if mol is not ctx.mol:
+                    raise Unsupported('molecule changed since install_gpu')
+                return xc.gpu_nr_rks(ctx.nlc_state, ni, mol, grids, xc_code, dms, relativity,
+                                     hermi, max_memory, verbose)
+            except Exception as exc:      # noqa: BLE001
+                _warn_fallback('nr_rks', exc)
+                return orig(mol, grids, xc_code, dms, relativity, hermi, max_memory, verbose)
+        return nr_rks
+
+    if semilocal:
+        hooks.wrap(ni, 'nr_rks', rks_factory)
     hooks.wrap(ni, 'nr_nlc_vxc', nlc_factory)
 
     if k:

@@ -53,6 +53,8 @@ _SIGS = {
     'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     'wb_boys_eval': (_i, [_vp, _i, _vp, _i, _i, _vp]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'wb_xc_rho': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    'wb_xc_vmat': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_cycle_graph_create': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i,
                                    _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     'wb_cycle_graph_launch': (_i, [_vp]),

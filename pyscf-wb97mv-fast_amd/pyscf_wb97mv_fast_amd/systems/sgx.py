@@ -56,6 +56,17 @@ class SyntheticNumInt:
                    verbose=None):
         raise NotImplementedError('no CPU NumInt offline')
 
+    def nr_rks(self, mol, grids, xc_code, dms, relativity=0, hermi=1, max_memory=2000,
+               verbose=None):
+        raise NotImplementedError('no CPU NumInt offline')
+
+    def _xc_type(self, xc_code):
+        return 'MGGA' if 'wb97m' in str(xc_code).lower().replace('_', '').replace('-', '') \
+            else 'GGA'
+
+    def eval_xc(self, xc_code, rho, spin=0, relativity=0, deriv=1, omega=None, verbose=None):
+        raise NotImplementedError('no libxc offline: the functional is PySCF\'s')
+
 
 class SyntheticRKS:
     def __init__(self, mol, sgx_grids, nlc_grids, xc='wb97m-v'):

@@ -16,11 +16,17 @@ from pyscf_wb97mv_fast_amd.systems import geometry, grids, mole, sgx
 
 class _NumInt(sgx.SyntheticNumInt):
     calls = 0
+    rks_calls = 0
 
     def nr_nlc_vxc(self, mol, grids, xc_code, dm, relativity=0, hermi=1, max_memory=2000,
                    verbose=None):
         type(self).calls += 1
         return 1.0, 2.0, numpy.zeros_like(dm)
+
+    def nr_rks(self, mol, grids, xc_code, dms, relativity=0, hermi=1, max_memory=2000,
+               verbose=None):
+        type(self).rks_calls += 1
+        return 3.0, 4.0, numpy.zeros_like(dms)
 
 
 def _mf():
@@ -69,15 +75,27 @@ def test_install_without_gpu_falls_back_with_warning():
             if install.gpu_available():
                 pytest.skip('a GPU is present: the fallback path is not exercised')
             r = mf._numint.nr_nlc_vxc(mf.mol, mf.nlcgrids, 'wb97m-v', dm)
+            x = mf._numint.nr_rks(mf.mol, mf.nlcgrids, 'wb97m-v', dm)
             k = mod.get_k_only(mf.with_df, dm)
         assert r[0] == 1.0 and _NumInt.calls >= 1
+        assert x[0] == 3.0 and _NumInt.rks_calls >= 1
         assert numpy.all(k == 7.0) and mod.calls == [mf.with_df]
         msgs = [str(x.message) for x in w if issubclass(x.category, RuntimeWarning)]
         assert any(m.startswith(install.WARN_PREFIX + ' nr_nlc_vxc') for m in msgs)
+        assert any(m.startswith(install.WARN_PREFIX + ' nr_rks') for m in msgs)
         assert any(m.startswith(install.WARN_PREFIX + ' get_k_only') for m in msgs)
     finally:
         hooks.restore_all()
-    assert 'nr_nlc_vxc' not in vars(mf._numint)
+    assert 'nr_nlc_vxc' not in vars(mf._numint) and 'nr_rks' not in vars(mf._numint)
+
+
+def test_semilocal_false_leaves_nr_rks_alone():
+    mf = _mf()
+    hooks = install.install_gpu(mf, backend='hip', semilocal=False)
+    try:
+        assert 'nr_rks' not in vars(mf._numint) and 'nr_nlc_vxc' in vars(mf._numint)
+    finally:
+        hooks.restore_all()
 
 
 def test_k_hook_leaves_other_sgx_objects_alone():
@@ -138,7 +156,7 @@ def test_no_gpu_installs_nothing():
     mf = _mf()
     with pytest.warns(RuntimeWarning):
         hooks = install.install_gpu(mf)
-    assert 'nr_nlc_vxc' not in vars(mf._numint)
+    assert 'nr_nlc_vxc' not in vars(mf._numint) and 'nr_rks' not in vars(mf._numint)
     hooks.restore_all()
 
 
