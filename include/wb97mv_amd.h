@@ -165,6 +165,17 @@ int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, cons
 int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, int deriv,
                double* out);
 
+/* Atomic partition weights of atom-centred grid points (host arrays): for
+ * point i owned by atom owner[i], out[i] = P_owner / sum_a P_a with the
+ * Stratmann-Scuseria-Frisch cells (a = 0.64), the scheme of the package's
+ * own Becke grids.  Replaces the cell-partition step of the SGX / XC grid
+ * build the reference runs on the host (grids.build(with_non0tab=True),
+ * gpu/install.py:164; with_df.build(level=...), staging/schedule.py:391).
+ * natm < 1, an owner outside 0..natm-1 or two atoms at one position is
+ * WB_INVALID. */
+int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
+                      const double* coords, const int* owner, double* out);
+
 /* Boys function F_0..F_nmax (nmax <= 6) at n values T (host arrays), out
  * (nmax + 1, n), by the device routines the exchange kernels use: FP32
  * (WB_PREC_MIXED, T rounded to FP32) or FP64.  Replaces boys_eval(cp, T,

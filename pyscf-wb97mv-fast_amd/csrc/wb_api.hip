@@ -15,6 +15,10 @@
 #include "wb_boys.h"
 #include "wb_internal.h"
 #include <memory>
+#include <atomic>
+#include <cstdlib>
+#include <exception>
+#include <thread>
 
 namespace wb {
 
@@ -172,12 +176,23 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
             order.emplace_back(a, b);
         }
     std::sort(order.begin(), order.end());
-    int comps_a[10][3], comps_b[10][3];
-    long long total = 0;
-    for (auto [a, b] : order) {
+    // Per pair (independent, on all host threads): the primitive-pair data
+    // (most important first), the pair record and its screening record;
+    // then, in pair order, the class arrays (pp0 = running primitive count)
+    struct PairOut {
+        bool keep = false;
+        PairRec pr{};
+        PairScreen ps{};
+        std::vector<double> P, p, pref, E;
+        std::vector<float4> pbc;
+    };
+    std::vector<PairOut> outs(order.size());
+    auto make_pair = [&](size_t idx) {
+        const auto [a, b] = order[idx];
+        PairOut& o = outs[idx];
         const ShellInfo &A = B.shells[a], &Bs = B.shells[b];
         const int la = A.l, lb = Bs.l;
-        ClassData& C = B.cls[class_id(la, lb)];
+        int comps_a[10][3], comps_b[10][3];
         const double AB[3] = {A.cx - Bs.cx, A.cy - Bs.cy, A.cz - Bs.cz};
         const double ab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
         // shell-pair magnitude cutoff (gpu/shellpairs.py:290-294)
@@ -189,17 +204,16 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
                                  std::exp(-ea * eb / (ea + eb) * ab2);
                 mag = std::max(mag, v);
             }
-        if (mag <= pair_cut) continue;
-        ++total;
-        PairRec pr{};
+        if (mag <= pair_cut) return;
+        o.keep = true;
+        PairRec& pr = o.pr;
         pr.a0 = A.ao0; pr.b0 = Bs.ao0;
-        pr.pp0 = C.nprims;
         pr.shA = a; pr.shB = b;
         pr.diag = (a == b);
         pr.onec = (ab2 == 0.0);
         cart_idx_list(la, comps_a);
         cart_idx_list(lb, comps_b);
-        PairScreen ps{};
+        PairScreen& ps = o.ps;
         double pmin = 1e300, beta = 0.0;
         double Pd[3] = {0, 0, 0};
         std::vector<double> Ps;
@@ -242,16 +256,16 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
             }
         std::stable_sort(prims.begin(), prims.end(),
                          [](const Prim& x, const Prim& y) { return x.key > y.key; });
+        const int L = la + lb;
         for (const Prim& pm : prims) {
-            C.P.insert(C.P.end(), pm.P, pm.P + 3);
-            C.p.push_back(pm.p);
-            C.pref.push_back(pm.pref);
-            const int L = la + lb;
+            o.P.insert(o.P.end(), pm.P, pm.P + 3);
+            o.p.push_back(pm.p);
+            o.pref.push_back(pm.pref);
             for (int d = 0; d < 3; ++d)
                 for (int ii = 0; ii <= la; ++ii)
                     for (int jj = 0; jj <= lb; ++jj)
-                        for (int t = 0; t <= L; ++t) C.E.push_back(pm.E[d][ii][jj][t]);
-            C.pbc.push_back(make_float4((float)pm.P[0], (float)pm.P[1], (float)pm.P[2],
+                        for (int t = 0; t <= L; ++t) o.E.push_back(pm.E[d][ii][jj][t]);
+            o.pbc.push_back(make_float4((float)pm.P[0], (float)pm.P[1], (float)pm.P[2],
                                         (float)(0.8862269254527580 * pm.beta /
                                                 std::sqrt(0.75 * pm.p) * (1 + 1e-5))));
             beta += pm.beta;
@@ -267,15 +281,57 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         ps.delta = (float)(delta * (1 + 1e-6) + 1e-5);
         ps.beta = (float)(beta * (1 + 1e-5));
         ps.q = (float)(0.75 * pmin * (1 - 1e-6));
-        pr.npp = C.nprims + A.nprim * Bs.nprim - pr.pp0;
+        pr.npp = (int)prims.size();
+    };
+    {
+        // WB_HOST_THREADS caps the host threads (the result does not depend on it)
+        size_t cap = std::max(1u, std::thread::hardware_concurrency());
+        if (const char* e = std::getenv("WB_HOST_THREADS")) cap = std::max(1, std::atoi(e));
+        const size_t nthr = std::min<size_t>({16, cap, order.size() / 256 + 1});
+        std::atomic<size_t> next{0};
+        std::vector<std::exception_ptr> errs(nthr);
+        auto worker = [&](size_t t) {
+            try {
+                for (;;) {
+                    const size_t i0 = next.fetch_add(64);
+                    if (i0 >= order.size()) break;
+                    const size_t i1 = std::min(order.size(), i0 + 64);
+                    for (size_t i = i0; i < i1; ++i) make_pair(i);
+                }
+            } catch (...) {
+                errs[t] = std::current_exception();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nthr; ++t) pool.emplace_back(worker, t);
+        worker(0);
+        for (auto& th : pool) th.join();
+        for (auto& e : errs) if (e) std::rethrow_exception(e);
+    }
+    long long total = 0;
+    for (size_t idx = 0; idx < order.size(); ++idx) {
+        PairOut& o = outs[idx];
+        if (!o.keep) continue;
+        ++total;
+        const ShellInfo &A = B.shells[o.pr.shA], &Bs = B.shells[o.pr.shB];
+        ClassData& C = B.cls[class_id(A.l, Bs.l)];
+        PairRec pr = o.pr;
+        pr.pp0 = C.nprims;
         if (pr.shA >= 32768 || pr.shB >= 32768 || pr.pp0 >= (1 << 25) || pr.npp >= 64)
             throw Error(WB_UNSUPPORTED, "basis too large for the packed screening records");
+        PairScreen ps = o.ps;
         ps.sh = pr.shA | pr.shB << 16;
         ps.pp = pr.pp0 << 6 | pr.npp;
-        C.nprims += A.nprim * Bs.nprim;
+        C.P.insert(C.P.end(), o.P.begin(), o.P.end());
+        C.p.insert(C.p.end(), o.p.begin(), o.p.end());
+        C.pref.insert(C.pref.end(), o.pref.begin(), o.pref.end());
+        C.E.insert(C.E.end(), o.E.begin(), o.E.end());
+        C.pbc.insert(C.pbc.end(), o.pbc.begin(), o.pbc.end());
+        C.nprims += pr.npp;
         C.pairs.push_back(pr);
         C.scr.push_back(ps);
         C.npairs++;
+        o = PairOut();                                   // release as we go
     }
     B.npairs_total = total;
     // shell blocks and pair groups per class
@@ -1227,6 +1283,44 @@ int wb_test_gemm(wb_ctx_t ctx, int kind, int M, int N, int K, const double* A, c
             gemm_nt_reduce(c->stream, M, N, K, dA.as<double>(), dB.p, false, 1, 0,
                            w ? dw.as<double>() : nullptr, dC.as<double>(), ws);
         WB_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * (size_t)M * N, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
+                      const double* coords, const int* owner, double* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (natm < 1 || npts < 0 || !atoms || (npts > 0 && (!coords || !owner || !out)))
+            throw Error(WB_INVALID, "bad grid partition arguments");
+        for (int i = 0; i < npts; ++i)
+            if (owner[i] < 0 || owner[i] >= natm)
+                throw Error(WB_INVALID, "grid point owner outside 0..natm-1");
+        // atom-pair distances and their inverses (0 on the diagonal)
+        std::vector<double> rab((size_t)natm * natm), irab((size_t)natm * natm);
+        for (int a = 0; a < natm; ++a)
+            for (int b = 0; b < natm; ++b) {
+                const double dx = atoms[3 * a] - atoms[3 * b], dy = atoms[3 * a + 1] - atoms[3 * b + 1],
+                             dz = atoms[3 * a + 2] - atoms[3 * b + 2];
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (a != b && r == 0.0) throw Error(WB_INVALID, "two atoms at one position");
+                rab[(size_t)a * natm + b] = r;
+                irab[(size_t)a * natm + b] = a == b ? 0.0 : 1.0 / r;
+            }
+        WB_HIP(hipSetDevice(c->device));
+        DevBuf dA, dR, dI, dX, dO, dW;
+        upload(dA, atoms, sizeof(double) * 3 * natm, c->stream);
+        upload(dR, rab.data(), sizeof(double) * rab.size(), c->stream);
+        upload(dI, irab.data(), sizeof(double) * irab.size(), c->stream);
+        upload(dX, coords, sizeof(double) * 3 * (size_t)npts, c->stream);
+        upload(dO, owner, sizeof(int) * (size_t)npts, c->stream);
+        dW.ensure(sizeof(double) * (size_t)std::max(npts, 1));
+        launch_ssf_partition(c->stream, npts, dX.as<double>(), dO.as<int>(), natm,
+                             dA.as<double>(), dR.as<double>(), dI.as<double>(), dW.as<double>());
+        if (npts)
+            WB_HIP(hipMemcpyAsync(out, dW.p, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost,
+                                  c->stream));
         WB_HIP(hipStreamSynchronize(c->stream));
     })
 }

@@ -221,6 +221,11 @@ void launch_dot_planes(hipStream_t s, int nao, int Ppad, const double* A, const 
 void launch_axpy(hipStream_t s, size_t n, double a, const double* x, double* y);
 // wv = (w v0 / 2, w v1, w v2, w v3, w v4 / 2), v and wv [5][Ppad]
 void launch_xc_weights(hipStream_t s, int Ppad, const double* w, const double* v, double* wv);
+// Stratmann-Scuseria-Frisch partition weight of each point's owner atom;
+// xyz (npts, 3), atoms (natm, 3), rab / inv_rab (natm, natm) (0 on the diagonal)
+void launch_ssf_partition(hipStream_t s, int npts, const double* xyz, const int* owner, int natm,
+                          const double* atoms, const double* rab, const double* inv_rab,
+                          double* out);
 
 // VV10
 // VV10 double sum over n = *nptr <= ncap compacted points.  pts: (x, y, z,

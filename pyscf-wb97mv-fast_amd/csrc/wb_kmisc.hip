@@ -1344,6 +1344,64 @@ void launch_axpy(hipStream_t s, size_t n, double a, const double* x, double* y)
     WB_HIP(hipGetLastError());
 }
 
+// Stratmann-Scuseria-Frisch partition (Chem. Phys. Lett. 257, 213 (1996)):
+// for the point r owned by atom o, w = P_o / sum_a P_a with the cell function
+// P_a = prod_{b != a} s(mu_ab), mu_ab = (|r - A| - |r - B|) / |A - B|,
+// s = (1 - g(mu / a)) / 2, g(x) = (35x - 35x^3 + 21x^5 - 5x^7) / 16 on |x| < 1
+// and sign(x) outside (a = 0.64).  Only atoms with |r - A| - |r - N| <
+// a |A - N| (N the nearest atom) can have P_a > 0; the others are skipped.
+// One thread per point; the atom index is uniform across the wave, so the
+// atom coordinates and the distance tables come through scalar loads.
+constexpr double SSF_A = 0.64;
+__global__ __launch_bounds__(256) void ssf_partition_kernel(
+    int npts, const double* __restrict__ xyz, const int* __restrict__ owner, int natm,
+    const double* __restrict__ atoms, const double* __restrict__ rab,
+    const double* __restrict__ inv_rab, double* __restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= npts) return;
+    const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    const int own = owner[i];
+    auto dist = [&](int b) {
+        const double dx = x - atoms[3 * b], dy = y - atoms[3 * b + 1], dz = z - atoms[3 * b + 2];
+        return sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    double dn = dist(0);
+    int near = 0;
+    for (int b = 1; b < natm; ++b) {
+        const double d = dist(b);
+        if (d < dn) { dn = d; near = b; }
+    }
+    double tot = 0.0, pown = 0.0;
+    for (int a = 0; a < natm; ++a) {
+        const double da = dist(a);
+        if (!(da - dn < SSF_A * rab[(size_t)near * natm + a] + 1e-12)) continue;
+        double P = 1.0;
+        const double* ir = inv_rab + (size_t)a * natm;
+        for (int b = 0; b < natm; ++b) {
+            if (b == a) continue;
+            const double mu = (da - dist(b)) * ir[b];
+            if (mu <= -SSF_A) continue;                  // s = 1
+            if (mu >= SSF_A) { P = 0.0; break; }         // s = 0
+            const double nu = mu / SSF_A, n2 = nu * nu;
+            const double g = nu * (35.0 + n2 * (-35.0 + n2 * (21.0 - 5.0 * n2))) / 16.0;
+            P *= 0.5 * (1.0 - g);
+        }
+        tot += P;
+        if (a == own) pown = P;
+    }
+    out[i] = pown / tot;
+}
+void launch_ssf_partition(hipStream_t s, int npts, const double* xyz, const int* owner, int natm,
+                          const double* atoms, const double* rab, const double* inv_rab,
+                          double* out)
+{
+    if (npts > 0)
+        hipLaunchKernelGGL(ssf_partition_kernel, dim3((npts + 255) / 256), dim3(256), 0, s, npts,
+                           xyz, owner, natm, atoms, rab, inv_rab, out);
+    WB_HIP(hipGetLastError());
+}
+
 // wv = (w v0 / 2, w v1, w v2, w v3, w v4 / 2) from the chain-ruled potential
 // v [5][Ppad]: the operand factors of the semilocal potential matrix
 __global__ void xc_weights_kernel(int Ppad, const double* __restrict__ w,

@@ -52,6 +52,7 @@ _SIGS = {
     'wb_vv10_sum': (_i, [_vp, _i, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
     'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     'wb_boys_eval': (_i, [_vp, _i, _vp, _i, _i, _vp]),
+    'wb_grid_partition': (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'wb_xc_rho': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_xc_vmat': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),

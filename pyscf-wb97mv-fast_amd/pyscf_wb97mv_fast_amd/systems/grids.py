@@ -12,7 +12,9 @@ this builds its own grid of the same construction:
 * pruning: lower angular degree inside 0.25 and 0.5 of the Bragg radius
   (the NWChem scheme's idea);
 * Stratmann-Scuseria-Frisch partition (Becke cells with compact support);
-  points with weight below ``w_cut`` are dropped.
+  points with weight below ``w_cut`` are dropped.  ``Grids(..., device=ctx)``
+  computes the partition on the GPU (gpu/grids.py); the host version here is
+  the default and the test reference.
 
 Radial/angular sizes per level follow PySCF's ``RAD_GRIDS``/``ANG_ORDER``
 tables for periods 1-3.  ``Grids`` mirrors the attributes the hot path reads
@@ -103,15 +105,18 @@ def _ssf_partition(coords, atoms, owner, chunk=8192):
 class Grids:
     """Becke grid with PySCF-style attributes (coords/weights in bohr)."""
 
-    def __init__(self, mol, level=3, prune=True, w_cut=1e-15):
+    def __init__(self, mol, level=3, prune=True, w_cut=1e-15, device=None):
         self.mol = mol
+        self.device = device
         self.level = int(level)
         self.prune = prune
         self.w_cut = w_cut
         self.coords = None
         self.weights = None
 
-    def build(self, **kwargs):
+    def raw_points(self):
+        """All atom-centred points before the partition: coords (n, 3),
+        radial x angular weights (n,), owner atom (n,)."""
         mol = self.mol
         atoms = mol.atom_coords()
         charges = mol.atom_charges()
@@ -142,10 +147,16 @@ class Grids:
                 pts.append(atoms[ia] + ri * u)
                 wts.append(wri * wa)
                 owner.append(numpy.full(len(wa), ia))
-        coords = numpy.concatenate(pts)
-        w = numpy.concatenate(wts)
-        owner = numpy.concatenate(owner)
-        w = w * _ssf_partition(coords, atoms, owner)
+        return numpy.concatenate(pts), numpy.concatenate(wts), numpy.concatenate(owner)
+
+    def build(self, **kwargs):
+        atoms = self.mol.atom_coords()
+        coords, w, owner = self.raw_points()
+        if self.device is not None:
+            from ..gpu.grids import ssf_partition
+            w = w * ssf_partition(self.device, coords, atoms, owner)
+        else:
+            w = w * _ssf_partition(coords, atoms, owner)
         keep = w > self.w_cut
         self.coords = numpy.ascontiguousarray(coords[keep])
         self.weights = numpy.ascontiguousarray(w[keep])

@@ -98,3 +98,34 @@ def vv10_literal(coords, q, W0, kappa):
         U[i] = Tu.sum()
         W[i] = (Tu * R2).sum()
     return F, U, W
+
+
+def ssf_polynomial():
+    """Coefficients (x, x^3, x^5, x^7) of the odd degree-7 polynomial g with
+    g(1) = 1 and g'(1) = g''(1) = g'''(1) = 0 (the Stratmann-Scuseria-Frisch
+    switching function), solved from those conditions."""
+    pw = numpy.array([1, 3, 5, 7])
+    rows = [numpy.ones(4), pw.astype(float), pw * (pw - 1.0), pw * (pw - 1.0) * (pw - 2.0)]
+    return numpy.linalg.solve(numpy.array(rows), numpy.array([1.0, 0, 0, 0]))
+
+
+def ssf_brute(coords, atoms, owner, a=0.64):
+    """Owner's cell weight P_o / sum_a P_a over ALL atoms (no nearest-atom
+    screening), P_a = prod_{b != a} (1 - g(clip(mu_ab / a))) / 2."""
+    c = ssf_polynomial()
+    out = numpy.empty(len(coords))
+    natm = len(atoms)
+    for i, r in enumerate(coords):
+        d = [math.dist(r, A) for A in atoms]
+        P = []
+        for ia in range(natm):
+            p = 1.0
+            for ib in range(natm):
+                if ib == ia:
+                    continue
+                mu = (d[ia] - d[ib]) / math.dist(atoms[ia], atoms[ib])
+                x = min(1.0, max(-1.0, mu / a))
+                p *= 0.5 * (1.0 - (c[0] * x + c[1] * x ** 3 + c[2] * x ** 5 + c[3] * x ** 7))
+            P.append(p)
+        out[i] = P[owner[i]] / sum(P)
+    return out

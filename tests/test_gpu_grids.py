@@ -1,0 +1,102 @@
+"""Device grid construction: the Stratmann-Scuseria-Frisch partition
+(wb_grid_partition, gpu/grids.py) against a brute-force cell weight over all
+atoms (tests/independent.py) and against the host partition of
+systems/grids.py (pinned in test_grids.py); and the host-threaded shell-pair
+build, whose result must not depend on the thread count."""
+import os
+
+import numpy
+import pytest
+
+from independent import ssf_brute
+from pyscf_wb97mv_fast_amd import lib
+from pyscf_wb97mv_fast_amd.basis import pack_mol
+from pyscf_wb97mv_fast_amd.errors import GpuError
+from pyscf_wb97mv_fast_amd.gpu import grids as dgrids
+from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def ctx():
+    return lib.Context(0)
+
+
+def test_partition_vs_brute_force(ctx):
+    for seed in range(3):
+        rng = numpy.random.default_rng(seed)
+        natm = 7
+        atoms = rng.uniform(-3, 3, (natm, 3))
+        owner = rng.integers(0, natm, 500)
+        coords = atoms[owner] + rng.normal(size=(500, 3)) * rng.uniform(0.1, 3, (500, 1))
+        got = dgrids.ssf_partition(ctx, coords, atoms, owner)
+        ref = ssf_brute(coords, atoms, owner)
+        assert numpy.allclose(got, ref, rtol=0, atol=1e-13), seed
+
+
+def test_partition_vs_host_water8(ctx):
+    mol = mole.Mole(geometry.water_cluster(8, seed=0), 'def2-svp')
+    coords, w, owner = grids.Grids(mol, level=1).raw_points()
+    atoms = mol.atom_coords()
+    print('points', len(coords), 'atoms', len(atoms), flush=True)
+    got = dgrids.ssf_partition(ctx, coords, atoms, owner)
+    ref = grids._ssf_partition(coords, atoms, owner)
+    assert numpy.abs(got - ref).max() < 1e-13
+
+
+def test_device_grid_matches_host_grid(ctx):
+    mol = mole.Mole(geometry.WATER_DIMER, 'def2-svp')
+    gh = grids.Grids(mol, level=3).build()
+    gd = grids.Grids(mol, level=3, device=ctx).build()
+    assert gd.size == gh.size
+    assert numpy.array_equal(gd.coords, gh.coords)
+    # weight = quadrature weight x partition; the partitions agree to 1e-13
+    coords, wq, _ = grids.Grids(mol, level=3).raw_points()
+    wq = wq[numpy.isin(coords.view('V24').ravel(), gh.coords.view('V24').ravel())]
+    assert len(wq) == gh.size
+    assert (numpy.abs(gd.weights - gh.weights) <= 1e-13 * wq).all()
+
+
+def test_partition_edge_cases(ctx):
+    atoms = numpy.array([[0.0, 0, 0], [1.5, 0, 0]])
+    assert dgrids.ssf_partition(ctx, numpy.zeros((0, 3)), atoms, []).shape == (0,)
+    # one atom: every point belongs to it entirely
+    one = dgrids.ssf_partition(ctx, numpy.random.default_rng(0).normal(size=(50, 3)),
+                               atoms[:1], numpy.zeros(50, dtype=int))
+    assert (one == 1.0).all()
+    with pytest.raises(GpuError):
+        dgrids.ssf_partition(ctx, numpy.zeros((2, 3)), atoms, [0, 2])
+    with pytest.raises(GpuError):
+        dgrids.ssf_partition(ctx, numpy.ones((1, 3)), numpy.zeros((2, 3)), [0])
+
+
+def _k(ctx, basis, grid, dm):
+    om = numpy.array([0.0, 0.3])
+    out = numpy.empty((2, basis.nao, basis.nao))
+    lib.check(lib.load().wb_k_build(ctx.h, basis.h, grid.h, lib.ptr(lib.f64(dm)), 2, lib.ptr(om),
+                                    1e-11, lib.PREC['mixed'], lib.ptr(out)))
+    return out
+
+
+def test_threaded_shell_pairs_are_thread_count_independent(ctx):
+    mol = mole.Mole(geometry.water_cluster(8, seed=1), 'def2-svp')
+    g = grids.Grids(mol, level=0).build()
+    pk = pack_mol(mol)
+    dm = guess.huckel_density(mol)
+    grid = lib.DeviceGrid(ctx, g.coords, g.weights)
+    res = []
+    old = os.environ.get('WB_HOST_THREADS')
+    try:
+        for nt in ('1', '3', '16'):
+            os.environ['WB_HOST_THREADS'] = nt
+            b = lib.DeviceBasis(ctx, pk)
+            res.append((b.npairs, b.nprims, _k(ctx, b, grid, dm)))
+    finally:
+        if old is None:
+            os.environ.pop('WB_HOST_THREADS', None)
+        else:
+            os.environ['WB_HOST_THREADS'] = old
+    for r in res[1:]:
+        assert r[:2] == res[0][:2]
+        assert numpy.array_equal(r[2], res[0][2])
