@@ -49,10 +49,12 @@ def log(*a):
 
 
 def build_workload(seed=0):
+    from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
     t = time.perf_counter()
     mol = mole.Mole(geometry.water_cluster(27, seed=seed), 'def2-svp')
-    g = grids.Grids(mol, level=1).build()
+    # the grid's cell partition runs on the GPU (wb_grid_partition)
+    g = grids.Grids(mol, level=1, device=lib.Context(0)).build()
     dm = guess.huckel_density(mol)
     log('workload: nao %d, %d grid points, setup %.1f s' % (mol.nao, g.size,
                                                             time.perf_counter() - t))
