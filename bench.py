@@ -72,6 +72,128 @@ def measured_traffic():
     return None
 
 
+def _clusters(count, nmax, seed):
+    """count water clusters, n ~ U{4..nmax} (seeded), def2-SVP, level-1 grids
+    (cell partition on the GPU) and extended-Hueckel densities."""
+    from pyscf_wb97mv_fast_amd import lib
+    from pyscf_wb97mv_fast_amd.basis import pack_mol
+    from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
+    rng = numpy.random.default_rng(seed)
+    pctx = lib.Context(0)
+    out = []
+    for i, n in enumerate(rng.integers(4, nmax + 1, count)):
+        mol = mole.Mole(geometry.water_cluster(int(n), seed=seed + i), 'def2-svp')
+        g = grids.Grids(mol, level=1, device=pctx).build()
+        out.append((int(n), pack_mol(mol), g.coords, g.weights, guess.huckel_density(mol)))
+    return out
+
+
+def bench_batch(args):
+    """One K(1/r) + K(erf) + VV10 cycle for each of N molecules of uneven
+    size: separate single-molecule calls vs one wb_cycle_batch call (1 and
+    --lanes lanes), mixed and FP64."""
+    from pyscf_wb97mv_fast_amd import lib
+    from pyscf_wb97mv_fast_amd.gpu import batch
+    t = time.perf_counter()
+    mols = _clusters(args.batch, args.batch_max, 11)
+    ctx = lib.Context(0)
+    bases = [lib.DeviceBasis(ctx, m[1]) for m in mols]
+    grs = [lib.DeviceGrid(ctx, m[2], m[3]) for m in mols]
+    log('batch workload: %d clusters, nao %d..%d, %d grid points in all, setup %.1f s'
+        % (len(mols), min(b.nao for b in bases), max(b.nao for b in bases),
+           sum(len(m[3]) for m in mols), time.perf_counter() - t))
+    om = numpy.array([0.0, OMEGA])
+    L = lib.load()
+
+    def separate(prec):
+        res = []
+        for b, g, m in zip(bases, grs, mols):
+            k = numpy.empty((2, b.nao, b.nao))
+            lib.check(L.wb_k_build(ctx.h, b.h, g.h, lib.ptr(m[4]), 2, lib.ptr(om), K_TOL, prec,
+                                   lib.ptr(k)))
+            o2, v = numpy.empty(2), numpy.empty((b.nao, b.nao))
+            lib.check(L.wb_nlc_vxc(ctx.h, b.h, g.h, lib.ptr(m[4]), NLC[0], NLC[1], 0.0, prec,
+                                   lib.ptr(o2), lib.ptr(v)))
+            res.append((k, o2))
+        return res
+
+    def batched(precision, lanes):
+        jobs = [batch.CycleJob(b, g, m[4], g, m[4]) for b, g, m in zip(bases, grs, mols)]
+        return batch.run_batch(ctx, jobs, om, K_TOL, NLC, 0.0, precision, lanes)
+
+    def timed(fn, *a):
+        fn(*a)                                      # sizes every workspace
+        t0 = time.perf_counter()
+        r = fn(*a)
+        return time.perf_counter() - t0, r
+
+    t_sep, sep = timed(separate, 0)
+    log('separate calls (mixed): %.3f s' % t_sep)
+    t_b1, _ = timed(batched, 'mixed', 1)
+    log('batch, 1 lane (mixed): %.3f s' % t_b1)
+    t_bl, jobs = timed(batched, 'mixed', args.lanes)
+    log('batch, %d lanes (mixed): %.3f s' % (args.lanes, t_bl))
+    t_64, jobs64 = timed(batched, 'fp64', args.lanes)
+    log('batch, %d lanes (fp64): %.3f s' % (args.lanes, t_64))
+    same = all(numpy.array_equal(j.k, s_[0]) and numpy.array_equal(j.nlc, s_[1])
+               for j, s_ in zip(jobs, sep))
+    dex = [abs(-0.25 * float(numpy.einsum('ij,ji->', m[4], 0.15 * (j.k[0] - j6.k[0])
+                                           + 0.85 * (j.k[1] - j6.k[1]))))
+           for j, j6, m in zip(jobs, jobs64, mols)]
+    dnl = [abs(j.nlc[1] - j6.nlc[1]) for j, j6 in zip(jobs, jobs64)]
+    ctx.trim()
+    print(json.dumps({
+        'metric': 'batched COSX K(1/r)+K(erf)+VV10 molecule-cycles/s', 'value': len(mols) / t_bl,
+        'unit': 'molecule-cycles/s', 'n_gpus': 1, 'higher_is_better': True,
+        'dtype': 'f32 integrals + f64 GEMMs', 'data': 'synthetic',
+        'config': {'workload': '%d water clusters, n ~ U{4..%d} (seed 11), def2-SVP, level-1 '
+                              'grids' % (len(mols), args.batch_max), 'lanes': args.lanes},
+        'seconds': {'separate_calls': t_sep, 'batch_1_lane': t_b1,
+                    'batch_%d_lanes' % args.lanes: t_bl, 'batch_fp64_%d_lanes' % args.lanes: t_64},
+        'batch_equals_separate': bool(same),
+        'max_abs_dE_x_vs_fp64': max(dex), 'max_abs_dE_nl_vs_fp64': max(dnl),
+        'mixed_vs_fp64': t_64 / t_bl}), flush=True)
+
+
+def bench_serve(args):
+    """Seeded Poisson arrivals of SCF-cycle jobs (water clusters of
+    U{4..--batch-max} molecules) through the continuous-batching loop."""
+    from pyscf_wb97mv_fast_amd.serving import CycleServer, Request
+    t = time.perf_counter()
+    mols = _clusters(args.serve, args.batch_max, 23)
+    gaps = numpy.random.default_rng(5).exponential(1.0 / args.rate, len(mols))
+    log('serve workload: %d jobs, setup %.1f s' % (len(mols), time.perf_counter() - t))
+    out = {}
+    for precision in ('mixed', 'fp64'):
+        srv = CycleServer(slots=args.lanes, hbm_budget=200e9, precision=precision)
+        # no warm-up: the first jobs on each slot include its workspace allocation
+        t0 = time.perf_counter()
+        futs = []
+        for gap, m in zip(gaps, mols):
+            time.sleep(float(gap))
+            futs.append(srv.submit(Request(pack=m[1], kgrid=(m[2], m[3]),
+                                           nlcgrid=(m[2], m[3]), dm=m[4], cycles=args.cycles)))
+        res = [f.result() for f in futs]
+        srv.close()
+        st = srv.stats()
+        st['wall_s'] = time.perf_counter() - t0
+        st['energies'] = [float(r['nlc'][1]) for r in res]
+        log('%s: %d jobs, %.2f jobs/s, p50 %.3f s, p99 %.3f s, at most %d at once'
+            % (precision, st['jobs'], st['jobs_per_s'], st['p50_s'], st['p99_s'],
+               st['max_active']))
+        out[precision] = st
+    dnl = max(abs(a - b) for a, b in zip(out['mixed'].pop('energies'), out['fp64'].pop('energies')))
+    print(json.dumps({
+        'metric': 'served SCF-cycle jobs/s and p50/p99 time to result', 'value':
+        out['mixed']['jobs_per_s'], 'unit': 'jobs/s', 'n_gpus': 1, 'higher_is_better': True,
+        'data': 'synthetic', 'config': {
+            'workload': '%d jobs, Poisson arrivals %.1f/s (seed 5), water clusters n ~ '
+                        'U{4..%d}, def2-SVP, level-1 grids, %d K(1/r)+K(erf)+VV10 cycles each'
+                        % (len(mols), args.rate, args.batch_max, args.cycles),
+            'slots': args.lanes, 'hbm_budget_bytes': 200e9},
+        'mixed': out['mixed'], 'fp64': out['fp64'], 'max_abs_dE_nl_vs_fp64': dnl}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -85,6 +207,17 @@ def main():
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a device copy of MIB MiB (5 times): the known-size '
                          'kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters')
+    ap.add_argument('--batch', type=int, default=0, metavar='N',
+                    help='batched mode: N water clusters (n ~ U{4..--batch-max}), one '
+                         'K(1/r) + K(erf) + VV10 cycle each, separate calls vs one '
+                         'wb_cycle_batch; prints its own JSON line')
+    ap.add_argument('--batch-max', type=int, default=64)
+    ap.add_argument('--lanes', type=int, default=4)
+    ap.add_argument('--serve', type=int, default=0, metavar='N',
+                    help='serving mode: N jobs with seeded Poisson arrivals (--rate per '
+                         's) through the continuous-batching loop; prints its own JSON line')
+    ap.add_argument('--rate', type=float, default=2.0)
+    ap.add_argument('--cycles', type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -103,6 +236,10 @@ def main():
         print(json.dumps({'copy_probe_bytes_read': 4 * n, 'copy_probe_bytes_written': 4 * n,
                           'launches': 5}), flush=True)
         return
+    if args.batch:
+        return bench_batch(args)
+    if args.serve:
+        return bench_serve(args)
     from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.basis import pack_mol
 

@@ -58,6 +58,12 @@ int wb_version(void);
  * stream (the reference's per-flow streams, gpu/install.py:118-135). */
 int wb_ctx_create(int device, void* stream, wb_ctx_t* out);
 int wb_ctx_destroy(wb_ctx_t ctx);
+/* Device bytes held by the context's grow-only workspaces (its batch lanes
+ * included), and their release (they regrow on the next call; the call waits
+ * for the context's streams).  No counterpart in the reference (CuPy's pool);
+ * used for admission by HBM footprint when serving many molecules. */
+int wb_ctx_mem(wb_ctx_t ctx, long long* bytes);
+int wb_ctx_trim(wb_ctx_t ctx);
 /* number of G partial copies of the exchange kernel (default 8) */
 int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
 /* mixed form: accumulate G in FP64 (1) or FP32 (0, default) */
@@ -153,6 +159,33 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
 /* enqueue one replay on the context's stream (asynchronous: wb_sync waits) */
 int wb_cycle_graph_launch(wb_graph_t graph);
 int wb_cycle_graph_destroy(wb_graph_t graph);
+
+/* One molecule's cycle in a batch: a K build (kgrid != NULL) and/or the VV10
+ * term (nlcgrid != NULL).  Host buffers, C-contiguous, nao = the basis's. */
+typedef struct wb_cycle_job {
+    wb_basis_t basis;
+    wb_grid_t kgrid;        /* K build on this grid, or NULL */
+    const double* qd;       /* in: Q dm (nao, nao) */
+    double* k;              /* out: (nop, nao, nao) raw K */
+    wb_grid_t nlcgrid;      /* VV10 on this grid, or NULL */
+    const double* dm;       /* in: dm (nao, nao) */
+    double* nlc;            /* out: (nelec, exc) */
+    double* vmat;           /* out: (nao, nao) */
+} wb_cycle_job;
+
+/* Cycles of many molecules of uneven size, drained largest first from one
+ * shared queue by `lanes` (1..16) host threads, each with its own stream and
+ * workspaces on the context's device (kept for the next call), so that the
+ * jobs' work overlaps on the device.  Each job's outputs equal wb_k_build's
+ * and wb_nlc_vxc's for it alone, bit for bit.  job_ms (optional, njob):
+ * each job's completion time in ms from the start of the call.  The
+ * reference has no batch path (one molecule per SCF; staging/schedule.py:
+ * 99-452 drives it); a job is the work of one get_k_only call
+ * (gpu/sgx_k.py:663-751) per operator and one nr_nlc_vxc call
+ * (gpu/vv10.py:268-342).  A job error stops the batch and names the job. */
+int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
+                   const double* omegas, double k_tol, double nlc_b, double nlc_C,
+                   double vv_tol, int precision, int lanes, double* job_ms);
 
 /* VV10 kernel sums on given points (coords n x 3): F = -1.5 sum T, U, W as
  * libdft's VXC_vv10nlc returns them (gpu/vv10.py:167-240). */

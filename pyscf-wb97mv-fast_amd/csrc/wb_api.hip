@@ -435,6 +435,7 @@ static void upload_basis(Basis& B, hipStream_t s)
 static const void* op_consts(ClassData& C, const std::vector<double>& omegas, bool f32,
                              hipStream_t s)
 {
+    std::lock_guard<std::mutex> lock(C.opc_mu);
     auto it = C.opc.find(omegas);
     if (it == C.opc.end()) {
         const int L = C.la + C.lb, nc = L + 2, erl = C.erl, nop = (int)omegas.size();
@@ -775,6 +776,47 @@ int wb_ctx_create(int device, void* stream, wb_ctx_t* out)
     })
 }
 
+namespace wb {
+size_t Ctx::ws_bytes() const
+{
+    const DevBuf* b[] = {&ws_X, &ws_Xv, &ws_F, &ws_G, &ws_tasks, &ws_counts, &ws_fmax,
+                         &ws_Kp, &ws_K, &ws_D, &ws_T, &ws_rho, &ws_vv, &ws_B,
+                         &ws_red, &ws_io, &ws_mask, &ws_skips, &ws_work};
+    size_t n = 0;
+    for (const DevBuf* d : b) n += d->bytes;
+    for (const auto& l : lanes) n += l->ws_bytes();
+    return n;
+}
+void Ctx::trim()
+{
+    DevBuf* b[] = {&ws_X, &ws_Xv, &ws_F, &ws_G, &ws_tasks, &ws_counts, &ws_fmax,
+                   &ws_Kp, &ws_K, &ws_D, &ws_T, &ws_rho, &ws_vv, &ws_B,
+                   &ws_red, &ws_io, &ws_mask, &ws_skips, &ws_work};
+    for (DevBuf* d : b) d->release();
+    for (auto& l : lanes) l->trim();
+}
+}  // namespace wb
+
+int wb_ctx_mem(wb_ctx_t ctx, long long* bytes)
+{
+    WB_TRY({
+        if (!ctx || !bytes) throw Error(WB_INVALID, "null argument");
+        *bytes = (long long)((Ctx*)ctx)->ws_bytes();
+    })
+}
+
+int wb_ctx_trim(wb_ctx_t ctx)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c) throw Error(WB_INVALID, "null context");
+        WB_HIP(hipSetDevice(c->device));
+        WB_HIP(hipStreamSynchronize(c->stream));
+        for (auto& l : c->lanes) WB_HIP(hipStreamSynchronize(l->stream));
+        c->trim();
+    })
+}
+
 int wb_ctx_destroy(wb_ctx_t ctx)
 {
     WB_TRY({
@@ -782,6 +824,11 @@ int wb_ctx_destroy(wb_ctx_t ctx)
         if (!c) return WB_OK;
         WB_HIP(hipSetDevice(c->device));
         WB_HIP(hipStreamSynchronize(c->stream));
+        for (auto& l : c->lanes) {
+            WB_HIP(hipStreamSynchronize(l->stream));
+            (void)hipStreamDestroy(l->stream);
+        }
+        c->lanes.clear();
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
     })
@@ -984,6 +1031,137 @@ static void xc_vmat_dev(Ctx& c, Basis& B, Grid& G, const double* v, bool mgga, d
             launch_axpy(s, (size_t)nao * nao, 1.0, Vt, d_vmat);
         }
     }
+}
+
+// ---------------------------------------------------------- cycle batch ---
+// Many molecules' cycles (K build and/or VV10) drained from one shared queue
+// by `lanes` host threads, each driving a sub-context with its own stream and
+// workspaces, so small molecules' launches overlap on the device and large
+// ones keep it full.  Jobs are taken largest first.  Each job runs exactly
+// the single-molecule code on a context of its own, so its outputs equal
+// wb_k_build's / wb_nlc_vxc's for it alone, bit for bit.
+static void run_cycle_job(Ctx& w, const wb_cycle_job& j, int nop, const double* omegas,
+                          double k_tol, double b, double Cc, double vv_tol, int precision)
+{
+    Basis& B = *(Basis*)j.basis;
+    const size_t nn = (size_t)B.nao * B.nao;
+    if (j.kgrid) {
+        double* d = (double*)w.ws_K.ensure(sizeof(double) * nn * (1 + nop));
+        WB_HIP(hipMemcpyAsync(d, j.qd, sizeof(double) * nn, hipMemcpyHostToDevice, w.stream));
+        k_build_dev(w, B, *(Grid*)j.kgrid, d, nop, omegas, k_tol, precision, d + nn);
+        WB_HIP(hipMemcpyAsync(j.k, d + nn, sizeof(double) * nn * nop, hipMemcpyDeviceToHost,
+                              w.stream));
+    }
+    if (j.nlcgrid) {
+        double* d = (double*)w.ws_io.ensure(sizeof(double) * (2 * nn + 2));
+        WB_HIP(hipMemcpyAsync(d, j.dm, sizeof(double) * nn, hipMemcpyHostToDevice, w.stream));
+        nlc_dev(w, B, *(Grid*)j.nlcgrid, d, b, Cc, vv_tol, precision, d + 2 * nn, d + nn);
+        WB_HIP(hipMemcpyAsync(j.vmat, d + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, w.stream));
+        WB_HIP(hipMemcpyAsync(j.nlc, d + 2 * nn, sizeof(double) * 2, hipMemcpyDeviceToHost,
+                              w.stream));
+    }
+    WB_HIP(hipStreamSynchronize(w.stream));
+}
+
+static Ctx& batch_lane(Ctx& c, size_t i)
+{
+    while (c.lanes.size() <= i) {
+        auto l = std::make_unique<Ctx>();
+        l->device = c.device;
+        WB_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+        l->own_stream = true;
+        std::vector<float> t32;
+        std::vector<double> t64;
+        boys_tables(t32, t64);
+        upload(l->d_b32, t32.data(), sizeof(float) * t32.size(), l->stream);
+        upload(l->d_b64, t64.data(), sizeof(double) * t64.size(), l->stream);
+        c.lanes.push_back(std::move(l));
+    }
+    Ctx& l = *c.lanes[i];
+    l.ncopy = c.ncopy;
+    l.g64 = c.g64;
+    return l;
+}
+
+int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
+                   const double* omegas, double k_tol, double b, double C, double vv_tol,
+                   int precision, int lanes, double* job_ms)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (njob < 0 || (njob > 0 && !jobs)) throw Error(WB_INVALID, "bad job list");
+        if (lanes < 1 || lanes > 16) throw Error(WB_INVALID, "lanes must be 1..16");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        bool anyk = false;
+        for (int i = 0; i < njob; ++i) {
+            const wb_cycle_job& j = jobs[i];
+            if (!j.basis) throw Error(WB_INVALID, "job " + std::to_string(i) + ": null basis");
+            if (j.kgrid && (!j.qd || !j.k))
+                throw Error(WB_INVALID, "job " + std::to_string(i) + ": K build needs qd and k");
+            if (j.nlcgrid && (!j.dm || !j.nlc || !j.vmat))
+                throw Error(WB_INVALID, "job " + std::to_string(i) + ": VV10 needs dm, nlc, vmat");
+            anyk |= j.kgrid != nullptr;
+        }
+        if (anyk && (nop < 1 || nop > 8 || !omegas)) throw Error(WB_INVALID, "nop out of range");
+        WB_HIP(hipSetDevice(c->device));
+        WB_HIP(hipStreamSynchronize(c->stream));     // work queued on the caller's context first
+        // largest first: nao^2 x (K points + VV10 points) as the size estimate
+        std::vector<int> order(njob);
+        std::vector<double> est(njob);
+        for (int i = 0; i < njob; ++i) {
+            const Basis& B = *(const Basis*)jobs[i].basis;
+            const double pk = jobs[i].kgrid ? ((const Grid*)jobs[i].kgrid)->Ppad : 0;
+            const double pv = jobs[i].nlcgrid ? ((const Grid*)jobs[i].nlcgrid)->Ppad : 0;
+            est[i] = (double)B.nao * B.nao * (pk + pv);
+            order[i] = i;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return est[x] > est[y]; });
+        const size_t nl = (size_t)std::max(1, std::min(lanes, njob));
+        std::vector<Ctx*> lc(nl);
+        for (size_t i = 0; i < nl; ++i) lc[i] = &batch_lane(*c, i);
+        std::atomic<int> next{0};
+        std::atomic<bool> failed{false};
+        std::vector<std::string> errs(nl);
+        std::vector<int> errcode(nl, WB_OK);
+        const auto t0 = std::chrono::steady_clock::now();
+        auto worker = [&](size_t li) {
+            Ctx& w = *lc[li];
+            int cur = -1;
+            try {
+                WB_HIP(hipSetDevice(w.device));
+                for (;;) {
+                    const int k = next.fetch_add(1);
+                    if (k >= njob || failed.load()) break;
+                    const int i = cur = order[k];
+                    run_cycle_job(w, jobs[i], nop, omegas, k_tol, b, C, vv_tol, precision);
+                    if (job_ms)
+                        job_ms[i] = std::chrono::duration<double, std::milli>(
+                                        std::chrono::steady_clock::now() - t0).count();
+                }
+            } catch (const Error& e) {
+                errcode[li] = e.code;
+                errs[li] = e.what();
+            } catch (const std::bad_alloc&) {
+                errcode[li] = WB_OOM;
+                errs[li] = "host out of memory";
+            } catch (const std::exception& e) {
+                errcode[li] = WB_DEVICE_ERROR;
+                errs[li] = e.what();
+            }
+            if (errcode[li] != WB_OK) {
+                failed = true;
+                errs[li] = "job " + std::to_string(cur) + ": " + errs[li];
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nl; ++t) pool.emplace_back(worker, t);
+        worker(0);
+        for (auto& th : pool) th.join();
+        for (size_t t = 0; t < nl; ++t)
+            if (errcode[t] != WB_OK) throw Error(errcode[t], "cycle batch: " + errs[t]);
+    })
 }
 
 // ------------------------------------------------------- cycle graphs ---

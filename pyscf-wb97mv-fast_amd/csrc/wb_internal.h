@@ -2,6 +2,7 @@
 #pragma once
 #include <map>
 #include <memory>
+#include <mutex>
 #include "wb_common.h"
 
 namespace wb {
@@ -93,6 +94,7 @@ struct ClassData {
     // operator set (omegas) -> primitive-pair records (FP32, FP64), built
     // on first use
     std::map<std::vector<double>, std::pair<std::shared_ptr<DevBuf>, std::shared_ptr<DevBuf>>> opc;
+    std::mutex opc_mu;                 // batch lanes may share the basis
 };
 
 struct Basis {
@@ -140,6 +142,12 @@ struct Ctx {
     double phase_ms[16] = {0};
     void mark(int phase);
     void start();
+    // sub-contexts of wb_cycle_batch (own streams and workspaces), made on first use
+    std::vector<std::unique_ptr<Ctx>> lanes;
+    // device bytes held by the workspaces (lanes included); release them all
+    // (the caller makes sure no work is in flight)
+    size_t ws_bytes() const;
+    void trim();
 };
 
 enum Phase { PH_AO = 0, PH_FGEMM, PH_SCREEN, PH_MD, PH_KGEMM, PH_NLC_AO, PH_NLC_RHO,

@@ -26,6 +26,15 @@ _lock = threading.Lock()
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
+
+
+class CycleJob(ctypes.Structure):
+    """wb_cycle_job (include/wb97mv_amd.h)."""
+    _fields_ = [('basis', ctypes.c_void_p), ('kgrid', ctypes.c_void_p), ('qd', ctypes.c_void_p),
+                ('k', ctypes.c_void_p), ('nlcgrid', ctypes.c_void_p), ('dm', ctypes.c_void_p),
+                ('nlc', ctypes.c_void_p), ('vmat', ctypes.c_void_p)]
+
+
 _SIGS = {
     'wb_last_error': (ctypes.c_char_p, []),
     'wb_version': (_i, []),
@@ -53,6 +62,10 @@ _SIGS = {
     'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     'wb_boys_eval': (_i, [_vp, _i, _vp, _i, _i, _vp]),
     'wb_grid_partition': (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
+    'wb_cycle_batch': (_i, [_vp, _i, ctypes.POINTER(CycleJob), _i, _vp, _d, _d, _d, _d, _i, _i,
+                            _vp]),
+    'wb_ctx_mem': (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    'wb_ctx_trim': (_i, [_vp]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'wb_xc_rho': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_xc_vmat': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
@@ -135,6 +148,17 @@ class Context:
     @property
     def last_ms(self):
         return float(load().wb_last_ms(self.h))
+
+    @property
+    def mem_bytes(self):
+        """Device bytes held by the context's workspaces (batch lanes included)."""
+        v = ctypes.c_longlong()
+        check(load().wb_ctx_mem(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def trim(self):
+        """Release the workspaces (they regrow on the next call)."""
+        check(load().wb_ctx_trim(self.h))
 
 
 class DeviceBasis:
