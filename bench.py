@@ -85,6 +85,8 @@ def _clusters(count, nmax, seed):
         mol = mole.Mole(geometry.water_cluster(int(n), seed=seed + i), 'def2-svp')
         g = grids.Grids(mol, level=1, device=pctx).build()
         out.append((int(n), pack_mol(mol), g.coords, g.weights, guess.huckel_density(mol)))
+        if (i + 1) % 8 == 0:
+            log('  %d/%d molecules set up' % (i + 1, count))
     return out
 
 
@@ -129,12 +131,15 @@ def bench_batch(args):
 
     t_sep, sep = timed(separate, 0)
     log('separate calls (mixed): %.3f s' % t_sep)
-    t_b1, _ = timed(batched, 'mixed', 1)
+    ctx.trim()                                      # the lanes get the device memory
+    t_b1, jobs1 = timed(batched, 'mixed', 1)
     log('batch, 1 lane (mixed): %.3f s' % t_b1)
-    t_bl, jobs = timed(batched, 'mixed', args.lanes)
-    log('batch, %d lanes (mixed): %.3f s' % (args.lanes, t_bl))
-    t_64, jobs64 = timed(batched, 'fp64', args.lanes)
-    log('batch, %d lanes (fp64): %.3f s' % (args.lanes, t_64))
+    t_bn, jobsn = timed(batched, 'mixed', args.lanes)
+    log('batch, %d lanes (mixed): %.3f s' % (args.lanes, t_bn))
+    # the reported configuration: the faster lane count
+    lanes, t_bl, jobs = (1, t_b1, jobs1) if t_b1 <= t_bn else (args.lanes, t_bn, jobsn)
+    t_64, jobs64 = timed(batched, 'fp64', lanes)
+    log('batch, %d lane(s) (fp64): %.3f s' % (lanes, t_64))
     same = all(numpy.array_equal(j.k, s_[0]) and numpy.array_equal(j.nlc, s_[1])
                for j, s_ in zip(jobs, sep))
     dex = [abs(-0.25 * float(numpy.einsum('ij,ji->', m[4], 0.15 * (j.k[0] - j6.k[0])
@@ -147,9 +152,9 @@ def bench_batch(args):
         'unit': 'molecule-cycles/s', 'n_gpus': 1, 'higher_is_better': True,
         'dtype': 'f32 integrals + f64 GEMMs', 'data': 'synthetic',
         'config': {'workload': '%d water clusters, n ~ U{4..%d} (seed 11), def2-SVP, level-1 '
-                              'grids' % (len(mols), args.batch_max), 'lanes': args.lanes},
+                              'grids' % (len(mols), args.batch_max), 'lanes': lanes},
         'seconds': {'separate_calls': t_sep, 'batch_1_lane': t_b1,
-                    'batch_%d_lanes' % args.lanes: t_bl, 'batch_fp64_%d_lanes' % args.lanes: t_64},
+                    'batch_%d_lanes' % args.lanes: t_bn, 'batch_fp64': t_64},
         'batch_equals_separate': bool(same),
         'max_abs_dE_x_vs_fp64': max(dex), 'max_abs_dE_nl_vs_fp64': max(dnl),
         'mixed_vs_fp64': t_64 / t_bl}), flush=True)

@@ -66,6 +66,21 @@ int wb_ctx_mem(wb_ctx_t ctx, long long* bytes);
 int wb_ctx_trim(wb_ctx_t ctx);
 /* number of G partial copies of the exchange kernel (default 8) */
 int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
+/* Autotuner of the K build's run-time choices on the caller's molecule:
+ * times (minimum of `reps` builds, HIP events) the G copy count over
+ * {1, 2, 4, 8, 16} (those whose G fits a quarter of the free device memory),
+ * then, class by class, fused vs two-pass exchange kernels for the classes
+ * with a fused two-operator form, and keeps the fastest on the context.
+ * report (optional, 5): best copies, fuse mask, best ms, ms of the default
+ * (8 copies, all fused), configurations timed.  Results stay within the
+ * mixed form's tolerance of the default's (the copies change the FP32
+ * summation order).  No counterpart in the reference (fixed launch shapes).
+ * get/set_tuning read or restore a configuration (e.g. a cached one). */
+int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd, int nop,
+                    const double* omegas, double tile_tol, int precision, int reps,
+                    double* report);
+int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask);
+int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask);
 /* mixed form: accumulate G in FP64 (1) or FP32 (0, default) */
 int wb_ctx_set_g64(wb_ctx_t ctx, int on);
 int wb_sync(wb_ctx_t ctx);
@@ -173,6 +188,15 @@ typedef struct wb_cycle_job {
     double* vmat;           /* out: (nao, nao) */
 } wb_cycle_job;
 
+/* Device bytes of one job's workspaces: *after = what the context will
+ * hold once the job has run on it (its grow-only workspaces as they are,
+ * grown where the job needs more), *alone = what the job needs from empty
+ * (after wb_ctx_trim).  Exact (the sizes the flows allocate); the basis and
+ * grid data are the caller's and not included.  No counterpart in the
+ * reference; used for admission by HBM footprint. */
+int wb_job_bytes(wb_ctx_t ctx, const wb_cycle_job* job, int nop, int precision,
+                 long long* after, long long* alone);
+
 /* Cycles of many molecules of uneven size, drained largest first from one
  * shared queue by `lanes` (1..16) host threads, each with its own stream and
  * workspaces on the context's device (kept for the next call), so that the
@@ -182,7 +206,10 @@ typedef struct wb_cycle_job {
  * reference has no batch path (one molecule per SCF; staging/schedule.py:
  * 99-452 drives it); a job is the work of one get_k_only call
  * (gpu/sgx_k.py:663-751) per operator and one nr_nlc_vxc call
- * (gpu/vv10.py:268-342).  A job error stops the batch and names the job. */
+ * (gpu/vv10.py:268-342).  A job that runs out of device memory is retried
+ * once alone, after every workspace of the context and its lanes has been
+ * released; any other job error stops the batch and names the job.  The
+ * context must not be used from another thread during the call. */
 int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
                    const double* omegas, double k_tol, double nlc_b, double nlc_C,
                    double vv_tol, int precision, int lanes, double* job_ms);

@@ -16,6 +16,8 @@
 #include "wb_internal.h"
 #include <memory>
 #include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <cstdlib>
 #include <exception>
 #include <thread>
@@ -654,7 +656,7 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
         const int* ids_k = ids + cls_off[k];
         const unsigned* skips_k = skips + cls_off[k];
         const int zrows = first ? zrow : 0;
-        if (nop == 2 && md_fuse(C.la, C.lb)) {
+        if (nop == 2 && md_fuse(C.la, C.lb) && (c.fuse_mask >> k & 1u)) {
             // both operators in one pass over the tasks
             const void* rc = op_consts(C, {omegas[0], omegas[1]}, f32, s);
             MdMasks mk{masks, mopstride, zrow, first};
@@ -779,20 +781,14 @@ int wb_ctx_create(int device, void* stream, wb_ctx_t* out)
 namespace wb {
 size_t Ctx::ws_bytes() const
 {
-    const DevBuf* b[] = {&ws_X, &ws_Xv, &ws_F, &ws_G, &ws_tasks, &ws_counts, &ws_fmax,
-                         &ws_Kp, &ws_K, &ws_D, &ws_T, &ws_rho, &ws_vv, &ws_B,
-                         &ws_red, &ws_io, &ws_mask, &ws_skips, &ws_work};
     size_t n = 0;
-    for (const DevBuf* d : b) n += d->bytes;
+    for (const DevBuf* d : bufs) n += d->bytes;
     for (const auto& l : lanes) n += l->ws_bytes();
     return n;
 }
 void Ctx::trim()
 {
-    DevBuf* b[] = {&ws_X, &ws_Xv, &ws_F, &ws_G, &ws_tasks, &ws_counts, &ws_fmax,
-                   &ws_Kp, &ws_K, &ws_D, &ws_T, &ws_rho, &ws_vv, &ws_B,
-                   &ws_red, &ws_io, &ws_mask, &ws_skips, &ws_work};
-    for (DevBuf* d : b) d->release();
+    for (DevBuf* d : bufs) d->release();
     for (auto& l : lanes) l->trim();
 }
 }  // namespace wb
@@ -953,6 +949,121 @@ int wb_k_build(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd,
     })
 }
 
+// Times the K build on the caller's molecule for each configuration of the
+// context's run-time choices and keeps the fastest: the number of G copies
+// (1, 2, 4, 8, 16, those whose G fits in a quarter of the free device
+// memory), then, one class at a time, whether a class with a fused
+// two-operator kernel runs it fused or as two passes.  Each configuration is
+// timed as the minimum of `reps` builds (HIP events, after one untimed).
+int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd, int nop,
+                    const double* omegas, double tile_tol, int precision, int reps,
+                    double* report)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* b = (Basis*)basis;
+        auto* g = (Grid*)grid;
+        if (!c || !b || !g || !qd || !omegas) throw Error(WB_INVALID, "null argument");
+        if (nop < 1 || nop > 8) throw Error(WB_INVALID, "nop out of range");
+        if (reps < 1 || reps > 100) throw Error(WB_INVALID, "reps must be 1..100");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        WB_HIP(hipSetDevice(c->device));
+        const size_t nn = (size_t)b->nao * b->nao;
+        hipEvent_t e0, e1;
+        WB_HIP(hipEventCreate(&e0));
+        WB_HIP(hipEventCreate(&e1));
+        int trials = 0;
+        auto time_cfg = [&](int ncopy, unsigned mask) {
+            c->ncopy = ncopy;
+            c->fuse_mask = mask;
+            double* d = (double*)c->ws_K.ensure(sizeof(double) * nn * (1 + nop));
+            WB_HIP(hipMemcpyAsync(d, qd, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+            k_build_dev(*c, *b, *g, d, nop, omegas, tile_tol, precision, d + nn);
+            float best = 1e30f;
+            for (int r = 0; r < reps; ++r) {
+                WB_HIP(hipEventRecord(e0, c->stream));
+                k_build_dev(*c, *b, *g, d, nop, omegas, tile_tol, precision, d + nn);
+                WB_HIP(hipEventRecord(e1, c->stream));
+                WB_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                WB_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best = std::min(best, ms);
+            }
+            ++trials;
+            return best;
+        };
+        const int ncopy0 = c->ncopy;
+        const unsigned mask0 = c->fuse_mask;
+        try {
+            const float t_default = time_cfg(8, ~0u);
+            int best_n = 8;
+            unsigned best_mask = ~0u;
+            float best_t = t_default;
+            size_t free_b = 0, total_b = 0;
+            WB_HIP(hipMemGetInfo(&free_b, &total_b));
+            const bool g64 = precision == WB_PREC_FP64 || c->g64;
+            for (int n : {1, 2, 4, 16}) {
+                const size_t gbytes = (size_t)n * (b->nao + MD_SPARE + 1) * g->Ppad * nop *
+                                      (g64 ? 8 : 4);
+                if (gbytes > (free_b + c->ws_G.bytes) / 4) continue;
+                const float t = time_cfg(n, best_mask);
+                if (t < best_t) { best_t = t; best_n = n; }
+            }
+            if (nop == 2)
+                for (int k = 0; k < NCLASS; ++k) {
+                    const ClassData& C = b->cls[k];
+                    if (!C.npairs || !md_fuse(C.la, C.lb)) continue;
+                    const unsigned m = best_mask & ~(1u << k);
+                    const float t = time_cfg(best_n, m);
+                    if (t < best_t) { best_t = t; best_mask = m; }
+                }
+            c->ncopy = best_n;
+            c->fuse_mask = best_mask;
+            // the largest configuration tried sized G: let it regrow to the kept one
+            WB_HIP(hipStreamSynchronize(c->stream));
+            c->ws_G.release();
+            c->ws_mask.release();
+            if (report) {
+                report[0] = best_n;
+                report[1] = (double)best_mask;
+                report[2] = best_t;
+                report[3] = t_default;
+                report[4] = trials;
+            }
+        } catch (...) {
+            c->ncopy = ncopy0;
+            c->fuse_mask = mask0;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        WB_HIP(hipEventDestroy(e0));
+        WB_HIP(hipEventDestroy(e1));
+    })
+}
+
+int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !ncopy || !fuse_mask) throw Error(WB_INVALID, "null argument");
+        *ncopy = c->ncopy;
+        *fuse_mask = c->fuse_mask;
+    })
+}
+
+int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c) throw Error(WB_INVALID, "null context");
+        if (ncopy < 1 || ncopy > 64) throw Error(WB_INVALID, "ncopy out of range");
+        c->ncopy = ncopy;
+        c->fuse_mask = fuse_mask;
+    })
+}
+
 int wb_k_stats(wb_ctx_t ctx, wb_grid_t grid, long long* out2)
 {
     WB_TRY({
@@ -1063,6 +1174,60 @@ static void run_cycle_job(Ctx& w, const wb_cycle_job& j, int nop, const double* 
     WB_HIP(hipStreamSynchronize(w.stream));
 }
 
+// Bytes each workspace needs for one job (what k_build_dev, nlc_dev and
+// run_cycle_job ensure(); a buffer both flows use needs the larger size).
+static void job_needs(const Ctx& w, const wb_cycle_job& j, int nop, int prec,
+                      size_t need[WS_COUNT])
+{
+    for (int b = 0; b < WS_COUNT; ++b) need[b] = 0;
+    const Basis& B = *(const Basis*)j.basis;
+    const size_t nao = B.nao, nn = nao * nao;
+    const bool f32 = prec == WB_PREC_MIXED;
+    auto at_least = [&](int b, size_t v) { need[b] = std::max(need[b], v); };
+    if (j.kgrid) {
+        const Grid& G = *(const Grid*)j.kgrid;
+        const size_t P = G.Ppad, nsub = G.nsub;
+        size_t nitem = 0;
+        for (const auto& C : B.cls) nitem += (size_t)C.ngroups * nsub;
+        const size_t zrow = nao + MD_SPARE;
+        const bool g64 = !f32 || w.g64;
+        at_least(WS_X, 8 * nao * P);
+        at_least(WS_D, 8 * nn);
+        at_least(WS_F, (f32 ? 4 : 8) * (nao + MD_SPARE) * P);
+        at_least(WS_FMAX, 4 * nsub * B.nshell);
+        at_least(WS_COUNTS, 4 * NCLASS * nsub);
+        at_least(WS_WORK, 8 * 2 * NCLASS);
+        at_least(WS_TASKS, 4 * std::max<size_t>(nitem, 1));
+        at_least(WS_SKIPS, 4 * std::max<size_t>(nitem, 1));
+        at_least(WS_G, (g64 ? 8 : 4) * (zrow + 1) * P * w.ncopy * nop);
+        if (zrow <= 32 * MD_MASK_WORDS)
+            at_least(WS_MASK, 4 * (size_t)w.ncopy * nsub * MD_MASK_WORDS * nop);
+        at_least(WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, true));
+        at_least(WS_K, 8 * nn * (1 + nop));
+    }
+    if (j.nlcgrid) {
+        const Grid& G = *(const Grid*)j.nlcgrid;
+        const size_t P = G.Ppad;
+        at_least(WS_XV, 8 * nao * P);
+        at_least(WS_T, 8 * nao * P);
+        at_least(WS_RHO, 8 * 4 * P);
+        at_least(WS_RED, ao_rho_bytes(B, (int)P));
+        at_least(WS_VV, 8 * 5 * P);
+        at_least(WS_B, vv_layout(G.nsub, (int)P, !f32).total);
+        at_least(WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, false));
+        at_least(WS_K, 8 * nn);
+        at_least(WS_IO, 8 * (2 * nn + 2));
+    }
+}
+
+// bytes the lane holds after running a job with these needs
+static size_t held_after(const Ctx& w, const size_t need[WS_COUNT])
+{
+    size_t n = 0;
+    for (int b = 0; b < WS_COUNT; ++b) n += std::max(w.bufs[b]->bytes, need[b]);
+    return n;
+}
+
 static Ctx& batch_lane(Ctx& c, size_t i)
 {
     while (c.lanes.size() <= i) {
@@ -1079,8 +1244,27 @@ static Ctx& batch_lane(Ctx& c, size_t i)
     }
     Ctx& l = *c.lanes[i];
     l.ncopy = c.ncopy;
+    l.fuse_mask = c.fuse_mask;
     l.g64 = c.g64;
     return l;
+}
+
+int wb_job_bytes(wb_ctx_t ctx, const wb_cycle_job* job, int nop, int precision,
+                 long long* after, long long* alone)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !job || !job->basis) throw Error(WB_INVALID, "null argument");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        if (job->kgrid && (nop < 1 || nop > 8)) throw Error(WB_INVALID, "nop out of range");
+        size_t need[WS_COUNT];
+        job_needs(*c, *job, nop, precision, need);
+        size_t sum = 0;
+        for (size_t v : need) sum += v;
+        if (after) *after = (long long)held_after(*c, need);
+        if (alone) *alone = (long long)sum;
+    })
 }
 
 int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
@@ -1125,6 +1309,102 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
         std::atomic<bool> failed{false};
         std::vector<std::string> errs(nl);
         std::vector<int> errcode(nl, WB_OK);
+        // Device memory: every lane's workspaces grow to the largest job it
+        // has run.  A lane starts its next job only when the bytes it will
+        // hold afterwards (exact, from the job's buffer needs) plus what the
+        // other lanes hold or will hold fit the device memory that was free
+        // at the start plus the lanes' own; an idle lane that holds memory
+        // releases it when the job does not fit, and a job that does not fit
+        // even alone runs with the device to itself.  A job that still runs
+        // out of memory is retried once alone after every workspace of the
+        // context and its lanes has been released.
+        size_t free_b = 0, total_b = 0;
+        WB_HIP(hipMemGetInfo(&free_b, &total_b));
+        size_t lanes_b = 0;
+        for (Ctx* l : lc) lanes_b += l->ws_bytes();
+        const size_t reserve = std::min<size_t>(total_b / 32, (size_t)8 << 30);
+        const size_t budget = free_b + lanes_b > reserve ? free_b + lanes_b - reserve : 0;
+        std::mutex gate_mu;
+        std::condition_variable gate_cv;
+        std::vector<size_t> committed(nl);
+        for (size_t t = 0; t < nl; ++t) committed[t] = lc[t]->ws_bytes();
+        std::vector<char> exited(nl, 0);
+        int running = 0;
+        bool exclusive = false;
+        auto others_of = [&](size_t li) {
+            size_t o = 0;
+            for (size_t t = 0; t < nl; ++t)
+                if (t != li) o += committed[t];
+            return o;
+        };
+        // wait for room; returns with the lane counted as running
+        auto enter = [&](size_t li, Ctx& w, const size_t* need) {
+            std::unique_lock<std::mutex> lk(gate_mu);
+            for (;;) {
+                if (!exclusive) {
+                    const size_t after = held_after(w, need), others = others_of(li);
+                    if (others + after <= budget || (others == 0 && running == 0)) {
+                        committed[li] = after;
+                        ++running;
+                        return;
+                    }
+                    // release memory held by idle lanes: this one, and lanes
+                    // that have left the queue (their streams are synchronized)
+                    bool freed = false;
+                    if (w.ws_bytes() > 0) {
+                        (void)hipStreamSynchronize(w.stream);
+                        w.trim();
+                        committed[li] = 0;
+                        freed = true;
+                    }
+                    for (size_t t = 0; t < nl; ++t)
+                        if (exited[t] && committed[t] > 0) {
+                            lc[t]->trim();
+                            committed[t] = 0;
+                            freed = true;
+                        }
+                    if (freed) {
+                        gate_cv.notify_all();
+                        continue;
+                    }
+                }
+                gate_cv.wait(lk);
+            }
+        };
+        auto leave = [&](size_t li, Ctx& w) {
+            std::lock_guard<std::mutex> lk(gate_mu);
+            --running;
+            committed[li] = w.ws_bytes();
+            gate_cv.notify_all();
+        };
+        auto run_alone = [&](size_t li, Ctx& w, int i) {
+            {
+                std::unique_lock<std::mutex> lk(gate_mu);
+                gate_cv.wait(lk, [&] { return !exclusive; });
+                exclusive = true;
+                gate_cv.wait(lk, [&] { return running == 0; });
+                c->trim();                   // every lane is idle and synchronized
+            }
+            auto done = [&] {
+                std::lock_guard<std::mutex> lk(gate_mu);
+                exclusive = false;
+                for (size_t t = 0; t < nl; ++t) committed[t] = lc[t]->ws_bytes();
+                gate_cv.notify_all();
+            };
+            try {
+                run_cycle_job(w, jobs[i], nop, omegas, k_tol, b, C, vv_tol, precision);
+            } catch (...) {
+                (void)hipStreamSynchronize(w.stream);
+                done();
+                throw;
+            }
+            done();
+            (void)li;
+        };
+        // WB_BATCH_FAIL_ONCE=i: job i's first attempt reports out-of-memory
+        // before it starts (exercises the retry path from the host)
+        const char* fe = std::getenv("WB_BATCH_FAIL_ONCE");
+        const int fail_once = fe ? std::atoi(fe) : -1;
         const auto t0 = std::chrono::steady_clock::now();
         auto worker = [&](size_t li) {
             Ctx& w = *lc[li];
@@ -1135,7 +1415,33 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
                     const int k = next.fetch_add(1);
                     if (k >= njob || failed.load()) break;
                     const int i = cur = order[k];
-                    run_cycle_job(w, jobs[i], nop, omegas, k_tol, b, C, vv_tol, precision);
+                    size_t need[WS_COUNT];
+                    job_needs(w, jobs[i], nop, precision, need);
+                    enter(li, w, need);
+                    bool oom = false;
+                    try {
+                        if (i == fail_once) throw Error(WB_OOM, "forced out-of-memory");
+                        run_cycle_job(w, jobs[i], nop, omegas, k_tol, b, C, vv_tol, precision);
+                    } catch (const Error& e) {
+                        if (e.code != WB_OOM) {
+                            (void)hipStreamSynchronize(w.stream);
+                            leave(li, w);
+                            throw;
+                        }
+                        oom = true;
+                    } catch (...) {
+                        (void)hipStreamSynchronize(w.stream);
+                        leave(li, w);
+                        throw;
+                    }
+                    if (oom) {
+                        (void)hipGetLastError();             // clear the failed allocation
+                        (void)hipStreamSynchronize(w.stream);
+                        leave(li, w);
+                        run_alone(li, w, i);
+                    } else {
+                        leave(li, w);
+                    }
                     if (job_ms)
                         job_ms[i] = std::chrono::duration<double, std::milli>(
                                         std::chrono::steady_clock::now() - t0).count();
@@ -1154,6 +1460,9 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
                 failed = true;
                 errs[li] = "job " + std::to_string(cur) + ": " + errs[li];
             }
+            std::lock_guard<std::mutex> lk(gate_mu);
+            exited[li] = 1;
+            gate_cv.notify_all();
         };
         std::vector<std::thread> pool;
         for (size_t t = 1; t < nl; ++t) pool.emplace_back(worker, t);
@@ -1201,6 +1510,7 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
         w.device = c->device;
         w.stream = c->stream;
         w.ncopy = c->ncopy;
+        w.fuse_mask = c->fuse_mask;
         w.g64 = c->g64;
         std::vector<float> t32;
         std::vector<double> t64;

@@ -120,6 +120,9 @@ struct Grid {
 };
 
 // --------------------------------------------------------------- context ---
+// the context's grow-only workspaces, in a fixed order (Ctx::bufs)
+enum WsBuf { WS_X, WS_XV, WS_F, WS_G, WS_TASKS, WS_COUNTS, WS_FMAX, WS_KP, WS_K, WS_D, WS_T,
+             WS_RHO, WS_VV, WS_B, WS_RED, WS_IO, WS_MASK, WS_SKIPS, WS_WORK, WS_COUNT };
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -132,6 +135,9 @@ struct Ctx {
     long long tasks_total = 0, tasks_kept = 0;
     float ms_last = 0.f;
     int ncopy = 8;
+    // classes whose two operators run in one pass (bit = class id; only where
+    // md_fuse allows it); set by wb_ctx_autotune
+    unsigned fuse_mask = ~0u;
     bool g64 = false;          // mixed form: G accumulated in FP64 (default FP32)
     // per-class work of the last K build: kept tasks, kept primitive pairs
     DevBuf ws_work;
@@ -148,6 +154,9 @@ struct Ctx {
     // (the caller makes sure no work is in flight)
     size_t ws_bytes() const;
     void trim();
+    DevBuf* bufs[WS_COUNT] = {&ws_X, &ws_Xv, &ws_F, &ws_G, &ws_tasks, &ws_counts, &ws_fmax,
+                              &ws_Kp, &ws_K, &ws_D, &ws_T, &ws_rho, &ws_vv, &ws_B,
+                              &ws_red, &ws_io, &ws_mask, &ws_skips, &ws_work};
 };
 
 enum Phase { PH_AO = 0, PH_FGEMM, PH_SCREEN, PH_MD, PH_KGEMM, PH_NLC_AO, PH_NLC_RHO,
@@ -251,6 +260,10 @@ void symadd(hipStream_t s, int n, const double* a, double* c);
 // The VV10 part of nr_nlc_vxc after the density: pointwise terms, compaction,
 // double sum, potential factors wv = [4][Ppad] (the operand B = launch_ao_proj
 // of them; vmat = X^T B + (X^T B)^T) and sums = (nelec, excsum).
+struct VvLayout { size_t cnt, off, pt, pk, idx, aux, ts, F, U, W, sym, total; };
+VvLayout vv_layout(int nsub, int Ppad, bool f64);
+// workspace bytes gemm_nt_fast takes (partials, plus the folded operand)
+size_t gemm_nt_ws_bytes(int M, int N, int K, bool fold);
 struct Ctx;
 void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
                const double* origin, const double* rho, double b, double C, double tol, bool f64,

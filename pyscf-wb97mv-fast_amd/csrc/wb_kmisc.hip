@@ -731,6 +731,22 @@ void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const dou
     WB_HIP(hipGetLastError());
 }
 
+// split of the K GEMM over the point axis: <= 96 chunks of a multiple of T_K
+static int nt_split(int K, int& kchunk)
+{
+    int nsplit = std::max(1, std::min(96, K / (T_K * 64)));
+    kchunk = (K + nsplit - 1) / nsplit;
+    kchunk = (kchunk + T_K - 1) / T_K * T_K;
+    return (K + kchunk - 1) / kchunk;
+}
+
+size_t gemm_nt_ws_bytes(int M, int N, int K, bool fold)
+{
+    int kchunk = 0;
+    const int nsplit = nt_split(K, kchunk);
+    return sizeof(double) * (size_t)nsplit * M * N + (fold ? sizeof(double) * (size_t)N * K : 0);
+}
+
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B, bool b_f32,
                   int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
                   const unsigned* masks, const int* row0, int nsub)
@@ -747,10 +763,8 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
     const size_t nbk = (size_t)N * K;
     const int wm = wave_tiles(M), wn = wave_tiles(N);
     const int mt = (M + 32 * wm - 1) / (32 * wm), ntile = mt * ((N + 32 * wn - 1) / (32 * wn));
-    int nsplit = std::max(1, std::min(96, K / (T_K * 64)));
-    int kchunk = (K + nsplit - 1) / nsplit;
-    kchunk = (kchunk + T_K - 1) / T_K * T_K;
-    nsplit = (K + kchunk - 1) / kchunk;
+    int kchunk = 0;
+    const int nsplit = nt_split(K, kchunk);
     const bool fold = b_f32 || nb != 1 || w != nullptr || !aligned16(B) || masks;
     const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
     char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * nbk : 0));
@@ -1420,23 +1434,38 @@ void launch_xc_weights(hipStream_t s, int Ppad, const double* w, const double* v
     WB_HIP(hipGetLastError());
 }
 
+// byte offsets of the VV10 flow's buffers in its workspace
+VvLayout vv_layout(int nsub, int Ppad, bool f64)
+{
+    VvLayout L;
+    const size_t rs = f64 ? 8 : 4;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
+    L.cnt = take(sizeof(int) * (nsub + 1));
+    L.off = take(sizeof(int) * (nsub + 1));
+    L.pt = take(4 * rs * Ppad);
+    L.pk = take(2 * rs * Ppad);
+    L.idx = take(sizeof(int) * Ppad);
+    L.aux = take(sizeof(double) * 3 * Ppad);
+    L.ts = take(sizeof(float4) * 2 * ((Ppad + 63) / 64));
+    L.F = take(sizeof(double) * Ppad);
+    L.U = take(sizeof(double) * Ppad);
+    L.W = take(sizeof(double) * Ppad);
+    L.sym = take(vv10_sym_bytes(Ppad, f64, nullptr));
+    L.total = o;
+    return L;
+}
+
 void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
                const double* origin, const double* rho, double b, double C, double tol, bool f64,
                DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof)
 {
     // workspace layout
-    const size_t rs = f64 ? 8 : 4;
-    size_t o = 0;
-    auto take = [&](size_t bytes) { size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
-    const size_t o_cnt = take(sizeof(int) * (nsub + 1)), o_off = take(sizeof(int) * (nsub + 1));
-    const size_t o_pt = take(4 * rs * Ppad), o_pk = take(2 * rs * Ppad);
-    const size_t o_idx = take(sizeof(int) * Ppad), o_aux = take(sizeof(double) * 3 * Ppad);
+    const VvLayout L = vv_layout(nsub, Ppad, f64);
+    const size_t o_cnt = L.cnt, o_off = L.off, o_pt = L.pt, o_pk = L.pk, o_idx = L.idx,
+                 o_aux = L.aux, o_ts = L.ts, o_F = L.F, o_U = L.U, o_W = L.W, o_sym = L.sym;
     const size_t ntile_max = (Ppad + 63) / 64;
-    const size_t o_ts = take(sizeof(float4) * 2 * ntile_max);
-    const size_t o_F = take(sizeof(double) * Ppad), o_U = take(sizeof(double) * Ppad),
-                 o_W = take(sizeof(double) * Ppad);
-    const size_t o_sym = take(vv10_sym_bytes(Ppad, f64, nullptr));
-    char* base = (char*)ws.ensure(o);
+    char* base = (char*)ws.ensure(L.total);
     int* cnt = (int*)(base + o_cnt);
     int* off = (int*)(base + o_off);
     int* idx = (int*)(base + o_idx);

@@ -1,0 +1,81 @@
+"""Autotuner of the K build's run-time choices (wb_ctx_autotune).
+
+The library times, on the caller's own molecule and grid, the number of G
+partial copies the exchange kernels scatter into (fewer copies: less memory
+and a cheaper fold; more: fewer colliding atomic adds) and, per (la, lb)
+class with a fused two-operator kernel, fused vs two passes, and keeps the
+fastest on the context.  ``Tuner`` caches the choice per size class
+(AOs and grid points rounded up to powers of two, operator count,
+precision), so molecules of similar size reuse one tuning run; the cache
+can be saved as JSON and loaded by a later process.  The reference runs
+fixed launch shapes and has no tuner.
+"""
+import ctypes
+import json
+
+import numpy
+
+from .. import lib
+
+
+def autotune(ctx, basis, grid, qd, omegas=(0.0, 0.3), tile_tol=1e-11, precision='mixed',
+             reps=3):
+    """Tune ``ctx`` on one K build; returns the report as a dict."""
+    om = lib.f64(omegas)
+    qd = lib.f64(qd)
+    if qd.shape != (basis.nao, basis.nao):
+        raise ValueError('qd must be (nao, nao)')
+    rep = numpy.zeros(5)
+    lib.check(lib.load().wb_ctx_autotune(ctx.h, basis.h, grid.h, lib.ptr(qd), len(om),
+                                         lib.ptr(om), float(tile_tol), lib.PREC[precision],
+                                         int(reps), lib.ptr(rep)))
+    return {'ncopy': int(rep[0]), 'fuse_mask': int(rep[1]), 'best_ms': float(rep[2]),
+            'default_ms': float(rep[3]), 'configs': int(rep[4])}
+
+
+def get_tuning(ctx):
+    n, m = ctypes.c_int(), ctypes.c_uint()
+    lib.check(lib.load().wb_ctx_get_tuning(ctx.h, ctypes.byref(n), ctypes.byref(m)))
+    return int(n.value), int(m.value)
+
+
+def set_tuning(ctx, ncopy, fuse_mask):
+    lib.check(lib.load().wb_ctx_set_tuning(ctx.h, int(ncopy), ctypes.c_uint(int(fuse_mask))))
+
+
+def _pow2(n):
+    return 1 << max(0, int(n) - 1).bit_length()
+
+
+class Tuner:
+    """Per-size-class cache of autotune results."""
+
+    def __init__(self, reps=3):
+        self.reps = reps
+        self.table = {}
+
+    @staticmethod
+    def key(nao, npts, nop, precision):
+        return '%d/%d/%d/%s' % (_pow2(nao), _pow2(npts), nop, precision)
+
+    def apply(self, ctx, basis, grid, qd, omegas=(0.0, 0.3), tile_tol=1e-11,
+              precision='mixed'):
+        """Set ctx's tuning for this size class, tuning on this molecule the
+        first time the class is seen; returns (key, entry)."""
+        k = self.key(basis.nao, grid.npts, len(omegas), precision)
+        e = self.table.get(k)
+        if e is None:
+            e = autotune(ctx, basis, grid, qd, omegas, tile_tol, precision, self.reps)
+            self.table[k] = e
+        else:
+            set_tuning(ctx, e['ncopy'], e['fuse_mask'])
+        return k, e
+
+    def save(self, path):
+        with open(path, 'w') as f:
+            json.dump(self.table, f, indent=1, sort_keys=True)
+
+    def load(self, path):
+        with open(path) as f:
+            self.table.update(json.load(f))
+        return self

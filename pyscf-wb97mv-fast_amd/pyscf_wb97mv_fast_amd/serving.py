@@ -6,21 +6,25 @@ given operators and the VV10 term, then an optional host ``update(dm, k,
 nlc, vmat) -> dm`` (the SCF step the caller owns; PySCF's in the reference,
 staging/schedule.py:99-452).  Requests arrive at any time; ``slots`` worker
 threads, each with its own library context (HIP stream and workspaces),
-take them in arrival order as soon as a slot is free and the job's
-estimated device footprint fits the HBM budget beside what the other slots
-hold.  Requests that arrive while others run are admitted into free slots,
-their device work overlaps with the running jobs', and every job runs
-exactly the single-molecule code on a context of its own, so its results do
-not depend on the mix.  Each job's basis, grids and density stay on the
-device for all its cycles and are released when it ends; an idle slot
-releases its workspaces when the next request does not fit, so the head of
-the queue always runs once the busy slots finish.
+take them in arrival order as soon as a slot is free; a taken job starts
+when the device bytes its slot will hold afterwards (exact, wb_job_bytes)
+plus what the other slots hold fit the HBM budget, and no job that arrived
+earlier is still waiting for memory (so jobs start in arrival order).  Requests that arrive
+while others run are admitted into free slots, their device work overlaps
+with the running jobs', and every job runs exactly the single-molecule code
+on a context of its own, so its results do not depend on the mix.  Each
+job's basis, grids and density stay on the device for all its cycles and
+are released when it ends.  Slots keep their grow-only workspaces between
+jobs; a waiting job makes its own slot and the idle slots release theirs,
+so it starts once the busy slots finish, and a job that needs more than the
+whole budget is refused with GpuOOM.
 
 Latency is time from submission to the job's last cycle (p50/p99 over the
 finished jobs), beside throughput in jobs/s.  The reference has no serving
 path.
 """
 import collections
+import ctypes
 import threading
 import time
 from concurrent.futures import Future
@@ -46,25 +50,8 @@ class Request:
     t_submit: float = field(default=None, repr=False)
     t_start: float = field(default=None, repr=False)
     t_end: float = field(default=None, repr=False)
-    est_bytes: int = field(default=0, repr=False)
-
-
-def footprint(nao, kpts=0, nlcpts=0, nop=2, ncopy=8, precision='mixed', npairs=0):
-    """Estimate of the device bytes one job needs (workspaces + basis and grid
-    data): X, F, the G copies and the task lists (at most one word pair per
-    quarter of the shell pairs per 64-point sub-block) for the K build, AO
-    values, density intermediates and the pair-sum buffers for VV10."""
-    pk = -(-kpts // 64) * 64
-    pv = -(-nlcpts // 64) * 64
-    gb = 8 if precision == 'fp64' else 4
-    k = 0
-    if kpts:
-        k = nao * pk * (8 + 8) + ncopy * nop * (nao + 8) * pk * gb \
-            + (pk // 64) * npairs * 2 + 96 * nao * nao * 8 + 8 * nao * nao * (1 + nop)
-    v = 0
-    if nlcpts:
-        v = nao * pv * 8 * 4 + pv * 8 * 32 + 8 * nao * nao * 3
-    return int(1.10 * (k + v)) + (64 << 20)
+    est_bytes: int = field(default=0, repr=False)     # device bytes the job needs from empty
+    _seq: int = field(default=0, repr=False)          # arrival number
 
 
 def _percentile(xs, p):
@@ -76,12 +63,14 @@ class CycleServer:
                  nlc=(6.0, 0.01), vv_tol=0.0, precision='mixed', ncopy=8, runner=None,
                  contexts=None):
         """runner / contexts: for tests of the scheduling logic (a callable
-        runner(ctx, request) -> result and one context object per slot)."""
+        runner(ctx, request) -> result, one context object per slot, and
+        request.est_bytes as the job's memory need)."""
         self.omegas = tuple(omegas)
         self.k_tol, self.nlc, self.vv_tol = float(k_tol), tuple(nlc), float(vv_tol)
         self.precision, self.ncopy = precision, int(ncopy)
         self.budget = int(hbm_budget)
-        self._run = runner or self._run_device
+        self._runner = runner
+        self._own = contexts is None
         self._ctx = contexts if contexts is not None else \
             [lib.Context(device, ncopy=ncopy) for _ in range(slots)]
         n = len(self._ctx)
@@ -89,6 +78,9 @@ class CycleServer:
         self._q = collections.deque()
         self._cv = threading.Condition()
         self._closed = False
+        self._mem_waiters = 0
+        self._seq = 0
+        self._waiting = set()           # arrival numbers of taken jobs waiting for memory
         self.active = 0
         self.max_active = 0
         self.done = []
@@ -99,28 +91,28 @@ class CycleServer:
 
     # -- client side -------------------------------------------------------
     def submit(self, req):
-        if req.est_bytes <= 0:
-            nsh = len(req.pack.l)
-            req.est_bytes = footprint(int(req.pack.nao), len(req.kgrid[1]) if req.kgrid else 0,
-                                      len(req.nlcgrid[1]) if req.nlcgrid else 0,
-                                      len(self.omegas), self.ncopy, self.precision,
-                                      nsh * (nsh + 1) // 2)
         fut = Future()
         req.t_submit = time.perf_counter()
         with self._cv:
             if self._closed:
                 raise RuntimeError('server is closed')
+            self._seq += 1
+            req._seq = self._seq
             self._q.append((req, fut))
             self._cv.notify_all()
         return fut
 
     def close(self):
-        """Finish every submitted request, then stop the slots."""
+        """Finish every submitted request, stop the slots and release the
+        device memory of the contexts the server made."""
         with self._cv:
             self._closed = True
             self._cv.notify_all()
         for t in self._threads:
             t.join()
+        if self._own:
+            for c in self._ctx:
+                c.close()
 
     def stats(self):
         lat = [r.t_end - r.t_submit for r in self.done]
@@ -132,51 +124,73 @@ class CycleServer:
                 'max_active': self.max_active}
 
     # -- slots ---------------------------------------------------------------
+    def _release(self, i):
+        """Trim slot i's workspaces (called with the lock held, slot idle)."""
+        trim = getattr(self._ctx[i], 'trim', None)
+        if trim is not None:
+            trim()
+        self._held[i] = 0
+        self._cv.notify_all()
+
     def _worker(self, i):
         ctx = self._ctx[i]
         while True:
             with self._cv:
-                trim = False
+                while not self._q:
+                    if self._closed:
+                        return
+                    if self._mem_waiters and self._held[i] > 0:
+                        self._release(i)            # idle memory another slot waits for
+                        continue
+                    self._cv.wait()
+                req, fut = self._q.popleft()
+            try:
+                state = self._prepare(ctx, req)
+                after, alone = self._need(ctx, req, state)
+            except BaseException as e:   # noqa: BLE001 -- delivered to the caller
+                fut.set_exception(e)
+                continue
+            with self._cv:
+                ok = True
+                waiting = False
                 while True:
-                    if not self._q:
-                        if self._closed:
-                            return
-                        self._cv.wait()
-                        continue
-                    req, fut = self._q[0]
-                    if req.est_bytes > self.budget:
-                        self._q.popleft()
-                        fut.set_exception(GpuOOM('job needs about %d bytes, budget %d'
-                                                 % (req.est_bytes, self.budget)))
-                        continue
                     others = sum(h for j, h in enumerate(self._held) if j != i)
-                    need = max(self._held[i], req.est_bytes)
-                    if others + need <= self.budget:
-                        self._q.popleft()
-                        self._held[i] = need
-                        self.active += 1
-                        self.max_active = max(self.max_active, self.active)
+                    older = any(q < req._seq for q in self._waiting)
+                    if others + after <= self.budget and not older:
+                        break
+                    if alone > self.budget:
+                        ok = False
                         break
                     if self._held[i] > 0:
-                        # idle and holding memory that may be what blocks the
-                        # head of the queue (here or on another slot): release
-                        # it; once every idle slot has, the head fits
-                        trim = True
-                        break
+                        self._release(i)
+                        after = alone
+                        continue
+                    if not waiting:
+                        waiting = True
+                        self._mem_waiters += 1
+                        self._waiting.add(req._seq)
+                        self._cv.notify_all()       # idle slots may release memory
                     self._cv.wait()
-            if trim:
-                ctx.trim()
-                with self._cv:
-                    self._held[i] = 0
+                if waiting:
+                    self._mem_waiters -= 1
+                    self._waiting.discard(req._seq)
                     self._cv.notify_all()
+                if ok:
+                    self._held[i] = after
+                    self.active += 1
+                    self.max_active = max(self.max_active, self.active)
+            if not ok:
+                fut.set_exception(GpuOOM('job needs %d bytes of device memory, budget %d'
+                                         % (alone, self.budget)))
                 continue
             req.t_start = time.perf_counter()
             try:
-                res = self._run(ctx, req)
+                res = self._run(ctx, req, state)
             except BaseException as e:   # noqa: BLE001 -- delivered to the caller
                 res, err = None, e
             else:
                 err = None
+            del state
             req.t_end = time.perf_counter()
             with self._cv:
                 self.active -= 1
@@ -190,12 +204,42 @@ class CycleServer:
             else:
                 fut.set_exception(err)
 
-    def _run_device(self, ctx, req):
-        L = lib.load()
+    # -- device work -----------------------------------------------------------
+    def _prepare(self, ctx, req):
+        """The job's device state: basis, grids (resident for all its cycles)."""
+        if self._runner is not None:
+            return None
         basis = lib.DeviceBasis(ctx, req.pack)
-        nao = basis.nao
         kg = lib.DeviceGrid(ctx, *req.kgrid) if req.kgrid is not None else None
         vg = lib.DeviceGrid(ctx, *req.nlcgrid) if req.nlcgrid is not None else None
+        return basis, kg, vg
+
+    def _need(self, ctx, req, state):
+        """(bytes the slot holds after the job, bytes the job needs from empty)."""
+        if self._runner is not None:
+            held = int(getattr(ctx, 'mem_bytes', 0) or 0)
+            return max(held, req.est_bytes), req.est_bytes
+        basis, kg, vg = state
+        job = lib.CycleJob()
+        job.basis = basis.h.value
+        job.kgrid = kg.h.value if kg is not None else None
+        job.nlcgrid = vg.h.value if vg is not None else None
+        after, alone = ctypes.c_longlong(), ctypes.c_longlong()
+        lib.check(lib.load().wb_job_bytes(ctx.h, ctypes.byref(job), len(self.omegas),
+                                          lib.PREC[self.precision], ctypes.byref(after),
+                                          ctypes.byref(alone)))
+        req.est_bytes = int(alone.value)
+        return int(after.value), int(alone.value)
+
+    def _run(self, ctx, req, state):
+        if self._runner is not None:
+            return self._runner(ctx, req)
+        return self._run_device(ctx, req, state)
+
+    def _run_device(self, ctx, req, state=None):
+        L = lib.load()
+        basis, kg, vg = state if state is not None else self._prepare(ctx, req)
+        nao = basis.nao
         om = lib.f64(self.omegas)
         dm = lib.f64(req.dm)
         out = {}

@@ -44,32 +44,44 @@ def _overlap_1d(la, lb, a, b, xa, xb):
 
 
 def overlap_matrix(mol):
+    """Analytic AO overlap, shell pairs of one kind (angular momenta, primitive
+    and contraction counts) evaluated together as arrays."""
     ao_loc = mol.ao_loc_nr()
     nao = int(ao_loc[-1])
     S = numpy.zeros((nao, nao))
-    shells = []
-    for ib in range(mol.nbas):
-        l = mol.bas_angular(ib)
-        shells.append((l, mol.bas_exp(ib), mol._libcint_ctr_coeff(ib),
-                       mol.bas_coord(ib), cart2sph(l), cart_components(l)))
-    for i in range(mol.nbas):
-        li, ei, ci, Ai, Ti, comps_i = shells[i]
+    nb = mol.nbas
+    ls = [int(mol.bas_angular(ib)) for ib in range(nb)]
+    exps = [numpy.asarray(mol.bas_exp(ib), dtype=float) for ib in range(nb)]
+    coefs = [numpy.asarray(mol._libcint_ctr_coeff(ib), dtype=float).reshape(len(exps[ib]), -1)
+             for ib in range(nb)]
+    xyz = numpy.array([mol.bas_coord(ib) for ib in range(nb)], dtype=float)
+    kind = [(ls[ib], len(exps[ib]), coefs[ib].shape[1]) for ib in range(nb)]
+    groups = {}
+    for i in range(nb):
         for j in range(i + 1):
-            lj, ej, cj, Bj, Tj, comps_j = shells[j]
-            a = ei[:, None]
-            b = ej[None, :]
-            sx = _overlap_1d(li, lj, a, b, Ai[0], Bj[0])
-            sy = _overlap_1d(li, lj, a, b, Ai[1], Bj[1])
-            sz = _overlap_1d(li, lj, a, b, Ai[2], Bj[2])
-            cart = numpy.zeros((len(comps_i), len(comps_j), ci.shape[1], cj.shape[1]))
-            for u, (x1, y1, z1) in enumerate(comps_i):
-                for v, (x2, y2, z2) in enumerate(comps_j):
-                    prim = sx[x1, x2] * sy[y1, y2] * sz[z1, z2]
-                    cart[u, v] = ci.T @ prim @ cj
-            blk = numpy.einsum('mu,nv,uvcd->cmdn', Ti, Tj, cart)
-            blk = blk.reshape(ci.shape[1] * (2 * li + 1), cj.shape[1] * (2 * lj + 1))
-            S[ao_loc[i]:ao_loc[i + 1], ao_loc[j]:ao_loc[j + 1]] = blk
-            S[ao_loc[j]:ao_loc[j + 1], ao_loc[i]:ao_loc[i + 1]] = blk.T
+            groups.setdefault((kind[i], kind[j]), []).append((i, j))
+    for ((li, npi, nci), (lj, npj, ncj)), pairs in groups.items():
+        I = numpy.array([p[0] for p in pairs])
+        J = numpy.array([p[1] for p in pairs])
+        a = numpy.stack([exps[i] for i in I])[:, :, None]          # (k, npi, 1)
+        b = numpy.stack([exps[j] for j in J])[:, None, :]          # (k, 1, npj)
+        ci = numpy.stack([coefs[i] for i in I])                    # (k, npi, nci)
+        cj = numpy.stack([coefs[j] for j in J])                    # (k, npj, ncj)
+        s1 = [_overlap_1d(li, lj, a, b, xyz[I, d][:, None, None], xyz[J, d][:, None, None])
+              for d in range(3)]
+        comps_i, comps_j = cart_components(li), cart_components(lj)
+        cart = numpy.empty((len(comps_i), len(comps_j), len(I), nci, ncj))
+        for u, (x1, y1, z1) in enumerate(comps_i):
+            for v, (x2, y2, z2) in enumerate(comps_j):
+                prim = s1[0][x1, x2] * s1[1][y1, y2] * s1[2][z1, z2]
+                cart[u, v] = numpy.einsum('kpc,kpq,kqd->kcd', ci, prim, cj)
+        blk = numpy.einsum('mu,nv,uvkcd->kcmdn', cart2sph(li), cart2sph(lj), cart)
+        di, dj = nci * (2 * li + 1), ncj * (2 * lj + 1)
+        blk = blk.reshape(len(I), di, dj)
+        rows = ao_loc[I][:, None] + numpy.arange(di)[None, :]
+        cols = ao_loc[J][:, None] + numpy.arange(dj)[None, :]
+        S[rows[:, :, None], cols[:, None, :]] = blk
+        S[cols[:, :, None], rows[:, None, :]] = blk.transpose(0, 2, 1)
     return S
 
 

@@ -10,7 +10,7 @@ import time
 import pytest
 
 from pyscf_wb97mv_fast_amd.errors import GpuOOM
-from pyscf_wb97mv_fast_amd.serving import CycleServer, Request, footprint
+from pyscf_wb97mv_fast_amd.serving import CycleServer, Request
 
 
 class FakeCtx:
@@ -46,6 +46,19 @@ def test_slots_fill_and_results_arrive():
     st = srv.stats()
     assert st['jobs'] == 6 and st['max_active'] == 3
     assert st['p99_s'] >= st['p50_s'] > 0
+
+
+def test_idle_slot_releases_memory_a_waiting_job_needs():
+    # slot 0 ends up holding 1000 of a 1500 budget and goes idle; a job of
+    # 800 taken by slot 1 starts only after slot 0 releases its memory
+    ctxs = [FakeCtx(hold=1000), FakeCtx(hold=800)]
+    srv = CycleServer(hbm_budget=1500, runner=_runner(0.01), contexts=ctxs)
+    srv.submit(_req('a', 1000)).result(timeout=10)
+    time.sleep(0.05)
+    futs = [srv.submit(_req(t, 800)) for t in 'bcd']
+    assert [f.result(timeout=10) for f in futs] == list('bcd')
+    srv.close()
+    assert srv.stats()['jobs'] == 4
 
 
 def test_budget_serializes_jobs_in_arrival_order():
@@ -97,10 +110,3 @@ def test_idle_slots_trim_to_admit_the_next_request():
     srv.close()
     assert sum(c.trims for c in ctxs) >= 1
     assert srv.stats()['jobs'] == 3
-
-
-def test_footprint_grows_with_size():
-    a = footprint(648, 218566, 218566, npairs=29000)
-    b = footprint(1536, 508343, 508343, npairs=102000)
-    assert 0 < a < b
-    assert footprint(648, 218566, precision='fp64') > footprint(648, 218566)
