@@ -48,11 +48,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(seed=0):
+WORKLOADS = {
+    'water27': '(H2O)27 def2-SVP',
+    'water8': '(H2O)8 def2-SVP',
+    'drug': 'drug-like chain, 40 heavy atoms (seed 0), def2-TZVP',
+}
+
+
+def build_workload(kind='water27', seed=0):
     from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
     t = time.perf_counter()
-    mol = mole.Mole(geometry.water_cluster(27, seed=seed), 'def2-svp')
+    if kind == 'drug':
+        mol = mole.Mole(geometry.drug_like(40, seed=seed), 'def2-tzvp')
+    else:
+        mol = mole.Mole(geometry.water_cluster(27 if kind == 'water27' else 8, seed=seed),
+                        'def2-svp')
     # the grid's cell partition runs on the GPU (wb_grid_partition)
     g = grids.Grids(mol, level=1, device=lib.Context(0)).build()
     dm = guess.huckel_density(mol)
@@ -212,6 +223,9 @@ def main():
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a device copy of MIB MiB (5 times): the known-size '
                          'kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters')
+    ap.add_argument('--workload', choices=sorted(WORKLOADS), default='water27',
+                    help='the molecule of the cycle measurement (default: the headline '
+                         '(H2O)27 def2-SVP)')
     ap.add_argument('--batch', type=int, default=0, metavar='N',
                     help='batched mode: N water clusters (n ~ U{4..--batch-max}), one '
                          'K(1/r) + K(erf) + VV10 cycle each, separate calls vs one '
@@ -248,7 +262,7 @@ def main():
     from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.basis import pack_mol
 
-    mol, g, dm = build_workload()
+    mol, g, dm = build_workload(args.workload)
     nao = mol.nao
     L = lib.load()
     ctx = lib.Context(0)
@@ -360,8 +374,8 @@ def main():
         'ms_per_step': round(1e3 * total / args.steps, 3),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32+f64', 'data': 'synthetic',
-        'config': {'workload': '(H2O)27 def2-SVP: COSX K(1/r) + K(erf, 0.3) + VV10 '
-                               'nr_nlc_vxc per cycle, dm resident in HBM',
+        'config': {'workload': WORKLOADS[args.workload] + ': COSX K(1/r) + K(erf, 0.3) + '
+                               'VV10 nr_nlc_vxc per cycle, dm resident in HBM',
                    'nao': nao, 'grid_points': int(g.size), 'grid': 'own Becke level 1',
                    'k_tile_tol': K_TOL, 'vv10_tile_tol': args.vv10_tol,
                    'guess': 'extended Hueckel'},
@@ -382,7 +396,7 @@ def main():
                      'achieved': round(achieved / 1e12, 3), 'peak': PEAK_FP32_PACKED / 1e12,
                      'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_PACKED, 4),
                      'frac_vs_nonpacked_78.6': round(achieved / PEAK_FP32_VALU, 4),
-                     'traffic': measured_traffic(),
+                     'traffic': measured_traffic() if args.workload == 'water27' else None,
                      'traffic_unit': 'bytes per operator (rocprofv3 FETCH_SIZE + WRITE_SIZE, '
                                      'copy-calibrated; profiles/md_k_kernel_water27_svp_traffic.txt)',
                      'tasks_kept': int(kept[1]), 'tasks_total': int(kept[0]),
