@@ -101,6 +101,44 @@ def _clusters(count, nmax, seed):
     return out
 
 
+def bench_setup(args):
+    """Per-molecule set-up on the device path: wb_basis_create (shell pairs on
+    the host threads) and wb_grid_partition (cell weights on the GPU) for
+    water clusters of 8, 27 and 64 molecules, level-1 grids; the host numpy
+    partition timed beside it on (H2O)8."""
+    from pyscf_wb97mv_fast_amd import lib
+    from pyscf_wb97mv_fast_amd.basis import pack_mol
+    from pyscf_wb97mv_fast_amd.gpu import grids as dgrids
+    from pyscf_wb97mv_fast_amd.systems import geometry, grids, mole
+    ctx = lib.Context(0)
+    rows = []
+    for n in (8, 27, 64):
+        mol = mole.Mole(geometry.water_cluster(n, seed=0), 'def2-svp')
+        pk = pack_mol(mol)
+        t = time.perf_counter()
+        b = lib.DeviceBasis(ctx, pk)
+        t_basis = time.perf_counter() - t
+        coords, _, owner = grids.Grids(mol, level=1).raw_points()
+        atoms = mol.atom_coords()
+        dgrids.ssf_partition(ctx, coords, atoms, owner)          # first call: set-up
+        t = time.perf_counter()
+        part = dgrids.ssf_partition(ctx, coords, atoms, owner)
+        t_part = time.perf_counter() - t
+        row = {'waters': n, 'nao': b.nao, 'shell_pairs': b.npairs, 'primitive_pairs': b.nprims,
+               'basis_create_s': round(t_basis, 4), 'raw_points': len(coords),
+               'partition_s': round(t_part, 4)}
+        if n == 8:
+            t = time.perf_counter()
+            ref = grids._ssf_partition(coords, atoms, owner)
+            row['host_partition_s'] = round(time.perf_counter() - t, 3)
+            row['max_abs_diff_vs_host'] = float(abs(ref - part).max())
+        log('(H2O)%d: %s' % (n, row))
+        rows.append(row)
+    print(json.dumps({'metric': 'set-up seconds per molecule (shell pairs, grid-cell partition)',
+                      'unit': 's', 'n_gpus': 1, 'higher_is_better': False, 'data': 'synthetic',
+                      'host_threads': min(16, os.cpu_count() or 1), 'molecules': rows}), flush=True)
+
+
 def bench_batch(args):
     """One K(1/r) + K(erf) + VV10 cycle for each of N molecules of uneven
     size: separate single-molecule calls vs one wb_cycle_batch call (1 and
@@ -232,6 +270,9 @@ def main():
                          'wb_cycle_batch; prints its own JSON line')
     ap.add_argument('--batch-max', type=int, default=64)
     ap.add_argument('--lanes', type=int, default=4)
+    ap.add_argument('--setup', action='store_true',
+                    help='set-up mode: shell-pair construction and the grid-cell partition '
+                         'for (H2O)8/27/64; prints its own JSON line')
     ap.add_argument('--serve', type=int, default=0, metavar='N',
                     help='serving mode: N jobs with seeded Poisson arrivals (--rate per '
                          's) through the continuous-batching loop; prints its own JSON line')
@@ -255,6 +296,8 @@ def main():
         print(json.dumps({'copy_probe_bytes_read': 4 * n, 'copy_probe_bytes_written': 4 * n,
                           'launches': 5}), flush=True)
         return
+    if args.setup:
+        return bench_setup(args)
     if args.batch:
         return bench_batch(args)
     if args.serve:
