@@ -257,7 +257,8 @@ def main():
     ap.add_argument('--vv10-tol', type=float, default=0.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--mixed-only', action='store_true',
-                    help='time the mixed form only (for profiler runs)')
+                    help='time the mixed form only, every cycle with the K build and VV10 '
+                         'flows in sequence and phase timing (for profiler runs)')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run a device copy of MIB MiB (5 times): the known-size '
                          'kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters')
@@ -320,18 +321,25 @@ def main():
     om = numpy.array([0.0, OMEGA])
 
     def cycle(prec, vv10_tol):
-        lib.check(L.wb_k_build_dev(ctx.h, basis.h, grid.h, d_dm.data_ptr(), 2, lib.ptr(om),
-                                   K_TOL, prec, d_K.data_ptr()))
-        lib.check(L.wb_nlc_vxc_dev(ctx.h, basis.h, grid.h, d_dm.data_ptr(), NLC[0], NLC[1],
-                                   vv10_tol, prec, d_e.data_ptr(), d_V.data_ptr()))
+        # K build and VV10 term of one cycle; the VV10 flow runs concurrently
+        # on a second stream unless phase timing is on (wb_cycle_dev)
+        lib.check(L.wb_cycle_dev(ctx.h, basis.h, grid.h, d_dm.data_ptr(), 2, lib.ptr(om), K_TOL,
+                                 grid.h, d_dm.data_ptr(), NLC[0], NLC[1], vv10_tol, prec,
+                                 d_K.data_ptr(), d_e.data_ptr(), d_V.data_ptr()))
         ctx.sync()
 
-    def run(prec, steps, warmup, profile=False, vv10_tol=args.vv10_tol):
+    def run(prec, steps, warmup, profile=False, vv10_tol=args.vv10_tol, profile_steps=2,
+            profile_all=False):
+        """Time `steps` cycles (concurrent flows, no phase timing), then, with
+        profile, `profile_steps` more with the flows in sequence and HIP
+        events between the phases (per-kernel-phase times).  profile_all:
+        every cycle in sequence with phase timing (profiler runs)."""
         for _ in range(warmup):
             cycle(prec, vv10_tol)
-        lib.check(L.wb_ctx_set_profile(ctx.h, 1 if profile else 0))
         torch.cuda.synchronize()
         ctx.sync()
+        if profile_all:
+            lib.check(L.wb_ctx_set_profile(ctx.h, 1))
         times = []
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -341,17 +349,28 @@ def main():
         ctx.sync()
         torch.cuda.synchronize()
         total = time.perf_counter() - t0
-        ph = numpy.zeros(len(PHASES))
-        lib.check(L.wb_profile_read(ctx.h, lib.ptr(ph), len(PHASES), 1))
-        lib.check(L.wb_ctx_set_profile(ctx.h, 0))
         out = (d_K.cpu().numpy().copy(), d_e.cpu().numpy().copy(), d_V.cpu().numpy().copy())
-        return total, numpy.array(times), ph / max(steps, 1), out
+        ph = numpy.zeros(len(PHASES))
+        if profile_all:
+            lib.check(L.wb_profile_read(ctx.h, lib.ptr(ph), len(PHASES), 1))
+            lib.check(L.wb_ctx_set_profile(ctx.h, 0))
+            ph /= max(steps, 1)
+        elif profile:
+            lib.check(L.wb_ctx_set_profile(ctx.h, 1))
+            for _ in range(profile_steps):
+                cycle(prec, vv10_tol)
+            lib.check(L.wb_profile_read(ctx.h, lib.ptr(ph), len(PHASES), 1))
+            lib.check(L.wb_ctx_set_profile(ctx.h, 0))
+            ph /= profile_steps
+        return total, numpy.array(times), ph, out
 
     # mixed precision: the measured configuration
-    total, times, ph, res32 = run(lib.PREC['mixed'], args.steps, args.warmup, profile=True)
+    total, times, ph, res32 = run(lib.PREC['mixed'], args.steps, args.warmup, profile=True,
+                                  profile_all=args.mixed_only)
     value = args.steps / total
-    log('mixed: %.2f ms/cycle, phases (ms) %s' % (1e3 * total / args.steps,
-                                                 dict(zip(PHASES, numpy.round(ph, 3)))))
+    log('mixed: %.2f ms/cycle (K build and VV10 on two streams); phases with the flows in '
+        'sequence (ms) %s, sum %.2f' % (1e3 * total / args.steps,
+                                        dict(zip(PHASES, numpy.round(ph, 3))), ph.sum()))
     work = numpy.zeros(20, dtype=numpy.int64)
     lib.check(L.wb_k_work(ctx.h, lib.ptr(work)))
     kept = numpy.zeros(2, dtype=numpy.int64)
@@ -370,6 +389,8 @@ def main():
     if args.mixed_only:
         print(json.dumps({'mixed_ms_per_step': round(1e3 * total / args.steps, 3),
                           'phases_ms': dict(zip(PHASES, [round(float(v), 4) for v in ph])),
+        'phases_note': 'per-phase times from a profiled pass with the K build and VV10 flows '
+                       'in sequence (HIP events); the timed cycles run them concurrently',
                           'md_algorithmic_flops_per_operator': flops}), flush=True)
         return
     # the same mixed cycle replayed as one HIP graph (wb_cycle_graph_*)
@@ -397,7 +418,7 @@ def main():
     # all-FP64 form of the same kernels
     # (exact VV10: the reference for the mixed form's error, pruning included)
     total64, times64, ph64, res64 = run(lib.PREC['fp64'], args.fp64_steps, 1, profile=True,
-                                        vv10_tol=0.0)
+                                        vv10_tol=0.0, profile_steps=1)
     value64 = args.fp64_steps / total64
     log('fp64: %.2f ms/cycle, phases (ms) %s' % (1e3 * total64 / args.fp64_steps,
                                                 dict(zip(PHASES, numpy.round(ph64, 3)))))
@@ -434,6 +455,8 @@ def main():
                              'dE_nl': dEnl, 'max_abs_dK': float(numpy.abs(dK).max()),
                              'max_abs_dVnl': float(numpy.abs(res32[2] - res64[2]).max())},
         'phases_ms': dict(zip(PHASES, [round(float(v), 4) for v in ph])),
+        'phases_note': 'per-phase times from a profiled pass with the K build and VV10 flows '
+                       'in sequence (HIP events); the timed cycles run them concurrently',
         'roofline': {'bound': 'valu',
                      'kernel': 'md_k_kernel (all classes; per operator = both operators / 2)',
                      'achieved': round(achieved / 1e12, 3), 'peak': PEAK_FP32_PACKED / 1e12,

@@ -175,6 +175,21 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
 int wb_cycle_graph_launch(wb_graph_t graph);
 int wb_cycle_graph_destroy(wb_graph_t graph);
 
+/* One SCF cycle's device work, stream-ordered on the context's stream
+ * (device pointers; wb_sync waits): the K build on kgrid (d_qd -> d_k, nop
+ * operators) and/or the VV10 term on nlcgrid (d_dm -> d_out2 = (nelec, exc),
+ * d_vmat).  With both, the VV10 flow runs concurrently on a second stream of
+ * the context (its own workspaces), forked and joined with events, so its
+ * FP32 pair sum overlaps the K build's FP64 GEMMs; the outputs equal those
+ * of wb_k_build_dev followed by wb_nlc_vxc_dev bit for bit.  Replaces the
+ * get_k_only calls (gpu/sgx_k.py:983-991) and the nr_nlc_vxc call
+ * (gpu/vv10.py:268-342) of one cycle for callers that issue them together
+ * (device-resident SCF loops, serving). */
+int wb_cycle_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const double* d_qd, int nop,
+                 const double* omegas, double k_tol, wb_grid_t nlcgrid, const double* d_dm,
+                 double nlc_b, double nlc_C, double vv_tol, int precision, double* d_k,
+                 double* d_out2, double* d_vmat);
+
 /* One molecule's cycle in a batch: a K build (kgrid != NULL) and/or the VV10
  * term (nlcgrid != NULL).  Host buffers, C-contiguous, nao = the basis's. */
 typedef struct wb_cycle_job {

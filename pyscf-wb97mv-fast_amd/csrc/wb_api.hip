@@ -784,12 +784,42 @@ size_t Ctx::ws_bytes() const
     size_t n = 0;
     for (const DevBuf* d : bufs) n += d->bytes;
     for (const auto& l : lanes) n += l->ws_bytes();
+    if (side) n += side->ws_bytes();
     return n;
 }
 void Ctx::trim()
 {
     for (DevBuf* d : bufs) d->release();
     for (auto& l : lanes) l->trim();
+    if (side) side->trim();
+}
+Ctx::~Ctx()
+{
+    lanes.clear();
+    side.reset();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (t0) (void)hipEventDestroy(t0);
+    if (own_stream && stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+// a context on c's device with its own stream and workspaces (batch lanes,
+// the VV10 branch of a concurrent cycle)
+static std::unique_ptr<Ctx> make_sub_ctx(const Ctx& c)
+{
+    auto l = std::make_unique<Ctx>();
+    l->device = c.device;
+    WB_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+    l->own_stream = true;
+    std::vector<float> t32;
+    std::vector<double> t64;
+    boys_tables(t32, t64);
+    upload(l->d_b32, t32.data(), sizeof(float) * t32.size(), l->stream);
+    upload(l->d_b64, t64.data(), sizeof(double) * t64.size(), l->stream);
+    return l;
 }
 }  // namespace wb
 
@@ -809,6 +839,7 @@ int wb_ctx_trim(wb_ctx_t ctx)
         WB_HIP(hipSetDevice(c->device));
         WB_HIP(hipStreamSynchronize(c->stream));
         for (auto& l : c->lanes) WB_HIP(hipStreamSynchronize(l->stream));
+        if (c->side) WB_HIP(hipStreamSynchronize(c->side->stream));
         c->trim();
     })
 }
@@ -820,13 +851,7 @@ int wb_ctx_destroy(wb_ctx_t ctx)
         if (!c) return WB_OK;
         WB_HIP(hipSetDevice(c->device));
         WB_HIP(hipStreamSynchronize(c->stream));
-        for (auto& l : c->lanes) {
-            WB_HIP(hipStreamSynchronize(l->stream));
-            (void)hipStreamDestroy(l->stream);
-        }
-        c->lanes.clear();
-        if (c->own_stream) (void)hipStreamDestroy(c->stream);
-        delete c;
+        delete c;                       // the destructor waits for the sub-contexts' streams
     })
 }
 
@@ -1230,18 +1255,7 @@ static size_t held_after(const Ctx& w, const size_t need[WS_COUNT])
 
 static Ctx& batch_lane(Ctx& c, size_t i)
 {
-    while (c.lanes.size() <= i) {
-        auto l = std::make_unique<Ctx>();
-        l->device = c.device;
-        WB_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
-        l->own_stream = true;
-        std::vector<float> t32;
-        std::vector<double> t64;
-        boys_tables(t32, t64);
-        upload(l->d_b32, t32.data(), sizeof(float) * t32.size(), l->stream);
-        upload(l->d_b64, t64.data(), sizeof(double) * t64.size(), l->stream);
-        c.lanes.push_back(std::move(l));
-    }
+    while (c.lanes.size() <= i) c.lanes.push_back(make_sub_ctx(c));
     Ctx& l = *c.lanes[i];
     l.ncopy = c.ncopy;
     l.fuse_mask = c.fuse_mask;
@@ -1473,6 +1487,63 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
     })
 }
 
+// ------------------------------------------------------------ cycle ---
+// One SCF cycle's device work: the K build on kg and the VV10 term on vg.
+// With both (and phase timing off) the VV10 flow runs on the side context's
+// stream, forked from and joined back to c's stream by events, so that its
+// FP32 pair sum overlaps the K build's FP64 MFMA GEMMs and the K build's
+// exchange kernels overlap the VV10 GEMMs.  The two flows share no buffer
+// (each context has its own workspaces) and their arithmetic is unchanged:
+// the outputs equal the sequential cycle's bit for bit.
+static Ctx& side_ctx(Ctx& c)
+{
+    if (!c.side) c.side = make_sub_ctx(c);
+    if (!c.ev_fork) WB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    if (!c.ev_join) WB_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    c.side->ncopy = c.ncopy;
+    c.side->fuse_mask = c.fuse_mask;
+    c.side->g64 = c.g64;
+    return *c.side;
+}
+
+static void cycle_dev(Ctx& c, Basis& B, Grid* kg, const double* d_qd, int nop,
+                      const double* omegas, double k_tol, Grid* vg, const double* d_dm, double b,
+                      double Cc, double vv_tol, int prec, double* d_k, double* d_out2,
+                      double* d_vmat)
+{
+    if (kg && vg && !c.profile) {
+        Ctx& sd = side_ctx(c);
+        WB_HIP(hipEventRecord(c.ev_fork, c.stream));
+        WB_HIP(hipStreamWaitEvent(sd.stream, c.ev_fork, 0));
+        k_build_dev(c, B, *kg, d_qd, nop, omegas, k_tol, prec, d_k);
+        nlc_dev(sd, B, *vg, d_dm, b, Cc, vv_tol, prec, d_out2, d_vmat);
+        WB_HIP(hipEventRecord(c.ev_join, sd.stream));
+        WB_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+        return;
+    }
+    if (kg) k_build_dev(c, B, *kg, d_qd, nop, omegas, k_tol, prec, d_k);
+    if (vg) nlc_dev(c, B, *vg, d_dm, b, Cc, vv_tol, prec, d_out2, d_vmat);
+}
+
+int wb_cycle_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const double* d_qd, int nop,
+                 const double* omegas, double k_tol, wb_grid_t nlcgrid, const double* d_dm,
+                 double b, double C, double vv_tol, int precision, double* d_k, double* d_out2,
+                 double* d_vmat)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !basis) throw Error(WB_INVALID, "null argument");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        if (kgrid && (nop < 1 || nop > 8 || !omegas || !d_qd || !d_k))
+            throw Error(WB_INVALID, "K build: nop out of range or null buffer");
+        if (nlcgrid && (!d_dm || !d_out2 || !d_vmat)) throw Error(WB_INVALID, "VV10: null buffer");
+        WB_HIP(hipSetDevice(c->device));
+        cycle_dev(*c, *(Basis*)basis, (Grid*)kgrid, d_qd, nop, omegas, k_tol, (Grid*)nlcgrid,
+                  d_dm, b, C, vv_tol, precision, d_k, d_out2, d_vmat);
+    })
+}
+
 // ------------------------------------------------------- cycle graphs ---
 // One SCF cycle's device work recorded once as a HIP graph.  The graph owns
 // its workspaces (a private context on the owner's stream): an eager call on
@@ -1520,9 +1591,8 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
         std::vector<double> om(omegas, omegas + (kgrid ? nop : 0));
         Basis& B = *(Basis*)basis;
         auto body = [&]() {
-            if (kgrid) k_build_dev(w, B, *(Grid*)kgrid, d_qd, nop, om.data(), k_tol, precision, d_k);
-            if (nlcgrid)
-                nlc_dev(w, B, *(Grid*)nlcgrid, d_dm, b, C, vv_tol, precision, d_out2, d_vmat);
+            cycle_dev(w, B, (Grid*)kgrid, d_qd, nop, om.data(), k_tol, (Grid*)nlcgrid, d_dm, b, C,
+                      vv_tol, precision, d_k, d_out2, d_vmat);
         };
         body();                 // sizes every workspace and fills the constant caches
         WB_HIP(hipStreamSynchronize(w.stream));

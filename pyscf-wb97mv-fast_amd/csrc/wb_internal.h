@@ -150,6 +150,14 @@ struct Ctx {
     void start();
     // sub-contexts of wb_cycle_batch (own streams and workspaces), made on first use
     std::vector<std::unique_ptr<Ctx>> lanes;
+    // sub-context of the VV10 branch of a concurrent cycle, and the events
+    // that fork it from and join it to this context's stream
+    std::unique_ptr<Ctx> side;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    Ctx() = default;
+    Ctx(const Ctx&) = delete;
+    Ctx& operator=(const Ctx&) = delete;
+    ~Ctx();            // waits for and destroys what the context owns
     // device bytes held by the workspaces (lanes included); release them all
     // (the caller makes sure no work is in flight)
     size_t ws_bytes() const;

@@ -76,6 +76,8 @@ _SIGS = {
     'wb_xc_vmat': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_cycle_graph_create': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i,
                                    _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    'wb_cycle_dev': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i, _vp, _vp,
+                          _vp]),
     'wb_cycle_graph_launch': (_i, [_vp]),
     'wb_cycle_graph_destroy': (_i, [_vp]),
 }

@@ -1,7 +1,10 @@
-"""Per-cycle HIP graph (wb_cycle_graph_*): every replay equals the eager
-calls (wb_k_build_dev + wb_nlc_vxc_dev) on the same density bit for bit, with
-eager work and synchronizations between the replays, in both precisions,
-and the graph's workspaces survive the owner context's buffers regrowing."""
+"""Per-cycle HIP graph (wb_cycle_graph_*, K build and VV10 on two forked
+branches): every replay equals the eager calls (wb_k_build_dev +
+wb_nlc_vxc_dev, in sequence) on the same density bit for bit, with eager work
+and synchronizations between the replays, in both precisions, and the
+graph's workspaces survive the owner context's buffers regrowing.  The eager
+concurrent cycle (wb_cycle_dev, VV10 on a second stream) equals the
+sequential calls bit for bit too."""
 import numpy
 import pytest
 
@@ -111,3 +114,42 @@ def test_cycle_graph_k_only_and_argument_checks():
     h = ctypes.c_void_p()
     assert L.wb_cycle_graph_create(ctx.h, basis.h, None, None, 0, None, 0.0, None, None, 6.0,
                                    0.01, 0.0, 0, None, None, None, ctypes.byref(h)) == lib.WB_INVALID
+
+
+@pytest.mark.parametrize('precision', ['mixed', 'fp64'])
+def test_concurrent_cycle_equals_sequential(precision):
+    import torch
+    from pyscf_wb97mv_fast_amd import lib
+    from pyscf_wb97mv_fast_amd.basis import pack_mol
+    mol, coords, weights, dm0 = _system()
+    nao = mol.nao
+    ctx = lib.Context(0)
+    L = lib.load()
+    basis = lib.DeviceBasis(ctx, pack_mol(mol))
+    grid = lib.DeviceGrid(ctx, coords, weights)
+    dev = torch.device('cuda:0')
+    f64 = dict(dtype=torch.float64, device=dev)
+    c_k, c_e, c_v = torch.zeros((2, nao, nao), **f64), torch.zeros(2, **f64), torch.zeros((nao, nao), **f64)
+    s_k, s_e, s_v = torch.zeros_like(c_k), torch.zeros_like(c_e), torch.zeros_like(c_v)
+    prec = lib.PREC[precision]
+    for rep, dm in enumerate(_densities(dm0) * 2):
+        d = torch.from_numpy(numpy.ascontiguousarray(dm)).to(dev)
+        torch.cuda.synchronize()
+        lib.check(L.wb_k_build_dev(ctx.h, basis.h, grid.h, d.data_ptr(), 2, lib.ptr(OMEGAS),
+                                   1e-11, prec, s_k.data_ptr()))
+        lib.check(L.wb_nlc_vxc_dev(ctx.h, basis.h, grid.h, d.data_ptr(), NLC[0], NLC[1], 0.0,
+                                   prec, s_e.data_ptr(), s_v.data_ptr()))
+        # phase timing on for the second half: the flows then run in sequence
+        lib.check(L.wb_ctx_set_profile(ctx.h, 1 if rep >= 3 else 0))
+        lib.check(L.wb_cycle_dev(ctx.h, basis.h, grid.h, d.data_ptr(), 2, lib.ptr(OMEGAS), 1e-11,
+                                 grid.h, d.data_ptr(), NLC[0], NLC[1], 0.0, prec,
+                                 c_k.data_ptr(), c_e.data_ptr(), c_v.data_ptr()))
+        ctx.sync()
+        assert torch.equal(c_k, s_k), (precision, rep)
+        assert torch.equal(c_e, s_e) and torch.equal(c_v, s_v), (precision, rep)
+    lib.check(L.wb_ctx_set_profile(ctx.h, 0))
+    # K only / VV10 only through the same entry
+    lib.check(L.wb_cycle_dev(ctx.h, basis.h, None, None, 0, None, 1e-11, grid.h, d.data_ptr(),
+                             NLC[0], NLC[1], 0.0, prec, None, c_e.data_ptr(), c_v.data_ptr()))
+    ctx.sync()
+    assert torch.equal(c_e, s_e)
