@@ -251,6 +251,13 @@ int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, i
 int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
                       const double* coords, const int* owner, double* out);
 
+/* Overlap matrix on the grid, S_g = sum_g w_g chi_mu(r_g) chi_nu(r_g) (nao x
+ * nao, host out), FP64 AO values and FP64 MFMA: the quadrature overlap that
+ * COSX's overlap-fitted projector is built from (PySCF's fit_ovlp; the
+ * reference reads the finished projector, sgx._pjs_data.
+ * _overlap_correction_matrix, gpu/sgx_k.py:698-704). */
+int wb_grid_overlap(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, double* out);
+
 /* Boys function F_0..F_nmax (nmax <= 6) at n values T (host arrays), out
  * (nmax + 1, n), by the device routines the exchange kernels use: FP32
  * (WB_PREC_MIXED, T rounded to FP32) or FP64.  Replaces boys_eval(cp, T,

@@ -1787,6 +1787,27 @@ int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, i
     })
 }
 
+int wb_grid_overlap(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, double* out)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* b = (Basis*)basis;
+        auto* g = (Grid*)grid;
+        if (!c || !b || !g || !out) throw Error(WB_INVALID, "null argument");
+        WB_HIP(hipSetDevice(c->device));
+        const int nao = b->nao, Ppad = g->Ppad;
+        double* X = (double*)c->ws_X.ensure(sizeof(double) * nao * (size_t)Ppad);
+        launch_ao_eval(c->stream, *b, *g, 0, X);
+        // padding points carry weight 0
+        double* S = (double*)c->ws_K.ensure(sizeof(double) * nao * (size_t)nao);
+        gemm_nt_fast(c->stream, nao, nao, Ppad, X, X, false, 1, 0, g->d_w.as<double>(), S,
+                     c->ws_Kp);
+        WB_HIP(hipMemcpyAsync(out, S, sizeof(double) * nao * (size_t)nao, hipMemcpyDeviceToHost,
+                              c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
 int wb_ctx_set_profile(wb_ctx_t ctx, int on)
 {
     WB_TRY({

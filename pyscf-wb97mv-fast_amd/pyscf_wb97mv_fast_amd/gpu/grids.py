@@ -1,4 +1,5 @@
-"""Grid construction on the MI355X: the atomic cell partition.
+"""Grid construction on the MI355X: the atomic cell partition, and the
+quadrature overlap of the AOs on a grid.
 
 Building an atom-centred grid is cheap except for the cell partition, which
 weighs every point against every nearby atom pair (O(points x atoms^2) in the
@@ -27,4 +28,13 @@ def ssf_partition(ctx, coords, atoms, owner):
     out = numpy.empty(len(coords))
     lib.check(L.wb_grid_partition(ctx.h, len(atoms), lib.ptr(atoms), len(coords),
                                   lib.ptr(coords), lib.ptr(owner), lib.ptr(out)))
+    return out
+
+
+def grid_overlap(ctx, basis, grid):
+    """S_g = X^T diag(w) X on the device (FP64): the quadrature overlap that
+    the overlap-fitted COSX projector is made from (PySCF's fit_ovlp; the
+    reference takes the finished projector from PySCF, gpu/sgx_k.py:698-704)."""
+    out = numpy.empty((basis.nao, basis.nao))
+    lib.check(lib.load().wb_grid_overlap(ctx.h, basis.h, grid.h, lib.ptr(out)))
     return out

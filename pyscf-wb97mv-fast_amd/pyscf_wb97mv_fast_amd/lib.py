@@ -62,6 +62,7 @@ _SIGS = {
     'wb_ao_eval': (_i, [_vp, _vp, _i, _vp, _i, _vp]),
     'wb_boys_eval': (_i, [_vp, _i, _vp, _i, _i, _vp]),
     'wb_grid_partition': (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
+    'wb_grid_overlap': (_i, [_vp, _vp, _vp, _vp]),
     'wb_cycle_batch': (_i, [_vp, _i, ctypes.POINTER(CycleJob), _i, _vp, _d, _d, _d, _d, _i, _i,
                             _vp]),
     'wb_ctx_mem': (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),

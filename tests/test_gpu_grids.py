@@ -100,3 +100,21 @@ def test_threaded_shell_pairs_are_thread_count_independent(ctx):
     for r in res[1:]:
         assert r[:2] == res[0][:2]
         assert numpy.array_equal(r[2], res[0][2])
+
+
+def test_grid_overlap_vs_quadrature_and_analytic(ctx):
+    import oracle
+    mol = mole.Mole(geometry.WATER_DIMER, 'def2-svp')
+    b = lib.DeviceBasis(ctx, pack_mol(mol))
+    # against the oracle's AO values on the same points (dense quadrature)
+    g = grids.Grids(mol, level=1).build()
+    gr = lib.DeviceGrid(ctx, g.coords, g.weights)
+    S = dgrids.grid_overlap(ctx, b, gr)
+    X = oracle.ao_eval(oracle.OracleBasis.from_mol(mol), g.coords)[0]
+    ref = (X * g.weights[:, None]).T @ X
+    assert numpy.abs(S - ref).max() <= 1e-12 * numpy.abs(ref).max()
+    assert numpy.array_equal(S, S.T) or numpy.abs(S - S.T).max() <= 1e-14
+    # a fine grid reproduces the analytic overlap
+    gf = grids.Grids(mol, level=5, device=ctx).build()
+    Sf = dgrids.grid_overlap(ctx, b, lib.DeviceGrid(ctx, gf.coords, gf.weights))
+    assert numpy.abs(Sf - guess.overlap_matrix(mol)).max() < 1e-6
