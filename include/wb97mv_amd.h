@@ -201,7 +201,22 @@ typedef struct wb_cycle_job {
     const double* dm;       /* in: dm (nao, nao) */
     double* nlc;            /* out: (nelec, exc) */
     double* vmat;           /* out: (nao, nao) */
+    int flags;              /* WB_JOB_SEQUENTIAL: the two flows one after the other */
 } wb_cycle_job;
+/* wb_cycle_job.flags: run the K build and the VV10 flow of the job in
+ * sequence on the job's context (default 0: concurrently, the VV10 flow on
+ * the context's second stream).  Results are the same either way; with
+ * other work sharing the device the sequence avoids extra contention. */
+#define WB_JOB_SEQUENTIAL 1
+
+/* One molecule's cycle (a wb_cycle_job) on the context with host buffers:
+ * the inputs go up, the K build and the VV10 term run as in wb_cycle_dev
+ * (concurrently when both are asked for), the outputs come down; the call
+ * returns when they are in place.  Outputs equal wb_k_build's and
+ * wb_nlc_vxc's bit for bit.  Replaces one cycle's get_k_only calls
+ * (gpu/sgx_k.py:983-991) and nr_nlc_vxc call (gpu/vv10.py:268-342). */
+int wb_cycle(wb_ctx_t ctx, const wb_cycle_job* job, int nop, const double* omegas,
+             double k_tol, double nlc_b, double nlc_C, double vv_tol, int precision);
 
 /* Device bytes of one job's workspaces: *after = what the context will
  * hold once the job has run on it (its grow-only workspaces as they are,

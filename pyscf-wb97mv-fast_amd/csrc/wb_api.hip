@@ -1176,39 +1176,60 @@ static void xc_vmat_dev(Ctx& c, Basis& B, Grid& G, const double* v, bool mgga, d
 // ones keep it full.  Jobs are taken largest first.  Each job runs exactly
 // the single-molecule code on a context of its own, so its outputs equal
 // wb_k_build's / wb_nlc_vxc's for it alone, bit for bit.
+static void cycle_dev(Ctx& c, Basis& B, Grid* kg, const double* d_qd, int nop,
+                      const double* omegas, double k_tol, Grid* vg, const double* d_dm, double b,
+                      double Cc, double vv_tol, int prec, double* d_k, double* d_out2,
+                      double* d_vmat);
+
+// one job on context w with host buffers: inputs up, the cycle (K build and
+// VV10 concurrent when both are asked for), outputs down
 static void run_cycle_job(Ctx& w, const wb_cycle_job& j, int nop, const double* omegas,
                           double k_tol, double b, double Cc, double vv_tol, int precision)
 {
     Basis& B = *(Basis*)j.basis;
     const size_t nn = (size_t)B.nao * B.nao;
+    double *dk = nullptr, *dv = nullptr;
     if (j.kgrid) {
-        double* d = (double*)w.ws_K.ensure(sizeof(double) * nn * (1 + nop));
-        WB_HIP(hipMemcpyAsync(d, j.qd, sizeof(double) * nn, hipMemcpyHostToDevice, w.stream));
-        k_build_dev(w, B, *(Grid*)j.kgrid, d, nop, omegas, k_tol, precision, d + nn);
-        WB_HIP(hipMemcpyAsync(j.k, d + nn, sizeof(double) * nn * nop, hipMemcpyDeviceToHost,
-                              w.stream));
+        dk = (double*)w.ws_K.ensure(sizeof(double) * nn * (1 + nop));
+        WB_HIP(hipMemcpyAsync(dk, j.qd, sizeof(double) * nn, hipMemcpyHostToDevice, w.stream));
     }
     if (j.nlcgrid) {
-        double* d = (double*)w.ws_io.ensure(sizeof(double) * (2 * nn + 2));
-        WB_HIP(hipMemcpyAsync(d, j.dm, sizeof(double) * nn, hipMemcpyHostToDevice, w.stream));
-        nlc_dev(w, B, *(Grid*)j.nlcgrid, d, b, Cc, vv_tol, precision, d + 2 * nn, d + nn);
-        WB_HIP(hipMemcpyAsync(j.vmat, d + nn, sizeof(double) * nn, hipMemcpyDeviceToHost, w.stream));
-        WB_HIP(hipMemcpyAsync(j.nlc, d + 2 * nn, sizeof(double) * 2, hipMemcpyDeviceToHost,
+        dv = (double*)w.ws_io.ensure(sizeof(double) * (2 * nn + 2));
+        WB_HIP(hipMemcpyAsync(dv, j.dm, sizeof(double) * nn, hipMemcpyHostToDevice, w.stream));
+    }
+    if (j.flags & WB_JOB_SEQUENTIAL) {
+        if (j.kgrid) k_build_dev(w, B, *(Grid*)j.kgrid, dk, nop, omegas, k_tol, precision, dk + nn);
+        if (j.nlcgrid)
+            nlc_dev(w, B, *(Grid*)j.nlcgrid, dv, b, Cc, vv_tol, precision, dv + 2 * nn, dv + nn);
+    } else {
+        cycle_dev(w, B, (Grid*)j.kgrid, dk, nop, omegas, k_tol, (Grid*)j.nlcgrid, dv, b, Cc,
+                  vv_tol, precision, dk ? dk + nn : nullptr, dv ? dv + 2 * nn : nullptr,
+                  dv ? dv + nn : nullptr);
+    }
+    if (j.kgrid)
+        WB_HIP(hipMemcpyAsync(j.k, dk + nn, sizeof(double) * nn * nop, hipMemcpyDeviceToHost,
+                              w.stream));
+    if (j.nlcgrid) {
+        WB_HIP(hipMemcpyAsync(j.vmat, dv + nn, sizeof(double) * nn, hipMemcpyDeviceToHost,
+                              w.stream));
+        WB_HIP(hipMemcpyAsync(j.nlc, dv + 2 * nn, sizeof(double) * 2, hipMemcpyDeviceToHost,
                               w.stream));
     }
     WB_HIP(hipStreamSynchronize(w.stream));
 }
 
 // Bytes each workspace needs for one job (what k_build_dev, nlc_dev and
-// run_cycle_job ensure(); a buffer both flows use needs the larger size).
+// run_cycle_job ensure(); a buffer both flows use on one context needs the
+// larger size).  need: the job's context; side: its VV10 side context, which
+// runs the VV10 flow when the job has both a K build and a VV10 term.
 static void job_needs(const Ctx& w, const wb_cycle_job& j, int nop, int prec,
-                      size_t need[WS_COUNT])
+                      size_t need[WS_COUNT], size_t side[WS_COUNT])
 {
-    for (int b = 0; b < WS_COUNT; ++b) need[b] = 0;
+    for (int b = 0; b < WS_COUNT; ++b) need[b] = side[b] = 0;
     const Basis& B = *(const Basis*)j.basis;
     const size_t nao = B.nao, nn = nao * nao;
     const bool f32 = prec == WB_PREC_MIXED;
-    auto at_least = [&](int b, size_t v) { need[b] = std::max(need[b], v); };
+    auto at_least = [](size_t* n, int b, size_t v) { n[b] = std::max(n[b], v); };
     if (j.kgrid) {
         const Grid& G = *(const Grid*)j.kgrid;
         const size_t P = G.Ppad, nsub = G.nsub;
@@ -1216,40 +1237,47 @@ static void job_needs(const Ctx& w, const wb_cycle_job& j, int nop, int prec,
         for (const auto& C : B.cls) nitem += (size_t)C.ngroups * nsub;
         const size_t zrow = nao + MD_SPARE;
         const bool g64 = !f32 || w.g64;
-        at_least(WS_X, 8 * nao * P);
-        at_least(WS_D, 8 * nn);
-        at_least(WS_F, (f32 ? 4 : 8) * (nao + MD_SPARE) * P);
-        at_least(WS_FMAX, 4 * nsub * B.nshell);
-        at_least(WS_COUNTS, 4 * NCLASS * nsub);
-        at_least(WS_WORK, 8 * 2 * NCLASS);
-        at_least(WS_TASKS, 4 * std::max<size_t>(nitem, 1));
-        at_least(WS_SKIPS, 4 * std::max<size_t>(nitem, 1));
-        at_least(WS_G, (g64 ? 8 : 4) * (zrow + 1) * P * w.ncopy * nop);
+        at_least(need, WS_X, 8 * nao * P);
+        at_least(need, WS_D, 8 * nn);
+        at_least(need, WS_F, (f32 ? 4 : 8) * (nao + MD_SPARE) * P);
+        at_least(need, WS_FMAX, 4 * nsub * B.nshell);
+        at_least(need, WS_COUNTS, 4 * NCLASS * nsub);
+        at_least(need, WS_WORK, 8 * 2 * NCLASS);
+        at_least(need, WS_TASKS, 4 * std::max<size_t>(nitem, 1));
+        at_least(need, WS_SKIPS, 4 * std::max<size_t>(nitem, 1));
+        at_least(need, WS_G, (g64 ? 8 : 4) * (zrow + 1) * P * w.ncopy * nop);
         if (zrow <= 32 * MD_MASK_WORDS)
-            at_least(WS_MASK, 4 * (size_t)w.ncopy * nsub * MD_MASK_WORDS * nop);
-        at_least(WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, true));
-        at_least(WS_K, 8 * nn * (1 + nop));
+            at_least(need, WS_MASK, 4 * (size_t)w.ncopy * nsub * MD_MASK_WORDS * nop);
+        at_least(need, WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, true));
+        at_least(need, WS_K, 8 * nn * (1 + nop));
     }
     if (j.nlcgrid) {
         const Grid& G = *(const Grid*)j.nlcgrid;
         const size_t P = G.Ppad;
-        at_least(WS_XV, 8 * nao * P);
-        at_least(WS_T, 8 * nao * P);
-        at_least(WS_RHO, 8 * 4 * P);
-        at_least(WS_RED, ao_rho_bytes(B, (int)P));
-        at_least(WS_VV, 8 * 5 * P);
-        at_least(WS_B, vv_layout(G.nsub, (int)P, !f32).total);
-        at_least(WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, false));
-        at_least(WS_K, 8 * nn);
-        at_least(WS_IO, 8 * (2 * nn + 2));
+        // the VV10 flow runs on the side context unless the job is sequential
+        // or has no K build
+        size_t* v = (j.kgrid && !(j.flags & WB_JOB_SEQUENTIAL) && !w.profile) ? side : need;
+        at_least(v, WS_XV, 8 * nao * P);
+        at_least(v, WS_T, 8 * nao * P);
+        at_least(v, WS_RHO, 8 * 4 * P);
+        at_least(v, WS_RED, ao_rho_bytes(B, (int)P));
+        at_least(v, WS_VV, 8 * 5 * P);
+        at_least(v, WS_B, vv_layout(G.nsub, (int)P, !f32).total);
+        at_least(v, WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, false));
+        at_least(v, WS_K, 8 * nn);
+        at_least(need, WS_IO, 8 * (2 * nn + 2));
     }
 }
 
-// bytes the lane holds after running a job with these needs
-static size_t held_after(const Ctx& w, const size_t need[WS_COUNT])
+// bytes the context (with its VV10 side context) holds after a job with
+// these needs; batch lanes and sub-contexts of other kinds are not counted
+static size_t held_after(const Ctx& w, const size_t need[WS_COUNT], const size_t side[WS_COUNT])
 {
     size_t n = 0;
-    for (int b = 0; b < WS_COUNT; ++b) n += std::max(w.bufs[b]->bytes, need[b]);
+    for (int b = 0; b < WS_COUNT; ++b) {
+        n += std::max(w.bufs[b]->bytes, need[b]);
+        n += std::max(w.side ? w.side->bufs[b]->bytes : (size_t)0, side[b]);
+    }
     return n;
 }
 
@@ -1263,6 +1291,23 @@ static Ctx& batch_lane(Ctx& c, size_t i)
     return l;
 }
 
+int wb_cycle(wb_ctx_t ctx, const wb_cycle_job* job, int nop, const double* omegas,
+             double k_tol, double b, double C, double vv_tol, int precision)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !job || !job->basis) throw Error(WB_INVALID, "null argument");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        if (job->kgrid && (nop < 1 || nop > 8 || !omegas || !job->qd || !job->k))
+            throw Error(WB_INVALID, "K build: nop out of range or null buffer");
+        if (job->nlcgrid && (!job->dm || !job->nlc || !job->vmat))
+            throw Error(WB_INVALID, "VV10: null buffer");
+        WB_HIP(hipSetDevice(c->device));
+        run_cycle_job(*c, *job, nop, omegas, k_tol, b, C, vv_tol, precision);
+    })
+}
+
 int wb_job_bytes(wb_ctx_t ctx, const wb_cycle_job* job, int nop, int precision,
                  long long* after, long long* alone)
 {
@@ -1272,11 +1317,11 @@ int wb_job_bytes(wb_ctx_t ctx, const wb_cycle_job* job, int nop, int precision,
         if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
             throw Error(WB_INVALID, "unknown precision");
         if (job->kgrid && (nop < 1 || nop > 8)) throw Error(WB_INVALID, "nop out of range");
-        size_t need[WS_COUNT];
-        job_needs(*c, *job, nop, precision, need);
+        size_t need[WS_COUNT], side[WS_COUNT];
+        job_needs(*c, *job, nop, precision, need, side);
         size_t sum = 0;
-        for (size_t v : need) sum += v;
-        if (after) *after = (long long)held_after(*c, need);
+        for (int k = 0; k < WS_COUNT; ++k) sum += need[k] + side[k];
+        if (after) *after = (long long)held_after(*c, need, side);
         if (alone) *alone = (long long)sum;
     })
 }
@@ -1352,11 +1397,11 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
             return o;
         };
         // wait for room; returns with the lane counted as running
-        auto enter = [&](size_t li, Ctx& w, const size_t* need) {
+        auto enter = [&](size_t li, Ctx& w, const size_t* need, const size_t* side) {
             std::unique_lock<std::mutex> lk(gate_mu);
             for (;;) {
                 if (!exclusive) {
-                    const size_t after = held_after(w, need), others = others_of(li);
+                    const size_t after = held_after(w, need, side), others = others_of(li);
                     if (others + after <= budget || (others == 0 && running == 0)) {
                         committed[li] = after;
                         ++running;
@@ -1429,9 +1474,9 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
                     const int k = next.fetch_add(1);
                     if (k >= njob || failed.load()) break;
                     const int i = cur = order[k];
-                    size_t need[WS_COUNT];
-                    job_needs(w, jobs[i], nop, precision, need);
-                    enter(li, w, need);
+                    size_t need[WS_COUNT], side[WS_COUNT];
+                    job_needs(w, jobs[i], nop, precision, need, side);
+                    enter(li, w, need, side);
                     bool oom = false;
                     try {
                         if (i == fail_once) throw Error(WB_OOM, "forced out-of-memory");

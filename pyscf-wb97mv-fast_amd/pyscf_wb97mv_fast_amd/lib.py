@@ -32,7 +32,10 @@ class CycleJob(ctypes.Structure):
     """wb_cycle_job (include/wb97mv_amd.h)."""
     _fields_ = [('basis', ctypes.c_void_p), ('kgrid', ctypes.c_void_p), ('qd', ctypes.c_void_p),
                 ('k', ctypes.c_void_p), ('nlcgrid', ctypes.c_void_p), ('dm', ctypes.c_void_p),
-                ('nlc', ctypes.c_void_p), ('vmat', ctypes.c_void_p)]
+                ('nlc', ctypes.c_void_p), ('vmat', ctypes.c_void_p), ('flags', ctypes.c_int)]
+
+
+JOB_SEQUENTIAL = 1
 
 
 _SIGS = {
@@ -66,6 +69,7 @@ _SIGS = {
     'wb_cycle_batch': (_i, [_vp, _i, ctypes.POINTER(CycleJob), _i, _vp, _d, _d, _d, _d, _i, _i,
                             _vp]),
     'wb_ctx_mem': (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    'wb_cycle': (_i, [_vp, ctypes.POINTER(CycleJob), _i, _vp, _d, _d, _d, _d, _i]),
     'wb_job_bytes': (_i, [_vp, ctypes.POINTER(CycleJob), _i, _i, ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_longlong)]),
     'wb_ctx_trim': (_i, [_vp]),

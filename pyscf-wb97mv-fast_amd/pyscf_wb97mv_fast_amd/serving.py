@@ -52,6 +52,7 @@ class Request:
     t_end: float = field(default=None, repr=False)
     est_bytes: int = field(default=0, repr=False)     # device bytes the job needs from empty
     _seq: int = field(default=0, repr=False)          # arrival number
+    _shared: bool = field(default=False, repr=False)  # started while other jobs ran
 
 
 def _percentile(xs, p):
@@ -145,6 +146,9 @@ class CycleServer:
                     self._cv.wait()
                 req, fut = self._q.popleft()
             try:
+                # a job that starts while others run shares the device: its two
+                # flows run one after the other (fixed for all its cycles)
+                req._shared = self.active > 0
                 state = self._prepare(ctx, req)
                 after, alone = self._need(ctx, req, state)
             except BaseException as e:   # noqa: BLE001 -- delivered to the caller
@@ -222,6 +226,7 @@ class CycleServer:
         basis, kg, vg = state
         job = lib.CycleJob()
         job.basis = basis.h.value
+        job.flags = lib.JOB_SEQUENTIAL if req._shared else 0
         job.kgrid = kg.h.value if kg is not None else None
         job.nlcgrid = vg.h.value if vg is not None else None
         after, alone = ctypes.c_longlong(), ctypes.c_longlong()
@@ -244,17 +249,24 @@ class CycleServer:
         dm = lib.f64(req.dm)
         out = {}
         for _ in range(max(1, int(req.cycles))):
+            # K build and VV10 concurrent on the device (wb_cycle) when the job
+            # started alone, one after the other when it shares the device
+            job = lib.CycleJob()
+            job.basis = basis.h.value
+            job.flags = lib.JOB_SEQUENTIAL if req._shared else 0
             if kg is not None:
                 qd = lib.f64(dm if req.q is None else req.q @ dm)
                 k = numpy.empty((len(om), nao, nao))
-                lib.check(L.wb_k_build(ctx.h, basis.h, kg.h, lib.ptr(qd), len(om), lib.ptr(om),
-                                       self.k_tol, lib.PREC[self.precision], lib.ptr(k)))
-                out['k'] = k
+                job.kgrid, job.qd, job.k = kg.h.value, qd.ctypes.data, k.ctypes.data
             if vg is not None:
                 o2, vmat = numpy.empty(2), numpy.empty((nao, nao))
-                lib.check(L.wb_nlc_vxc(ctx.h, basis.h, vg.h, lib.ptr(dm), self.nlc[0],
-                                       self.nlc[1], self.vv_tol, lib.PREC[self.precision],
-                                       lib.ptr(o2), lib.ptr(vmat)))
+                job.nlcgrid, job.dm = vg.h.value, dm.ctypes.data
+                job.nlc, job.vmat = o2.ctypes.data, vmat.ctypes.data
+            lib.check(L.wb_cycle(ctx.h, ctypes.byref(job), len(om), lib.ptr(om), self.k_tol,
+                                 self.nlc[0], self.nlc[1], self.vv_tol, lib.PREC[self.precision]))
+            if kg is not None:
+                out['k'] = k
+            if vg is not None:
                 out['nlc'], out['vmat'] = o2, vmat
             if req.update is not None:
                 dm = lib.f64(req.update(dm, out.get('k'), out.get('nlc'), out.get('vmat')))
