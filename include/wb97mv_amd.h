@@ -70,17 +70,19 @@ int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
  * times (minimum of `reps` builds, HIP events) the G copy count over
  * {1, 2, 4, 8, 16} (those whose G fits a quarter of the free device memory),
  * then, class by class, fused vs two-pass exchange kernels for the classes
- * with a fused two-operator form, and keeps the fastest on the context.
- * report (optional, 5): best copies, fuse mask, best ms, ms of the default
- * (8 copies, all fused), configurations timed.  Results stay within the
- * mixed form's tolerance of the default's (the copies change the FP32
- * summation order).  No counterpart in the reference (fixed launch shapes).
- * get/set_tuning read or restore a configuration (e.g. a cached one). */
+ * with a fused two-operator form, then the FP64 GEMMs' MFMA tiles per wave
+ * side (3 or 4, against the padding rule's choice), and keeps the fastest
+ * on the context.  report (optional, 6): best copies, fuse mask, best ms, ms
+ * of the default (8 copies, all fused, tiles by padding), configurations
+ * timed, GEMM tiles (0: by padding).  Results stay within the mixed form's
+ * tolerance of the default's (the copies change the FP32 summation order).
+ * No counterpart in the reference (fixed launch shapes).  get/set_tuning
+ * read or restore a configuration (e.g. a cached one). */
 int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd, int nop,
                     const double* omegas, double tile_tol, int precision, int reps,
                     double* report);
-int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask);
-int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask);
+int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask, int* gemm_tiles);
+int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask, int gemm_tiles);
 /* mixed form: accumulate G in FP64 (1) or FP32 (0, default) */
 int wb_ctx_set_g64(wb_ctx_t ctx, int on);
 int wb_sync(wb_ctx_t ctx);

@@ -595,6 +595,7 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     const size_t rsz = f32 ? 4 : 8;
     const int nao = B.nao, Ppad = G.Ppad;
     hipStream_t s = c.stream;
+    GemmTilesScope tiles(c.gemm_tiles);
     c.start();
     double* X = (double*)c.ws_X.ensure(sizeof(double) * nao * (size_t)Ppad);
     launch_ao_eval(s, B, G, 0, X);
@@ -999,9 +1000,10 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
         WB_HIP(hipEventCreate(&e0));
         WB_HIP(hipEventCreate(&e1));
         int trials = 0;
-        auto time_cfg = [&](int ncopy, unsigned mask) {
+        auto time_cfg = [&](int ncopy, unsigned mask, int tiles) {
             c->ncopy = ncopy;
             c->fuse_mask = mask;
+            c->gemm_tiles = tiles;
             double* d = (double*)c->ws_K.ensure(sizeof(double) * nn * (1 + nop));
             WB_HIP(hipMemcpyAsync(d, qd, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
             k_build_dev(*c, *b, *g, d, nop, omegas, tile_tol, precision, d + nn);
@@ -1020,9 +1022,10 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
         };
         const int ncopy0 = c->ncopy;
         const unsigned mask0 = c->fuse_mask;
+        const int tiles0 = c->gemm_tiles;
         try {
-            const float t_default = time_cfg(8, ~0u);
-            int best_n = 8;
+            const float t_default = time_cfg(8, ~0u, 0);
+            int best_n = 8, best_tiles = 0;
             unsigned best_mask = ~0u;
             float best_t = t_default;
             size_t free_b = 0, total_b = 0;
@@ -1032,7 +1035,7 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
                 const size_t gbytes = (size_t)n * (b->nao + MD_SPARE + 1) * g->Ppad * nop *
                                       (g64 ? 8 : 4);
                 if (gbytes > (free_b + c->ws_G.bytes) / 4) continue;
-                const float t = time_cfg(n, best_mask);
+                const float t = time_cfg(n, best_mask, 0);
                 if (t < best_t) { best_t = t; best_n = n; }
             }
             if (nop == 2)
@@ -1040,11 +1043,17 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
                     const ClassData& C = b->cls[k];
                     if (!C.npairs || !md_fuse(C.la, C.lb)) continue;
                     const unsigned m = best_mask & ~(1u << k);
-                    const float t = time_cfg(best_n, m);
+                    const float t = time_cfg(best_n, m, 0);
                     if (t < best_t) { best_t = t; best_mask = m; }
                 }
+            // GEMM wave tiles: both shapes besides the padding rule's choice
+            for (int tl : {3, 4}) {
+                const float t = time_cfg(best_n, best_mask, tl);
+                if (t < best_t) { best_t = t; best_tiles = tl; }
+            }
             c->ncopy = best_n;
             c->fuse_mask = best_mask;
+            c->gemm_tiles = best_tiles;
             // the largest configuration tried sized G: let it regrow to the kept one
             WB_HIP(hipStreamSynchronize(c->stream));
             c->ws_G.release();
@@ -1055,10 +1064,12 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
                 report[2] = best_t;
                 report[3] = t_default;
                 report[4] = trials;
+                report[5] = best_tiles;
             }
         } catch (...) {
             c->ncopy = ncopy0;
             c->fuse_mask = mask0;
+            c->gemm_tiles = tiles0;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             throw;
@@ -1068,24 +1079,28 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
     })
 }
 
-int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask)
+int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask, int* gemm_tiles)
 {
     WB_TRY({
         auto* c = (Ctx*)ctx;
-        if (!c || !ncopy || !fuse_mask) throw Error(WB_INVALID, "null argument");
+        if (!c || !ncopy || !fuse_mask || !gemm_tiles) throw Error(WB_INVALID, "null argument");
         *ncopy = c->ncopy;
         *fuse_mask = c->fuse_mask;
+        *gemm_tiles = c->gemm_tiles;
     })
 }
 
-int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask)
+int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask, int gemm_tiles)
 {
     WB_TRY({
         auto* c = (Ctx*)ctx;
         if (!c) throw Error(WB_INVALID, "null context");
         if (ncopy < 1 || ncopy > 64) throw Error(WB_INVALID, "ncopy out of range");
+        if (gemm_tiles != 0 && gemm_tiles != 3 && gemm_tiles != 4)
+            throw Error(WB_INVALID, "gemm_tiles must be 0, 3 or 4");
         c->ncopy = ncopy;
         c->fuse_mask = fuse_mask;
+        c->gemm_tiles = gemm_tiles;
     })
 }
 
@@ -1287,6 +1302,7 @@ static Ctx& batch_lane(Ctx& c, size_t i)
     Ctx& l = *c.lanes[i];
     l.ncopy = c.ncopy;
     l.fuse_mask = c.fuse_mask;
+    l.gemm_tiles = c.gemm_tiles;
     l.g64 = c.g64;
     return l;
 }
@@ -1627,6 +1643,7 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
         w.stream = c->stream;
         w.ncopy = c->ncopy;
         w.fuse_mask = c->fuse_mask;
+        w.gemm_tiles = c->gemm_tiles;
         w.g64 = c->g64;
         std::vector<float> t32;
         std::vector<double> t64;

@@ -138,6 +138,8 @@ struct Ctx {
     // classes whose two operators run in one pass (bit = class id; only where
     // md_fuse allows it); set by wb_ctx_autotune
     unsigned fuse_mask = ~0u;
+    // MFMA tiles per wave side of the K build's GEMMs (0: by padding)
+    int gemm_tiles = 0;
     bool g64 = false;          // mixed form: G accumulated in FP64 (default FP32)
     // per-class work of the last K build: kept tasks, kept primitive pairs
     DevBuf ws_work;
@@ -175,6 +177,15 @@ void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, dou
 void launch_ao_eval_pts(hipStream_t s, const Basis& b, const double* xyz, int Ppad,
                         int deriv, double* X);
 
+// MFMA tiles per wave side of the FP64 GEMMs launched from this host thread
+// (3 or 4; 0: by padding); gemm_tiles_scope sets it for one flow
+void set_gemm_tiles(int t);
+int get_gemm_tiles();
+struct GemmTilesScope {
+    int prev;
+    explicit GemmTilesScope(int t) : prev(get_gemm_tiles()) { set_gemm_tiles(t); }
+    ~GemmTilesScope() { set_gemm_tiles(prev); }
+};
 // C[M][N] (+)= A[M][K] B[K][N], row-major; out float or double
 void gemm_nn(hipStream_t s, int M, int N, int K, const double* A, const double* B,
              void* C, bool out_f32);

@@ -704,9 +704,14 @@ void fold_masked_kernel(size_t n, int K, const BT* __restrict__ src, int nb, siz
     *(double2*)(out + i + 2) = make_double2(v2 * w1.x, v3 * w1.y);
 }
 
-// 3 (96) or 4 (128) MFMA tiles per wave side: the one with less padding
+// 3 (96) or 4 (128) MFMA tiles per wave side: the one with less padding,
+// unless the calling thread set a choice (gemm_tiles_scope, the autotuner's)
+static thread_local int g_tiles = 0;
+void set_gemm_tiles(int t) { g_tiles = (t == 3 || t == 4) ? t : 0; }
+int get_gemm_tiles() { return g_tiles; }
 static int wave_tiles(int D)
 {
+    if (g_tiles) return g_tiles;
     const int p96 = (D + 95) / 96 * 96, p128 = (D + 127) / 128 * 128;
     return p96 < p128 ? 3 : 4;
 }

@@ -2,9 +2,9 @@
 
 The library times, on the caller's own molecule and grid, the number of G
 partial copies the exchange kernels scatter into (fewer copies: less memory
-and a cheaper fold; more: fewer colliding atomic adds) and, per (la, lb)
-class with a fused two-operator kernel, fused vs two passes, and keeps the
-fastest on the context.  ``Tuner`` caches the choice per size class
+and a cheaper fold; more: fewer colliding atomic adds), per (la, lb) class
+with a fused two-operator kernel fused vs two passes, and the FP64 GEMMs'
+MFMA tiles per wave side, and keeps the fastest on the context.  ``Tuner`` caches the choice per size class
 (AOs and grid points rounded up to powers of two, operator count,
 precision), so molecules of similar size reuse one tuning run; the cache
 can be saved as JSON and loaded by a later process.  The reference runs
@@ -25,22 +25,25 @@ def autotune(ctx, basis, grid, qd, omegas=(0.0, 0.3), tile_tol=1e-11, precision=
     qd = lib.f64(qd)
     if qd.shape != (basis.nao, basis.nao):
         raise ValueError('qd must be (nao, nao)')
-    rep = numpy.zeros(5)
+    rep = numpy.zeros(6)
     lib.check(lib.load().wb_ctx_autotune(ctx.h, basis.h, grid.h, lib.ptr(qd), len(om),
                                          lib.ptr(om), float(tile_tol), lib.PREC[precision],
                                          int(reps), lib.ptr(rep)))
     return {'ncopy': int(rep[0]), 'fuse_mask': int(rep[1]), 'best_ms': float(rep[2]),
-            'default_ms': float(rep[3]), 'configs': int(rep[4])}
+            'default_ms': float(rep[3]), 'configs': int(rep[4]), 'gemm_tiles': int(rep[5])}
 
 
 def get_tuning(ctx):
-    n, m = ctypes.c_int(), ctypes.c_uint()
-    lib.check(lib.load().wb_ctx_get_tuning(ctx.h, ctypes.byref(n), ctypes.byref(m)))
-    return int(n.value), int(m.value)
+    """(copies, fuse mask, GEMM wave tiles) of the context."""
+    n, m, t = ctypes.c_int(), ctypes.c_uint(), ctypes.c_int()
+    lib.check(lib.load().wb_ctx_get_tuning(ctx.h, ctypes.byref(n), ctypes.byref(m),
+                                           ctypes.byref(t)))
+    return int(n.value), int(m.value), int(t.value)
 
 
-def set_tuning(ctx, ncopy, fuse_mask):
-    lib.check(lib.load().wb_ctx_set_tuning(ctx.h, int(ncopy), ctypes.c_uint(int(fuse_mask))))
+def set_tuning(ctx, ncopy, fuse_mask, gemm_tiles=0):
+    lib.check(lib.load().wb_ctx_set_tuning(ctx.h, int(ncopy), ctypes.c_uint(int(fuse_mask)),
+                                           int(gemm_tiles)))
 
 
 def _pow2(n):
@@ -68,7 +71,7 @@ class Tuner:
             e = autotune(ctx, basis, grid, qd, omegas, tile_tol, precision, self.reps)
             self.table[k] = e
         else:
-            set_tuning(ctx, e['ncopy'], e['fuse_mask'])
+            set_tuning(ctx, e['ncopy'], e['fuse_mask'], e.get('gemm_tiles', 0))
         return k, e
 
     def save(self, path):

@@ -74,8 +74,9 @@ _SIGS = {
                           ctypes.POINTER(ctypes.c_longlong)]),
     'wb_ctx_trim': (_i, [_vp]),
     'wb_ctx_autotune': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _i, _i, _vp]),
-    'wb_ctx_get_tuning': (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_uint)]),
-    'wb_ctx_set_tuning': (_i, [_vp, _i, ctypes.c_uint]),
+    'wb_ctx_get_tuning': (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_uint),
+                               ctypes.POINTER(_i)]),
+    'wb_ctx_set_tuning': (_i, [_vp, _i, ctypes.c_uint, _i]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'wb_xc_rho': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_xc_vmat': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),

@@ -40,17 +40,26 @@ def test_autotune_keeps_fastest_and_accuracy(case):
     print(rep, flush=True)
     assert rep['configs'] >= 5
     assert rep['best_ms'] <= rep['default_ms']
-    assert autotune.get_tuning(ctx) == (rep['ncopy'], rep['fuse_mask'])
+    assert autotune.get_tuning(ctx) == (rep['ncopy'], rep['fuse_mask'], rep['gemm_tiles'])
+    assert rep['configs'] >= 7 and rep['gemm_tiles'] in (0, 3, 4)
     k_tuned = _k(ctx, b, gr, dm)
     err_def = numpy.abs(k_def - k64).max()
     err_tuned = numpy.abs(k_tuned - k64).max()
     print('max|dK| default %.3e tuned %.3e' % (err_def, err_tuned), flush=True)
     assert err_tuned <= 2 * err_def + 1e-10
     # restoring the default configuration restores the default K bit for bit
-    autotune.set_tuning(ctx, 8, 0xffffffff)
+    autotune.set_tuning(ctx, 8, 0xffffffff, 0)
     assert numpy.array_equal(_k(ctx, b, gr, dm), k_def)
+    # either GEMM tile shape gives K within rounding of the default
+    for tiles in (3, 4):
+        autotune.set_tuning(ctx, 8, 0xffffffff, tiles)
+        kt = _k(ctx, b, gr, dm)
+        print('tiles %d: max|dK| %.3e' % (tiles, numpy.abs(kt - k_def).max()), flush=True)
+        assert numpy.abs(kt - k_def).max() <= 1e-9 * numpy.abs(k_def).max()
     with pytest.raises(GpuError):
         autotune.set_tuning(ctx, 0, 0)
+    with pytest.raises(GpuError):
+        autotune.set_tuning(ctx, 8, 0, 5)
 
 
 def test_tuner_cache_reuses_the_size_class(case, tmp_path):
@@ -63,7 +72,7 @@ def test_tuner_cache_reuses_the_size_class(case, tmp_path):
     autotune.set_tuning(ctx, 3, 0)
     k2, e2 = t.apply(ctx, b, gr, dm, OM)             # cached: no timing, tuning restored
     assert k1 == k2 and e1 is e2
-    assert autotune.get_tuning(ctx) == (e1['ncopy'], e1['fuse_mask'])
+    assert autotune.get_tuning(ctx) == (e1['ncopy'], e1['fuse_mask'], e1['gemm_tiles'])
     path = tmp_path / 'tuning.json'
     t.save(str(path))
     t2 = autotune.Tuner().load(str(path))
