@@ -2,7 +2,8 @@
 
 For every shell-pair class la >= lb (l <= 3) it emits
 
-  md_prim_<la><lb>_oc / _tc(dx, dy, dz, R0, E, V)
+  md_prim_<la><lb>_oc / _tc(dx, dy, dz, R0, E, V)  (_p0 / _p1: the two parts
+      of the A components -- without z, with z -- of a split class, SPLIT)
       one primitive pair at one grid point: the pruned Hermite Coulomb
       recursion R^0_tuv (t+u+v <= L, L = la+lb) from R0[n] = c_n F_n(T),
       then V[a][b] += sum_tuv Ex[ax][bx][t] Ey[ay][by][u] Ez[az][bz][v] R_tuv
@@ -81,7 +82,20 @@ def gen_r(L):
     return lines, out
 
 
-def gen_prim(la, lb, oc):
+# classes whose Cartesian block is formed in two parts of the A components,
+# one pass of the primitive loop each (fewer block registers): the
+# components without z (az = 0) and those with z, so that the two parts
+# share none of the z-contracted intermediates
+SPLIT = {(2, 2), (3, 2), (3, 3)}
+
+
+def a_part(la, part):
+    """indices of the A Cartesian components of a part"""
+    ca = cart_components(la)
+    return [i for i, a in enumerate(ca) if (a[2] > 0) == (part == 1)]
+
+
+def gen_prim(la, lb, oc, part=None):
     """oc: one-centre variant (P = A = B): E^{ij}_t vanishes unless i+j-t is
     even, and every remaining E is a function of p alone -- the same for
     every copy of an element -- so each FP32 product carries the E rounding
@@ -90,6 +104,8 @@ def gen_prim(la, lb, oc):
     depend on the pair's orientation) are plain."""
     L = la + lb
     ca, cb = cart_components(la), cart_components(lb)
+    if part is not None:
+        ca = [ca[i] for i in a_part(la, part)]
     nb = len(cb)
     naxis = (la + 1) * (lb + 1) * (L + 1)
     erl = 3 * naxis
@@ -98,12 +114,22 @@ def gen_prim(la, lb, oc):
         return axis * naxis + (i * (lb + 1) + j) * (L + 1) + t
 
     lines = ['template <typename real, typename vacc>',
-             '__device__ __forceinline__ void md_prim_%d%d_%s(real dx, real dy, real dz,'
-             % (la, lb, 'oc' if oc else 'tc'),
+             '__device__ __forceinline__ void md_prim_%d%d_%s%s(real dx, real dy, real dz,'
+             % (la, lb, 'oc' if oc else 'tc', '' if part is None else '_p%d' % part),
              '    const real* __restrict__ R0, const real* __restrict__ E,'
              ' vacc* __restrict__ V)',
              '{']
     rl, R = gen_r(L)
+    if part is not None:
+        # only the R values this half's contractions read
+        used = set()
+        for a in ca:
+            for b in cb:
+                for t in range(a[0] + b[0] + 1):
+                    for u in range(a[1] + b[1] + 1):
+                        for v in range(a[2] + b[2] + 1):
+                            used.add((t, u, v))
+        rl = _prune_r(rl, [R[k] for k in used])
     lines += rl
 
     def zero(i, j, t):
@@ -159,9 +185,28 @@ def gen_prim(la, lb, oc):
             ax, bx = a[0], b[0]
             expr = chain(0, ax, bx, [(t, yz[(a[1], b[1], a[2], b[2], t)])
                                      for t in range(ax + bx + 1)])
-            lines.append('    V[%d] += (vacc)(%s);' % (i * nb + j, expr))
+            lines.append('    V[%d] += (vacc)(%s);' % (i * nb + j, expr))   # i: within this half
     lines.append('}')
     return lines
+
+
+def _prune_r(lines, roots):
+    """The R recursion lines that the given R values depend on."""
+    defs = {}
+    for ln in lines:
+        nm = ln.split('const real ')[1].split(' =')[0]
+        defs[nm] = ln
+    need, stack = set(), [r for r in roots if r in defs]
+    while stack:
+        nm = stack.pop()
+        if nm in need:
+            continue
+        need.add(nm)
+        rhs = defs[nm].split('=', 1)[1]
+        for tok in rhs.replace('(', ' ').replace(')', ' ').replace(',', ' ').replace(';', ' ').split():
+            if tok in defs and tok not in need:
+                stack.append(tok)
+    return [ln for ln in lines if ln.split('const real ')[1].split(' =')[0] in need]
 
 
 def _cm(v, x):
@@ -228,13 +273,32 @@ def main(out_path):
              ' real acc) { return fma(E[k], x, acc); }',
              'template <typename real, int ERL>',
              '__device__ __forceinline__ real wb_emul(const real* __restrict__ E, int k, real x)'
-             ' { return E[k] * x; }']
+             ' { return E[k] * x; }',
+             '// the A Cartesian components of part P of a split class (see gen_md.py)',
+             'template <int LA, int LB, int P> struct MdPart;',
+             '// number of parts of a class (2: split, 1: whole block)',
+             '__host__ __device__ constexpr int md_nparts(int la, int lb) {',
+             '    return %s ? 2 : 1;' % ' || '.join('(la == %d && lb == %d)' % c for c in sorted(SPLIT)),
+             '}']
     for l in range(LMAX + 1):
         lines += gen_transforms(l)
     for la in range(LMAX + 1):
         for lb in range(la + 1):
-            lines += gen_prim(la, lb, True)
-            lines += gen_prim(la, lb, False)
+            if (la, lb) not in SPLIT:
+                lines += gen_prim(la, lb, True)
+                lines += gen_prim(la, lb, False)
+            else:
+                for part in range(2):
+                    idx = a_part(la, part)
+                    lines.append('template <> struct MdPart<%d, %d, %d> {' % (la, lb, part))
+                    lines.append('    static constexpr __host__ __device__ int n() { return %d; }'
+                                 % len(idx))
+                    lines.append('    static constexpr __host__ __device__ int at(int i) {'
+                                 ' constexpr int t[%d] = {%s}; return t[i]; }'
+                                 % (len(idx), ', '.join(map(str, idx))))
+                    lines.append('};')
+                    lines += gen_prim(la, lb, True, part)
+                    lines += gen_prim(la, lb, False, part)
     text = '\n'.join(lines) + '\n'
     os.makedirs(os.path.dirname(out_path), exist_ok=True)
     old = open(out_path).read() if os.path.exists(out_path) else None

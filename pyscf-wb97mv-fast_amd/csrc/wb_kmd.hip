@@ -24,8 +24,24 @@ namespace wb {
 
 template <int L> struct MdDispatch;
 
-// ONEC: the one-centre variant (A, B and P coincide)
-template <int LA, int LB, bool ONEC, typename real, typename vacc>
+// Classes whose Cartesian block is formed in two parts of the A components
+// (without z, with z), one pass of the primitive loop each (md_nparts,
+// generated): the large blocks otherwise leave one or two waves per SIMD.
+// A components of part P (all of them when the class is not split)
+template <int LA, int LB, int P> struct MdPartA {
+    static constexpr __host__ __device__ int n() {
+        if constexpr (md_nparts(LA, LB) == 1) return (LA + 1) * (LA + 2) / 2;
+        else return MdPart<LA, LB, P>::n();
+    }
+    static constexpr __host__ __device__ int at(int i) {
+        if constexpr (md_nparts(LA, LB) == 1) return i;
+        else return MdPart<LA, LB, P>::at(i);
+    }
+};
+
+// ONEC: the one-centre variant (A, B and P coincide); PART: the part of a
+// split class (V holds that part's rows in MdPartA order)
+template <int LA, int LB, bool ONEC, int PART, typename real, typename vacc>
 __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R0,
                                         const real* E, vacc* V)
 {
@@ -34,15 +50,26 @@ __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R
         if constexpr (ONEC) md_prim_##a##b##_oc<real, vacc>(dx, dy, dz, R0, E, V);    \
         else md_prim_##a##b##_tc<real, vacc>(dx, dy, dz, R0, E, V);                   \
     }
-    WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
-    WB_MD_PRIM(2, 2) WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1) WB_MD_PRIM(3, 2) WB_MD_PRIM(3, 3)
+#define WB_MD_PART(a, b, p)                                                           \
+    if constexpr (LA == a && LB == b && PART == p) {                                  \
+        if constexpr (ONEC) md_prim_##a##b##_oc_p##p<real, vacc>(dx, dy, dz, R0, E, V); \
+        else md_prim_##a##b##_tc_p##p<real, vacc>(dx, dy, dz, R0, E, V);              \
+    }
+    if constexpr (md_nparts(LA, LB) == 1) {
+        WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
+        WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1)
+    } else {
+        WB_MD_PART(2, 2, 0) WB_MD_PART(2, 2, 1)
+        WB_MD_PART(3, 2, 0) WB_MD_PART(3, 2, 1) WB_MD_PART(3, 3, 0) WB_MD_PART(3, 3, 1)
+    }
+#undef WB_MD_PART
 #undef WB_MD_PRIM
 }
 
 // the Cartesian blocks of one shell pair at this lane's grid point, summed
 // over the pair's primitive pairs, for NOP operators at once (they share
 // P - g, the E coefficients and every load; Vs holds NOP blocks)
-template <int LA, int LB, bool ONEC, int NOP, typename real, typename vacc>
+template <int LA, int LB, bool ONEC, int NOP, int PART, typename real, typename vacc>
 __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py, double pz,
                                          const char* __restrict__ recs,
                                          const real* __restrict__ tab, vacc* Vs)
@@ -51,7 +78,7 @@ __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py,
     constexpr bool F32 = sizeof(real) == 4;
     constexpr int CRL = md_crl(LA, LB, F32);
     constexpr int RB = md_rec_bytes(LA, LB, F32, NOP);
-    constexpr int NV = (LA + 1) * (LA + 2) / 2 * ((LB + 1) * (LB + 2) / 2);
+    constexpr int NV = MdPartA<LA, LB, PART>::n() * ((LB + 1) * (LB + 2) / 2);
     // unrolled by two: the second primitive pair's scalar loads issue with
     // the first's (measured: exchange -1.2 %)
 #pragma unroll 2
@@ -76,7 +103,7 @@ __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py,
 #pragma unroll
                 for (int n = 0; n <= L; ++n) R0[n] = Co[1 + n] * Fb[n];
             }
-            md_prim<LA, LB, ONEC, real, vacc>(dx, dy, dz, R0, C + NOP * CRL, Vs + o * NV);
+            md_prim<LA, LB, ONEC, PART, real, vacc>(dx, dy, dz, R0, C + NOP * CRL, Vs + o * NV);
         }
     }
 }
@@ -313,41 +340,51 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
             if (!(mask >> q & 1u)) continue;                 // uniform
             const int pp0 = pw[q] >> 8, npp = ((pw[q] >> 2) & 63) - (int)(skw >> (4 * q) & 15u);
             const bool onec = pw[q] & 2, diag = pw[q] & 1;
-            vacc Vs[NOP * NV];
-#pragma unroll
-            for (int i = 0; i < NOP * NV; ++i) Vs[i] = (vacc)0;
-            // one-centre pairs take the variant whose FP32 E products all
-            // carry their rounding remainders
-            if (onec) md_block<LA, LB, true, NOP, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
-            else md_block<LA, LB, false, NOP, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
-            // G[B] += V^T F[A], G[A] += V F[B]; for a diagonal pair (A = B,
-            // blocked side B) only the first
             const real* FaC = BA ? FkC[q] : FiC;
             const real* FbC = BA ? FiC : FkC[q];
+            // the block (or each part of it): formed over the primitive pairs,
+            // then G[B] += V^T F[A], G[A] += V F[B]; for a diagonal pair
+            // (A = B, blocked side B) only the first
+            auto part_block = [&](auto pc) {
+                constexpr int P = decltype(pc)::value;
+                using PA = MdPartA<LA, LB, P>;
+                constexpr int NVP = PA::n() * NCB;
+                vacc Vs[NOP * NVP];
 #pragma unroll
-            for (int o = 0; o < NOP; ++o) {
-                real V[NV];
+                for (int i = 0; i < NOP * NVP; ++i) Vs[i] = (vacc)0;
+                // one-centre pairs take the variant whose FP32 E products all
+                // carry their rounding remainders
+                if (onec)
+                    md_block<LA, LB, true, NOP, P, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
+                else
+                    md_block<LA, LB, false, NOP, P, real, vacc>(pp0, npp, px, py, pz, recs, tab, Vs);
 #pragma unroll
-                for (int i = 0; i < NV; ++i) V[i] = (real)Vs[o * NV + i];
-                gacc* gA = BA ? accK[o][q] : accI[o];
-                gacc* gB = BA ? accI[o] : accK[o][q];
+                for (int o = 0; o < NOP; ++o) {
+                    real V[NVP];
 #pragma unroll
-                for (int bb = 0; bb < NCB; ++bb) {
-                    real v = (real)0;
+                    for (int i = 0; i < NVP; ++i) V[i] = (real)Vs[o * NVP + i];
+                    gacc* gA = BA ? accK[o][q] : accI[o];
+                    gacc* gB = BA ? accI[o] : accK[o][q];
 #pragma unroll
-                    for (int a = 0; a < NCA; ++a) v = fma(V[a * NCB + bb], FaC[a], v);
-                    gB[bb] += (gacc)v;
-                }
-                if (BA || !diag) {                           // uniform
-#pragma unroll
-                    for (int a = 0; a < NCA; ++a) {
+                    for (int bb = 0; bb < NCB; ++bb) {
                         real v = (real)0;
 #pragma unroll
-                        for (int bb = 0; bb < NCB; ++bb) v = fma(V[a * NCB + bb], FbC[bb], v);
-                        gA[a] += (gacc)v;
+                        for (int i = 0; i < PA::n(); ++i) v = fma(V[i * NCB + bb], FaC[PA::at(i)], v);
+                        gB[bb] += (gacc)v;
+                    }
+                    if (BA || !diag) {                       // uniform
+#pragma unroll
+                        for (int i = 0; i < PA::n(); ++i) {
+                            real v = (real)0;
+#pragma unroll
+                            for (int bb = 0; bb < NCB; ++bb) v = fma(V[i * NCB + bb], FbC[bb], v);
+                            gA[PA::at(i)] += (gacc)v;
+                        }
                     }
                 }
-            }
+            };
+            part_block(std::integral_constant<int, 0>{});
+            if constexpr (md_nparts(LA, LB) > 1) part_block(std::integral_constant<int, 1>{});
             anyI = anyI || BA || !diag;
         }
         if (anyI) {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #define __device__
+#define __host__
 #define __forceinline__ inline
 #define __restrict__
 using std::fma;
@@ -31,8 +32,27 @@ static void prim(bool onec, double dx, double dy, double dz, const double* R0, c
         if (onec) md_prim_##a##b##_oc<double, double>(dx, dy, dz, R0, E, V);     \
         else md_prim_##a##b##_tc<double, double>(dx, dy, dz, R0, E, V);          \
     }
-    C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(2,2) C(3,0) C(3,1) C(3,2) C(3,3)
+    C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(3,0) C(3,1)
 #undef C
+    // split classes: the two parts of the A components, scattered back
+#define P(a, b)                                                                        \
+    if constexpr (LA == a && LB == b) {                                                \
+        constexpr int NCB = (b + 1) * (b + 2) / 2;                                     \
+        double V0[MdPart<a, b, 0>::n() * NCB] = {0}, V1[MdPart<a, b, 1>::n() * NCB] = {0}; \
+        if (onec) {                                                                    \
+            md_prim_##a##b##_oc_p0<double, double>(dx, dy, dz, R0, E, V0);             \
+            md_prim_##a##b##_oc_p1<double, double>(dx, dy, dz, R0, E, V1);             \
+        } else {                                                                       \
+            md_prim_##a##b##_tc_p0<double, double>(dx, dy, dz, R0, E, V0);             \
+            md_prim_##a##b##_tc_p1<double, double>(dx, dy, dz, R0, E, V1);             \
+        }                                                                              \
+        for (int i = 0; i < MdPart<a, b, 0>::n(); ++i)                                 \
+            for (int j = 0; j < NCB; ++j) V[MdPart<a, b, 0>::at(i) * NCB + j] += V0[i * NCB + j]; \
+        for (int i = 0; i < MdPart<a, b, 1>::n(); ++i)                                 \
+            for (int j = 0; j < NCB; ++j) V[MdPart<a, b, 1>::at(i) * NCB + j] += V1[i * NCB + j]; \
+    }
+    P(2,2) P(3,2) P(3,3)
+#undef P
 }
 
 // One shell pair (A: la, B: lb, la >= lb), one point; inputs per primitive
