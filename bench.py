@@ -334,12 +334,14 @@ def main():
         profile, `profile_steps` more with the flows in sequence and HIP
         events between the phases (per-kernel-phase times).  profile_all:
         every cycle in sequence with phase timing (profiler runs)."""
+        if profile_all:                 # warm-up cycles in sequence too (profiler runs)
+            lib.check(L.wb_ctx_set_profile(ctx.h, 1))
         for _ in range(warmup):
             cycle(prec, vv10_tol)
         torch.cuda.synchronize()
         ctx.sync()
         if profile_all:
-            lib.check(L.wb_ctx_set_profile(ctx.h, 1))
+            lib.check(L.wb_profile_read(ctx.h, lib.ptr(numpy.zeros(len(PHASES))), len(PHASES), 1))
         times = []
         t0 = time.perf_counter()
         for _ in range(steps):

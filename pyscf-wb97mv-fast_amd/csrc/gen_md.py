@@ -85,8 +85,11 @@ def gen_r(L):
 # classes whose Cartesian block is formed in two parts of the A components,
 # one pass of the primitive loop each (fewer block registers): the
 # components without z (az = 0) and those with z, so that the two parts
-# share none of the z-contracted intermediates
-SPLIT = {(2, 2), (3, 2), (3, 3)}
+# share none of the z-contracted intermediates.  Measured on MI355X: (d, d)
+# 249 -> 165 VGPRs and (f, d) 308 -> 222 registers run faster split; (f, f)
+# stays at one wave per SIMD either way and is slower split (17.5 -> 20.3 ms
+# on a 1642-AO def2-TZVP molecule), (f, p) keeps its occupancy.
+SPLIT = {(2, 2), (3, 2)}
 
 
 def a_part(la, part):

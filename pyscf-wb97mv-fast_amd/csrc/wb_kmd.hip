@@ -57,10 +57,9 @@ __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R
     }
     if constexpr (md_nparts(LA, LB) == 1) {
         WB_MD_PRIM(0, 0) WB_MD_PRIM(1, 0) WB_MD_PRIM(1, 1) WB_MD_PRIM(2, 0) WB_MD_PRIM(2, 1)
-        WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1)
+        WB_MD_PRIM(3, 0) WB_MD_PRIM(3, 1) WB_MD_PRIM(3, 3)
     } else {
-        WB_MD_PART(2, 2, 0) WB_MD_PART(2, 2, 1)
-        WB_MD_PART(3, 2, 0) WB_MD_PART(3, 2, 1) WB_MD_PART(3, 3, 0) WB_MD_PART(3, 3, 1)
+        WB_MD_PART(2, 2, 0) WB_MD_PART(2, 2, 1) WB_MD_PART(3, 2, 0) WB_MD_PART(3, 2, 1)
     }
 #undef WB_MD_PART
 #undef WB_MD_PRIM

@@ -32,7 +32,7 @@ static void prim(bool onec, double dx, double dy, double dz, const double* R0, c
         if (onec) md_prim_##a##b##_oc<double, double>(dx, dy, dz, R0, E, V);     \
         else md_prim_##a##b##_tc<double, double>(dx, dy, dz, R0, E, V);          \
     }
-    C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(3,0) C(3,1)
+    C(0,0) C(1,0) C(1,1) C(2,0) C(2,1) C(3,0) C(3,1) C(3,3)
 #undef C
     // split classes: the two parts of the A components, scattered back
 #define P(a, b)                                                                        \
@@ -51,7 +51,7 @@ static void prim(bool onec, double dx, double dy, double dz, const double* R0, c
         for (int i = 0; i < MdPart<a, b, 1>::n(); ++i)                                 \
             for (int j = 0; j < NCB; ++j) V[MdPart<a, b, 1>::at(i) * NCB + j] += V1[i * NCB + j]; \
     }
-    P(2,2) P(3,2) P(3,3)
+    P(2,2) P(3,2)
 #undef P
 }
 
