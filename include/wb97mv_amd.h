@@ -241,7 +241,9 @@ int wb_job_bytes(wb_ctx_t ctx, const wb_cycle_job* job, int nop, int precision,
  * (gpu/vv10.py:268-342).  A job that runs out of device memory is retried
  * once alone, after every workspace of the context and its lanes has been
  * released; any other job error stops the batch and names the job.  The
- * context must not be used from another thread during the call. */
+ * context must not be used from another thread during the call.  (Test
+ * hook: the environment variable WB_BATCH_FAIL_ONCE=i makes job i's first
+ * attempt report out-of-memory before it starts, exercising the retry.) */
 int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
                    const double* omegas, double k_tol, double nlc_b, double nlc_C,
                    double vv_tol, int precision, int lanes, double* job_ms);
