@@ -624,12 +624,25 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
     unsigned* skips = (unsigned*)c.ws_skips.ensure(sizeof(unsigned) * std::max<size_t>(nitem, 1));
     size_t toff = 0;
     std::vector<size_t> cls_off(NCLASS);
+    TaskLaunch tl;
     for (int k = 0; k < NCLASS; ++k) {
         cls_off[k] = toff;
-        launch_tasks(s, B.cls[k], G, fmax, B.nshell, (float)tol, counts + (size_t)k * G.nsub,
-                     ids + toff, skips + toff, work + 2 * k);
-        toff += (size_t)B.cls[k].ngroups * G.nsub;
+        const ClassData& C = B.cls[k];
+        if (C.npairs) {
+            TaskClass& t = tl.c[tl.n++];
+            t.scr = C.d_scr.as<PairScreen>();
+            t.groups = C.d_groups.as<PairGroup>();
+            t.pbc = C.d_pbc.as<float4>();
+            t.counts = counts + (size_t)k * G.nsub;
+            t.ids = ids + toff;
+            t.skips = skips + toff;
+            t.work = work + 2 * k;
+            t.ngroups = C.ngroups;
+            t.bb = md_bb(C.la, C.lb);
+        }
+        toff += (size_t)C.ngroups * G.nsub;
     }
+    launch_tasks(s, tl, G, fmax, B.nshell, (float)tol);
     c.tasks_total = (long long)ntask;
     const int ncopy = c.ncopy;
     // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),

@@ -209,11 +209,26 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
                  int Ppad, int nsub, const int* sh_ao0, const int* sh_nao, int nshell,
                  float* fmax);
+// one class's screening inputs and task-list outputs
+struct TaskClass {
+    const PairScreen* scr;
+    const PairGroup* groups;
+    const float4* pbc;
+    int* counts;
+    int* ids;
+    unsigned* skips;
+    unsigned long long* work;
+    int ngroups, bb;
+};
+struct TaskLaunch {
+    TaskClass c[NCLASS];
+    int n = 0;                 // classes with pairs
+};
 // kept items (group << 8 | kept-position mask) per sub-block, and per item
-// the trailing primitive pairs each kept task skips (4 bits per position)
-void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
-                  int nshell, float tol, int* counts, int* ids, unsigned* skips,
-                  unsigned long long* work);
+// the trailing primitive pairs each kept task skips (4 bits per position),
+// for every class of tl in one launch
+void launch_tasks(hipStream_t s, const TaskLaunch& tl, const Grid& g, const float* fmax,
+                  int nshell, float tol);
 // f32: FP32 integrals (else FP64); g64: accumulate G in FP64 (always with FP64 integrals)
 // F_0..F_L at n values T (device arrays), out [L+1][n]
 void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,

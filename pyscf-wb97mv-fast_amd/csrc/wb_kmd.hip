@@ -543,13 +543,20 @@ void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int na
 // work[0] += kept tasks, work[1] += their evaluated primitive pairs (integer
 // atomics: the totals do not depend on the order)
 __global__ __launch_bounds__(256)
-void task_kernel(const PairScreen* __restrict__ scr,
-                 const PairGroup* __restrict__ groups, int ngroups, int bb,
-                 const float4* __restrict__ sub, int nsub,
-                 const float* __restrict__ fmax, int nshell, float tol,
-                 const float4* __restrict__ pbc, int* __restrict__ counts, int* __restrict__ ids,
-                 unsigned* __restrict__ skips, unsigned long long* __restrict__ work)
+void task_kernel(TaskLaunch tl, const float4* __restrict__ sub, int nsub,
+                 const float* __restrict__ fmax, int nshell, float tol)
 {
+    // every class with pairs in one launch (blockIdx.y: the class's entry),
+    // so the light classes' waves hide the heavy classes' gather latency
+    const TaskClass& tc = tl.c[blockIdx.y];
+    const PairScreen* __restrict__ scr = tc.scr;
+    const PairGroup* __restrict__ groups = tc.groups;
+    const int ngroups = tc.ngroups, bb = tc.bb;
+    const float4* __restrict__ pbc = tc.pbc;
+    int* __restrict__ counts = tc.counts;
+    int* __restrict__ ids = tc.ids;
+    unsigned* __restrict__ skips = tc.skips;
+    unsigned long long* __restrict__ work = tc.work;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = blockIdx.x * 4 + wave;
     if (s >= nsub) return;
@@ -626,17 +633,13 @@ void task_kernel(const PairScreen* __restrict__ scr,
     }
 }
 
-void launch_tasks(hipStream_t s, const ClassData& c, const Grid& g, const float* fmax,
-                  int nshell, float tol, int* counts, int* ids, unsigned* skips,
-                  unsigned long long* work)
+void launch_tasks(hipStream_t s, const TaskLaunch& tl, const Grid& g, const float* fmax,
+                  int nshell, float tol)
 {
-    if (c.npairs == 0) return;
+    if (tl.n == 0) return;
     const int blocks = (g.nsub + 3) / 4;
-    hipLaunchKernelGGL(task_kernel, dim3(blocks), dim3(256), 0, s, c.d_scr.as<PairScreen>(),
-                       c.d_groups.as<PairGroup>(), c.ngroups,
-                       md_bb(c.la, c.lb), g.d_sub.as<float4>(), g.nsub, fmax, nshell, tol,
-                       c.d_pbc.as<float4>(), counts, ids, skips, work);
-
+    hipLaunchKernelGGL(task_kernel, dim3(blocks, tl.n), dim3(256), 0, s, tl, g.d_sub.as<float4>(),
+                       g.nsub, fmax, nshell, tol);
     WB_HIP(hipGetLastError());
 }
 
