@@ -591,6 +591,12 @@ static void collect(Ctx& c)
 static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
                         const double* omegas, double tol, int prec, double* d_K)
 {
+    // operators: 1/r (omega = 0) and erf(omega r)/r (omega > 0); the
+    // short-range erfc operator of PySCF's omega < 0 is outside this path
+    for (int op = 0; op < nop; ++op)
+        if (!(omegas[op] >= 0.0 && omegas[op] < 1e30))
+            throw Error(WB_UNSUPPORTED, "omega must be 0 (1/r) or positive (erf); the "
+                                        "short-range erfc operator (omega < 0) is not supported");
     const bool f32 = (prec == WB_PREC_MIXED);
     const size_t rsz = f32 ? 4 : 8;
     const int nao = B.nao, Ppad = G.Ppad;

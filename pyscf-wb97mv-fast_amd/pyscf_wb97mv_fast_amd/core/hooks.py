@@ -1,57 +1,62 @@
-"""Reversible attribute patching (the reference's core/hooks.py contract).
+"""Removable attribute patches, kept as a stack of undo actions.
 
-``HookSet.wrap(owner, name, factory)`` replaces ``owner.name`` by
-``factory(original)``; ``restore_all()`` undoes every patch in reverse
-order.  An attribute that was defined on the owner itself (a module
-function, a class attribute) is written back; one that only shadowed a
-class-level attribute on an instance is deleted, re-exposing the class one
-(core/hooks.py:11-23 of the reference).
+The contract is the one PySCF users of the reference rely on
+(core/hooks.py:11-60 of the reference): ``wrap(owner, name, factory)``
+puts ``factory(current)`` in place of ``owner.name``; ``restore_all()``
+undoes every patch and every registered callback, last first.  Undoing a
+patch writes back the value the owner itself held; when the owner held
+none (an instance whose attribute came from its class), the patch is
+deleted so the class attribute shows through again.
+
+The stack is a ``contextlib.ExitStack``: each patch pushes the closure that
+reverts it, and closing the stack runs them in reverse order, also when one
+of them raises (the remaining ones still run, the first error propagates).
 """
-_MISSING = object()
+import contextlib
 
 
-def _install(owner, name, replacement):
-    own = vars(owner).get(name, _MISSING) if hasattr(owner, '__dict__') else _MISSING
-    setattr(owner, name, replacement)
-    return (owner, name, own)
-
-
-def _restore(token):
-    owner, name, own = token
-    if own is _MISSING:
-        delattr(owner, name)
-    else:
-        setattr(owner, name, own)
+def _reverter(owner, name):
+    """The action that returns owner.name to its present state."""
+    table = getattr(owner, '__dict__', None)
+    if table is not None and name not in table:
+        # nothing of its own: the patch will only shadow an inherited value
+        return lambda: delattr(owner, name)
+    held = table[name] if table is not None else getattr(owner, name)
+    return lambda: setattr(owner, name, held)
 
 
 class HookSet:
+    """Patches applied to PySCF objects, undone together."""
+
     def __init__(self):
-        self._tokens = []
-        self._done = False
+        self._undo = contextlib.ExitStack()
+        self.closed = False
+
+    def _check_open(self):
+        if self.closed:
+            raise RuntimeError('this HookSet was restored; make a new one')
 
     def wrap(self, owner, name, factory):
-        if self._done:
-            raise RuntimeError('HookSet already restored')
-        token = _install(owner, name, factory(getattr(owner, name)))
-        self._tokens.append(token)
-        return token
+        """owner.name = factory(owner.name) until restore_all()."""
+        self._check_open()
+        revert = _reverter(owner, name)
+        replacement = factory(getattr(owner, name))
+        setattr(owner, name, replacement)
+        self._undo.callback(revert)
+        return replacement
 
     def on_restore(self, fn):
-        if self._done:
-            raise RuntimeError('HookSet already restored')
-        self._tokens.append(fn)
+        """Run fn() at restore_all(), in stack order with the patches."""
+        self._check_open()
+        self._undo.callback(fn)
 
     def restore_all(self):
-        for token in reversed(self._tokens):
-            if callable(token):
-                token()
-            else:
-                _restore(token)
-        self._tokens.clear()
-        self._done = True
+        self.closed = True
+        self._undo.close()
 
     def __enter__(self):
         return self
 
     def __exit__(self, *exc):
         self.restore_all()
+        return False

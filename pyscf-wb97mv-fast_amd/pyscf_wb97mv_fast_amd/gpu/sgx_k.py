@@ -84,11 +84,11 @@ class GpuKBuilder:
     def k_raw(self, grids, qd, omegas):
         """Raw K (nop, nao, nao) = X^T (w o A_op (X qd)) for each omega."""
         basis = self._build()
-        grid = self.grid_state(grids)
         qd = lib.f64(qd)
         nao = basis.nao
         if qd.shape != (nao, nao):
             raise Unsupported('density matrix does not match the basis')
+        grid = self.grid_state(grids)
         om = lib.f64(numpy.atleast_1d(omegas))
         out = numpy.empty((len(om), nao, nao))
         t0 = time.perf_counter()
@@ -146,6 +146,8 @@ def get_k_only_gpu(builder, sgx, dm, hermi=1, direct_scf_tol=1e-13, companions=(
     """pyscf.sgx.sgx_jk.get_k_only's signature and contract for one RKS dm;
     omega from sgx.mol.omega (set by mol.with_range_coulomb)."""
     omega = float(getattr(sgx.mol, 'omega', 0.0) or 0.0)
+    if omega < 0:
+        raise Unsupported('short-range (erfc, omega < 0) exchange is not on the GPU path')
     return builder.get_k(sgx, dm, omega, hermi=hermi, direct_scf_tol=direct_scf_tol,
                          companions=companions)
 
@@ -154,18 +156,21 @@ def rsh_companions(mf):
     """The exchange operators an SCF step of mf asks for: the full-range
     operator of mf.with_df, the range-separated copies it holds
     (_rsh_df keys, '%.6f' % omega as core/sgx_patch.py:39-48 makes them)
-    and, before those exist, the omega of mf's functional."""
+    and, before those exist, the omega of mf's functional; the GPU path
+    evaluates 1/r and erf(omega r)/r, omega > 0, only."""
     oms = {0.0}
     df = getattr(mf, 'with_df', None)
     for key in (getattr(df, '_rsh_df', None) or {}):
         try:
-            oms.add(float(key))
+            omega = float(key)
         except (TypeError, ValueError):
-            pass
+            continue
+        if omega > 0:                     # erfc operators (omega < 0) are not evaluated here
+            oms.add(omega)
     ni = getattr(mf, '_numint', None)
     try:
         omega = float(ni.rsh_coeff(mf.xc)[0])
-        if omega:
+        if omega > 0:
             oms.add(omega)
     except Exception:                     # noqa: BLE001 -- no RSH information
         pass

@@ -173,6 +173,13 @@ class Context:
         """Release the workspaces (they regrow on the next call)."""
         check(load().wb_ctx_trim(self.h))
 
+    def tuning(self):
+        """(G copies, fused-class mask, GEMM tiles) the context runs with."""
+        n, m, t = ctypes.c_int(), ctypes.c_uint(), ctypes.c_int()
+        check(load().wb_ctx_get_tuning(self.h, ctypes.byref(n), ctypes.byref(m),
+                                       ctypes.byref(t)))
+        return n.value, m.value, t.value
+
 
 class DeviceBasis:
     """Basis handle; shells as arrays from basis.pack_mol."""

@@ -128,26 +128,34 @@ def test_long_range_copy_is_dispatched():
         hooks.restore_all()
 
 
-def test_oom_retry_halves_copies_then_gives_up():
-    seen = []
+def test_oom_retry_halves_copies_from_current_and_keeps_them():
+    state = {'n': 6, 'calls': []}
+
+    def get():
+        return state['n']
+
+    def put(n):
+        state['n'] = n
 
     def flow():
+        state['calls'].append(state['n'])
         raise GpuOOM('out of memory')
     with pytest.raises(GpuOOM):
-        install.run_with_oom_retry(flow, seen.append)
-    assert seen == [8, 4, 2, 1]
-    seen.clear()
-    state = {'n': 0}
+        install.run_with_oom_retry(flow, get, put)
+    assert state['calls'] == [6, 3, 1]                 # starts from the context's count
+    state.update(n=8, calls=[])
 
     def flaky():
-        state['n'] += 1
-        if state['n'] < 2:
+        state['calls'].append(state['n'])
+        if len(state['calls']) < 3:
             raise MemoryError('out of memory')
         return 'ok'
-    assert install.run_with_oom_retry(flaky, seen.append) == 'ok'
-    assert seen == [8, 4]
+    assert install.run_with_oom_retry(flaky, get, put) == 'ok'
+    assert state['calls'] == [8, 4, 2] and state['n'] == 2   # the working count stays
+    state['calls'] = []
     with pytest.raises(ValueError):
-        install.run_with_oom_retry(lambda: (_ for _ in ()).throw(ValueError('x')), seen.append)
+        install.run_with_oom_retry(lambda: (_ for _ in ()).throw(ValueError('x')), get, put)
+    assert state['n'] == 2
 
 
 def test_no_gpu_installs_nothing():
