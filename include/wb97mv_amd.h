@@ -284,6 +284,21 @@ int wb_grid_overlap(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, double* out)
  * reference's ValueError. */
 int wb_boys_eval(wb_ctx_t ctx, int n, const double* T, int nmax, int precision, double* out);
 
+/* Test hook (host only, no device work): the screening records of the
+ * basis's shell pairs, built as wb_basis_create builds them, for checking
+ * the bounds against exact integrals (the reference checks its host bound
+ * the same way, tests/gpu/test_shellpairs.py:36-56).  counts2 = {pairs,
+ * primitive pairs}; with pairs and prims non-NULL (sized from a first call
+ * with NULLs) it also writes, per pair, {shell A, shell B, bound centre
+ * x, y, z, spread delta, beta, q, first primitive pair, primitive pairs} --
+ * |A_ab(g)| <= beta min(1, sqrt(pi) / (2 sqrt(q) d)), d = |g - centre| -
+ * delta -- and per primitive pair {P x, y, z, c, primitive of A, primitive
+ * of B} with |A^(ab)(g)| <= c / |g - P|, the primitive-tail bound.  FP32
+ * values as the device holds them, widened to double. */
+int wb_test_pair_bounds(int nshell, const int* l, const int* nprim, const double* center,
+                        const double* exps, const double* coefs, double pair_cut,
+                        long long* counts2, double* pairs, double* prims);
+
 /* Test hook (no reference counterpart): the FP64 MFMA GEMMs on host data.
  * kind 0: C[M][N] = A[M][K] B[K][N]; kind 1: C[M][N] = sum_k A[M][k]
  * B[N][k] w[k] (w may be NULL); kinds 2 / 3: the same with the 64 x 64

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <tuple>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -51,20 +52,19 @@ void boys_tables(std::vector<float>& t32, std::vector<double>& t64)
         for (int k = 0; k < B32_NODES; ++k) {
             const long double T = 0.25L * k;
             float* row = tab + k * rs;
-            // orders with a Taylor series in this table, and where each sits
-            const int n0 = L <= 1 ? 0 : L, n1 = L <= 1 ? L : L;
-            for (int n = n0; n <= n1; ++n) {
-                float* c = row + (L == 1 ? 7 * n : 0);
-                for (int j = 0; j <= 6; ++j)
-                    c[j] = (float)(boys_ld(n + j, T) / fact[j]);
-                const long double v = boys_ld(n, T);
-                const int lo = L == 0 ? 7 : (L == 1 ? 14 + n : 7);
-                row[lo] = (float)(v - (long double)c[0]);
-            }
-            if (L >= 2) {
-                const long double e = expl(-T);
-                row[8] = (float)e;
-                row[9] = (float)(e - (long double)row[8]);
+            if (L <= 1) {
+                // every order its own series; leading values with remainders
+                for (int n = 0; n <= L; ++n) {
+                    float* c = row + 7 * n;
+                    for (int j = 0; j <= 6; ++j)
+                        c[j] = (float)(boys_ld(n + j, T) / fact[j]);
+                    row[(L == 0 ? 7 : 14 + n)] = (float)(boys_ld(n, T) - (long double)c[0]);
+                }
+            } else {
+                // F_L(T_k) and e^{-T_k} as doubles, then c_{L,1..6}
+                const double lead[2] = {(double)boys_ld(L, T), (double)expl(-T)};
+                std::memcpy(row, lead, sizeof(lead));
+                for (int j = 1; j <= 6; ++j) row[3 + j] = (float)(boys_ld(L + j, T) / fact[j]);
             }
         }
     }
@@ -142,6 +142,15 @@ static double rowsum_c2s_max(int l)
     }
 }
 
+// a bound stored in FP32: never below the FP64 value, and never zero (a
+// factor below FP32's normal range, K_AB ~ 1e-50, becomes FLT_MIN)
+static float round_up_f32(double v)
+{
+    float f = (float)v;
+    if ((double)f < v) f = std::nextafter(f, INFINITY);
+    return std::max(f, FLT_MIN);
+}
+
 static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
                         const double* center, const double* exps, const double* coefs,
                         double pair_cut)
@@ -187,6 +196,7 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         PairScreen ps{};
         std::vector<double> P, p, pref, E;
         std::vector<float4> pbc;
+        std::vector<int> pij;
     };
     std::vector<PairOut> outs(order.size());
     auto make_pair = [&](size_t idx) {
@@ -221,12 +231,14 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         std::vector<double> Ps;
         const double rs = rowsum_c2s_max(la) * rowsum_c2s_max(lb);
         // primitive pairs, most important first: (far-field bound, index)
-        struct Prim { double key, P[3], p, pref, beta; double E[3][4][4][7]; };
+        struct Prim { double key, P[3], p, pref, beta; int ia, ib; double E[3][4][4][7]; };
         std::vector<Prim> prims;
         prims.reserve((size_t)A.nprim * Bs.nprim);
         for (int i = 0; i < A.nprim; ++i)
             for (int j = 0; j < Bs.nprim; ++j) {
                 Prim pm{};
+                pm.ia = i;
+                pm.ib = j;
                 const double ea = exps[A.prim0 + i], eb = exps[Bs.prim0 + j];
                 const double p = ea + eb, mu = ea * eb / p;
                 const double K = std::exp(-mu * ab2);
@@ -267,9 +279,11 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
                 for (int ii = 0; ii <= la; ++ii)
                     for (int jj = 0; jj <= lb; ++jj)
                         for (int t = 0; t <= L; ++t) o.E.push_back(pm.E[d][ii][jj][t]);
+            o.pij.push_back(pm.ia);
+            o.pij.push_back(pm.ib);
             o.pbc.push_back(make_float4((float)pm.P[0], (float)pm.P[1], (float)pm.P[2],
-                                        (float)(0.8862269254527580 * pm.beta /
-                                                std::sqrt(0.75 * pm.p) * (1 + 1e-5))));
+                                        round_up_f32(0.8862269254527580 * pm.beta /
+                                                     std::sqrt(0.75 * pm.p) * (1 + 1e-5))));
             beta += pm.beta;
             if (pm.p < pmin) { pmin = pm.p; Pd[0] = pm.P[0]; Pd[1] = pm.P[1]; Pd[2] = pm.P[2]; }
             Ps.insert(Ps.end(), pm.P, pm.P + 3);
@@ -281,7 +295,7 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         }
         ps.cx = (float)Pd[0]; ps.cy = (float)Pd[1]; ps.cz = (float)Pd[2];
         ps.delta = (float)(delta * (1 + 1e-6) + 1e-5);
-        ps.beta = (float)(beta * (1 + 1e-5));
+        ps.beta = round_up_f32(beta * (1 + 1e-5));
         ps.q = (float)(0.75 * pmin * (1 - 1e-6));
         pr.npp = (int)prims.size();
     };
@@ -329,6 +343,7 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         C.pref.insert(C.pref.end(), o.pref.begin(), o.pref.end());
         C.E.insert(C.E.end(), o.E.begin(), o.E.end());
         C.pbc.insert(C.pbc.end(), o.pbc.begin(), o.pbc.end());
+        C.pij.insert(C.pij.end(), o.pij.begin(), o.pij.end());
         C.nprims += pr.npp;
         C.pairs.push_back(pr);
         C.scr.push_back(ps);
@@ -1982,6 +1997,47 @@ int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
             WB_HIP(hipMemcpyAsync(out, dW.p, sizeof(double) * (size_t)npts, hipMemcpyDeviceToHost,
                                   c->stream));
         WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_test_pair_bounds(int nshell, const int* l, const int* nprim, const double* center,
+                        const double* exps, const double* coefs, double pair_cut,
+                        long long* counts2, double* pairs, double* prims)
+{
+    WB_TRY({
+        if (nshell < 1 || !l || !nprim || !center || !exps || !coefs || !counts2)
+            throw Error(WB_INVALID, "null argument");
+        auto B = std::make_unique<Basis>();
+        build_basis(*B, nshell, l, nprim, center, exps, coefs, pair_cut);
+        long long np = 0, nm = 0;
+        for (const auto& C : B->cls) {
+            np += C.npairs;
+            nm += C.nprims;
+        }
+        counts2[0] = np;
+        counts2[1] = nm;
+        if (pairs && prims) {
+            size_t ip = 0;
+            long long mbase = 0;
+            for (const auto& C : B->cls) {
+                for (int i = 0; i < C.npairs; ++i) {
+                    const PairRec& pr = C.pairs[i];
+                    const PairScreen& ps = C.scr[i];
+                    double* o = pairs + 10 * ip++;
+                    o[0] = pr.shA; o[1] = pr.shB;
+                    o[2] = ps.cx; o[3] = ps.cy; o[4] = ps.cz;
+                    o[5] = ps.delta; o[6] = ps.beta; o[7] = ps.q;
+                    o[8] = (double)(mbase + pr.pp0); o[9] = pr.npp;
+                }
+                for (int m = 0; m < C.nprims; ++m) {
+                    const float4 q = C.pbc[m];
+                    double* o = prims + 6 * (mbase + m);
+                    o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w;
+                    o[4] = C.pij[2 * (size_t)m]; o[5] = C.pij[2 * (size_t)m + 1];
+                }
+                mbase += C.nprims;
+            }
+        }
     })
 }
 

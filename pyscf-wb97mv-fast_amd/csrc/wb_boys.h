@@ -7,8 +7,15 @@
 // For L >= 2 only F_L comes from the table; the lower orders follow by the
 // stable downward recursion F_{n-1} = (2T F_n + e^{-T}) / (2n-1), with
 // e^{-T} = e^{-T_k} e^{-h} (table value times a Taylor polynomial), so no
-// transcendental is evaluated below the switch.  The reference uses the same
-// node spacing and switch for FP32 (gpu/boys.py:14-47, T_k = k/4, 36).
+// transcendental is evaluated below the switch.  In the FP32 routine the
+// recursion (and the asymptotic product for L >= 1) runs in FP64 from FP64
+// leading values: each step is a convex combination of the errors of F_L
+// and e^{-T}, so no order accumulates roundings, and every F_n is rounded
+// to FP32 once (<= 0.5 ulp plus ~3e-8), at about the instruction count of
+// an FP32 recursion with its divisions (FP64 FMA issues at the FP32 rate
+// on CDNA4).  The reference uses the same node spacing and switch for FP32
+// (gpu/boys.py:14-47, T_k = k/4, 36) and evaluates every order from a
+// series of its own.
 #pragma once
 #include "wb_common.h"
 
@@ -18,11 +25,12 @@ namespace wb {
 // kernel class needs only its own).  Taylor coefficients are stored divided
 // by j!, c_{n,j} = F_{n+j}(T_k) / j!, so the series is a plain Horner chain
 // in x = T_k - T.  Row of table L:
-//   L = 0:  c_{0,0..6}, lo_0                         (8 values)
-//   L = 1:  c_{0,0..6}, c_{1,0..6}, lo_0, lo_1       (16 values)
-//   L >= 2: c_{L,0..6}, lo_L, e_hi, e_lo             (10 values)
-// lo: FP32 rounding remainder of the leading value; e = e^{-T_k}.  Rows are
-// padded to a multiple of 4 floats (16-byte loads) plus 4, so that rows of
+//   L = 0:  c_{0,0..6}, lo_0                         (8 floats)
+//   L = 1:  c_{0,0..6}, c_{1,0..6}, lo_0, lo_1       (16 floats)
+//   L >= 2: F_L(T_k) and e^{-T_k} as two doubles, then c_{L,1..6}
+//           (10 floats' worth; the doubles 8-byte aligned)
+// lo: FP32 rounding remainder of the leading value.  Rows are padded to a
+// multiple of 4 floats (16-byte loads) plus 4, so that rows of
 // neighbouring nodes, which the lanes of a wave read together, start on
 // different LDS banks.
 constexpr int B32_NODES = 145;
@@ -59,10 +67,10 @@ __device__ __forceinline__ float taylor6(const float* c, float x) {
     return s * x;
 }
 
-// F_0..F_L(T); tab is the order-L table.  The leading table values and
-// sqrt(pi)/2 carry their FP32 rounding remainders: a table value serves
-// every (point, pair) near its node, so its rounding would bias all of them
-// the same way.
+// F_0..F_L(T); tab is the order-L table.  The leading table values carry
+// their FP32 rounding remainders (L <= 1) or are FP64 (L >= 2), and so does
+// sqrt(pi)/2: a table value serves every (point, pair) near its node, so its
+// rounding would bias all of them the same way.
 template <int L>
 __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
     if (T < B32_SWITCH) {
@@ -75,7 +83,8 @@ __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab,
             F[0] = row[0] + (row[14] + taylor6(row, x));
             F[1] = row[7] + (row[15] + taylor6(row + 7, x));
         } else {
-            F[L] = row[0] + (row[7] + taylor6(row, x));
+            const double* rd = (const double*)row;        // F_L(T_k), e^{-T_k}
+            const double fl = rd[0] + (double)taylor6(row + 3, x);
             // e^{x} - 1 to O(x^7)
             float es = 1.0f / 720.0f;
             es = fmaf(es, x, 1.0f / 120.0f);
@@ -84,15 +93,18 @@ __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab,
             es = fmaf(es, x, 0.5f);
             es = fmaf(es, x, 1.0f);
             es *= x;
-            // e^{-T} = e^{-T_k} e^{x} = (e_hi + e_lo)(1 + es)
-            const float e = row[8] + fmaf(row[8], es, row[9]);
-            const float t2 = T + T;
+            // e^{-T} = e^{-T_k} e^{x}
+            const double e = fma(rd[1], (double)es, rd[1]);
+            const double t2 = 2.0 * (double)T;
+            double f = fl;
+            F[L] = (float)f;
 #pragma unroll
-            for (int n = L; n > 0; --n)
-                F[n - 1] = div_small(fmaf(t2, F[n], e), (float)(2 * n - 1),
-                                     1.0f / (float)(2 * n - 1));
+            for (int n = L; n > 0; --n) {
+                f = fma(t2, f, e) * (1.0 / (double)(2 * n - 1));
+                F[n - 1] = (float)f;
+            }
         }
-    } else {
+    } else if constexpr (L == 0) {
         // 1/sqrt(T) to ~0.5 ulp: hardware estimate, then one Newton step on
         // the exact residual 1 - T s^2 (T s split into hi + lo by an FMA),
         // so the correction is not rounded against 1.0
@@ -100,10 +112,22 @@ __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab,
         const float th = T * s, tl = fmaf(T, s, -th);
         const float h = fmaf(-th, s, 1.0f) - tl * s;
         s = fmaf(0.5f * s, h, s);
-        const float u = s * s;
         F[0] = fmaf(0.886226952f, s, -2.66231996e-08f * s);      // sqrt(pi)/2 hi + lo
+    } else {
+        // FP64 from the hardware estimate: one Newton step (error ~1.5 r^2,
+        // r ~ 1e-7) gives 1/sqrt(T) and 1/T far below FP32 rounding; the
+        // upward product F_n = F_{n-1} (n - 1/2) / T then rounds once per order
+        const double sd = (double)__builtin_amdgcn_rsqf(T);
+        const double r = fma(-(double)T, sd * sd, 1.0);
+        const double s1 = fma(0.5 * sd, r, sd);
+        const double u = s1 * s1;
+        double f = 0.886226925452758014 * s1;
+        F[0] = (float)f;
 #pragma unroll
-        for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((float)n - 0.5f) * u;
+        for (int n = 1; n <= L; ++n) {
+            f *= ((double)n - 0.5) * u;
+            F[n] = (float)f;
+        }
     }
 }
 

@@ -86,6 +86,7 @@ struct ClassData {
     // and far-field bound factor beta sqrt(pi) / (2 sqrt(0.75 p)) (rounded
     // up) of the per-task primitive screening
     std::vector<float4> pbc;
+    std::vector<int> pij;             // (nprims, 2) primitive indices in shells A and B (host)
     // pairs are ordered by (B block, A, B); groups: runs of one (B block, A)
     std::vector<PairGroup> groups;
     std::vector<int> bbao;            // [nblk][md_bb]: AO offset of the shell at a block position, nao (spare rows) if none

@@ -149,7 +149,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     using vacc = typename std::conditional<(NCA * NCB <= 36), double, real>::type;
     // FP32: this class's Boys table (order L) is staged in LDS
     constexpr int TAB = F32 ? B32_NODES * b32_row(L) : 1;
-    __shared__ real s_tab[TAB];
+    __shared__ __align__(16) real s_tab[TAB];
     const real* tab = boys_g;
     if constexpr (F32) {
         const real* src = boys_g + b32_off(L);

@@ -78,6 +78,7 @@ _SIGS = {
                                ctypes.POINTER(_i)]),
     'wb_ctx_set_tuning': (_i, [_vp, _i, ctypes.c_uint, _i]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'wb_test_pair_bounds': (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     'wb_xc_rho': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_xc_vmat': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     'wb_cycle_graph_create': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i,
