@@ -83,6 +83,19 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
                     double* report);
 int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask, int* gemm_tiles);
 int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask, int gemm_tiles);
+/* Workspace budget of the context's K builds in bytes (0, the default: no
+ * limit).  A K build whose workspace (AO values, F, the G copies, task
+ * lists, the K GEMM's partials) would exceed it runs over chunks of the
+ * grid's 64-point sub-blocks, each a whole number of the K GEMM's point
+ * splits, and accumulates K chunk by chunk in split order: the result is
+ * bit for bit the one-chunk result.  A budget below one split's workspace
+ * makes the K build fail with WB_OOM.  The reference tiles its grid by a
+ * memory budget the same way (gpu/sgx_k.py:502-520, :558-570). */
+int wb_ctx_set_k_budget(wb_ctx_t ctx, long long bytes);
+/* The plan a K build on (basis, grid) would use with the context's copy
+ * count and budget: out3 = {workspace bytes, chunks, sub-blocks per chunk}. */
+int wb_k_plan(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, int nop, int precision,
+              long long* out3);
 /* mixed form: accumulate G in FP64 (1) or FP32 (0, default) */
 int wb_ctx_set_g64(wb_ctx_t ctx, int on);
 int wb_sync(wb_ctx_t ctx);

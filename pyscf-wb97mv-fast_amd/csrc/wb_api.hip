@@ -603,8 +603,102 @@ static void collect(Ctx& c)
 }
 
 // ------------------------------------------------------------- K build ----
-static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
-                        const double* omegas, double tol, int prec, double* d_K)
+// The workspace of one K build, carved from one arena (offsets 256-byte
+// aligned).  The grid is processed in chunks of csub sub-blocks, a whole
+// number of the K GEMM's point splits (kchunk points each, fixed by the
+// whole grid), so the split partials and the order in which they are added
+// are those of the one-chunk build: a tiled build equals the untiled one bit
+// for bit.  Everything else is per point or per sub-block.  csub = nsub: one
+// chunk, the chunk is the grid itself (no copies of its coordinates).
+struct KPlan {
+    int kchunk = 0, csub = 0, nchunk = 0;
+    size_t nitem_sub = 0;           // item-list entries per sub-block, all classes
+    size_t qdt = 0, x = 0, f = 0, fmax = 0, counts = 0, ids = 0, skips = 0, g = 0, mask = 0,
+           cp = 0, fold = 0, xyz = 0, w = 0, sub = 0, total = 0;
+};
+
+static KPlan k_plan(const Basis& B, const Grid& G, int nop, bool f32, bool g64, int ncopy,
+                    int csub)
+{
+    KPlan P;
+    nt_split(G.Ppad, P.kchunk);
+    P.csub = std::max(1, std::min(csub, G.nsub));
+    P.nchunk = (G.nsub + P.csub - 1) / P.csub;
+    for (const auto& C : B.cls) P.nitem_sub += (size_t)C.ngroups;
+    const size_t nao = B.nao, Pc = (size_t)P.csub * SUB, zrow = nao + MD_SPARE;
+    const size_t nsplit = (Pc + P.kchunk - 1) / P.kchunk;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    P.qdt = take(8 * nao * nao);
+    P.x = take(8 * nao * Pc);
+    P.f = take((f32 ? 4 : 8) * (nao + MD_SPARE) * Pc);
+    P.fmax = take(4 * (size_t)P.csub * B.nshell);
+    P.counts = take(4 * (size_t)NCLASS * P.csub);
+    P.ids = take(4 * std::max<size_t>(P.nitem_sub * P.csub, 1));
+    P.skips = take(4 * std::max<size_t>(P.nitem_sub * P.csub, 1));
+    P.g = take((g64 ? 8 : 4) * (zrow + 1) * Pc * ncopy * nop);
+    P.mask = take(zrow <= 32 * MD_MASK_WORDS ? 4 * (size_t)ncopy * P.csub * MD_MASK_WORDS * nop : 0);
+    P.cp = take(8 * nsplit * nao * nao);
+    P.fold = take(8 * nao * Pc);
+    if (P.nchunk > 1) {
+        P.xyz = take(8 * 3 * Pc);
+        P.w = take(8 * Pc);
+        P.sub = take(sizeof(float4) * P.csub);
+    }
+    P.total = off;
+    return P;
+}
+
+// the plan with the most sub-blocks per chunk whose workspace fits budget
+// bytes (0: the whole grid in one chunk)
+static KPlan k_plan_budget(const Basis& B, const Grid& G, int nop, bool f32, bool g64, int ncopy,
+                           size_t budget)
+{
+    KPlan one = k_plan(B, G, nop, f32, g64, ncopy, G.nsub);
+    if (budget == 0 || one.total <= budget) return one;
+    const int step = one.kchunk / SUB;                  // sub-blocks per split
+    const int nsteps = (G.nsub + step - 1) / step;
+    int lo = 1, hi = nsteps;                            // chunk = lo..hi whole splits
+    KPlan best = k_plan(B, G, nop, f32, g64, ncopy, step);
+    if (best.total > budget)
+        throw Error(WB_OOM, "K build needs at least " + std::to_string(best.total) +
+                                " bytes of workspace (budget " + std::to_string(budget) + ")");
+    while (lo < hi) {                                   // total grows with the chunk
+        const int mid = (lo + hi + 1) / 2;
+        KPlan p = k_plan(B, G, nop, f32, g64, ncopy, mid * step);
+        if (p.total <= budget) { lo = mid; best = p; }
+        else hi = mid - 1;
+    }
+    return best;
+}
+
+// the records of an operator set if they are on the device (no allocation
+// of any kind: the allocation-free K build uses this)
+static const void* op_records_find(ClassData& C, const double* omegas, int nop, bool f32)
+{
+    std::lock_guard<std::mutex> lock(C.opc_mu);
+    for (const auto& kv : C.opc)
+        if ((int)kv.first.size() == nop && std::equal(omegas, omegas + nop, kv.first.begin()))
+            return f32 ? kv.second.first->p : kv.second.second->p;
+    throw Error(WB_INVALID, "operator records for these omegas are not on the device yet: run "
+                            "wb_k_build or wb_k_build_dev once with them on this basis");
+}
+
+// primitive-pair records of an operator set: built (and cached on the
+// basis) when allowed, else only looked up
+static const void* op_records(ClassData& C, const std::vector<double>& omegas, bool f32,
+                              hipStream_t s, bool may_build)
+{
+    return may_build ? op_consts(C, omegas, f32, s) : op_records_find(C, omegas.data(),
+                                                                       (int)omegas.size(), f32);
+}
+
+// The K build on the arena (P.total bytes at arena; work: the per-class
+// counters, 2 NCLASS x u64).  noalloc: no allocating call at all (records
+// looked up, no profiling events).
+static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
+                          const double* omegas, double tol, int prec, double* d_K,
+                          const KPlan& P, char* arena, unsigned long long* work, bool noalloc)
 {
     // operators: 1/r (omega = 0) and erf(omega r)/r (omega > 0); the
     // short-range erfc operator of PySCF's omega < 0 is outside this path
@@ -613,112 +707,158 @@ static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
             throw Error(WB_UNSUPPORTED, "omega must be 0 (1/r) or positive (erf); the "
                                         "short-range erfc operator (omega < 0) is not supported");
     const bool f32 = (prec == WB_PREC_MIXED);
-    const size_t rsz = f32 ? 4 : 8;
-    const int nao = B.nao, Ppad = G.Ppad;
-    hipStream_t s = c.stream;
-    GemmTilesScope tiles(c.gemm_tiles);
-    c.start();
-    double* X = (double*)c.ws_X.ensure(sizeof(double) * nao * (size_t)Ppad);
-    launch_ao_eval(s, B, G, 0, X);
-    c.mark(PH_AO);
-    double* QDT = (double*)c.ws_D.ensure(sizeof(double) * nao * (size_t)nao);
-    transpose(s, nao, d_QD, QDT);
-    // F and G carry MD_SPARE spare rows after the AO rows for the padding
-    // positions of the exchange kernels' B blocks
-    void* F = c.ws_F.ensure(rsz * (nao + MD_SPARE) * (size_t)Ppad);
-    gemm_nn_fast(s, nao, Ppad, nao, QDT, X, F, f32);
-    c.mark(PH_FGEMM);
-    // screening and task lists
-    const int* d_sh = B.d_shao.as<int>();
-    float* fmax = (float*)c.ws_fmax.ensure(sizeof(float) * (size_t)G.nsub * B.nshell);
-    launch_fmax(s, F, f32, G.d_w.as<double>(), nao, Ppad, G.nsub, d_sh, d_sh + B.nshell, B.nshell, fmax);
-    size_t ntask = 0, nitem = 0;
-    for (auto& C : B.cls) {
-        ntask += (size_t)C.npairs * G.nsub;
-        nitem += (size_t)C.ngroups * G.nsub;
-    }
-    int* counts = (int*)c.ws_counts.ensure(sizeof(int) * (size_t)NCLASS * G.nsub);
-    zero_bytes(s, counts, sizeof(int) * (size_t)NCLASS * G.nsub);
-    auto* work = (unsigned long long*)c.ws_work.ensure(sizeof(unsigned long long) * 2 * NCLASS);
-    zero_bytes(s, work, sizeof(unsigned long long) * 2 * NCLASS);
-    int* ids = (int*)c.ws_tasks.ensure(sizeof(int) * std::max<size_t>(nitem, 1));
-    unsigned* skips = (unsigned*)c.ws_skips.ensure(sizeof(unsigned) * std::max<size_t>(nitem, 1));
-    size_t toff = 0;
-    std::vector<size_t> cls_off(NCLASS);
-    TaskLaunch tl;
-    for (int k = 0; k < NCLASS; ++k) {
-        cls_off[k] = toff;
-        const ClassData& C = B.cls[k];
-        if (C.npairs) {
-            TaskClass& t = tl.c[tl.n++];
-            t.scr = C.d_scr.as<PairScreen>();
-            t.groups = C.d_groups.as<PairGroup>();
-            t.pbc = C.d_pbc.as<float4>();
-            t.counts = counts + (size_t)k * G.nsub;
-            t.ids = ids + toff;
-            t.skips = skips + toff;
-            t.work = work + 2 * k;
-            t.ngroups = C.ngroups;
-            t.bb = md_bb(C.la, C.lb);
-        }
-        toff += (size_t)C.ngroups * G.nsub;
-    }
-    launch_tasks(s, tl, G, fmax, B.nshell, (float)tol);
-    c.tasks_total = (long long)ntask;
-    const int ncopy = c.ncopy;
-    // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),
-    // one zero row (read for a shell's first update in a region)
-    const int zrow = nao + MD_SPARE;
-    const size_t gstride = (size_t)(zrow + 1) * Ppad;
-    const size_t opstride = gstride * ncopy;              // elements between operators' G
     const bool g64 = !f32 || c.g64;
-    const size_t gsz = g64 ? 8 : 4;
-    char* Gb = (char*)c.ws_G.ensure(gsz * opstride * nop);
-    const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
-    c.mark(PH_SCREEN);
-    // G is never cleared: written-shell masks per (copy, sub-block) region
-    // send each shell's first update to the zero row (rows < 2048; larger
-    // bases clear G in the first class launch instead)
-    const bool tracked = zrow <= 32 * MD_MASK_WORDS;
-    const size_t mopstride = (size_t)ncopy * G.nsub * MD_MASK_WORDS;
-    unsigned* masks = tracked ? (unsigned*)c.ws_mask.ensure(sizeof(unsigned) * mopstride * nop)
-                              : nullptr;
-    bool first = true;
+    const int nao = B.nao, ncopy = c.ncopy;
+    const size_t nn = (size_t)nao * nao;
+    hipStream_t s = c.stream;
+    const bool prof = c.profile && !noalloc;
+    GemmTilesScope tiles(c.gemm_tiles);
+    if (prof) c.start();
+    auto mark = [&](int ph) { if (prof) c.mark(ph); };
+    // every class's records first (building them may allocate and synchronize)
+    const void* rec1[NCLASS][8] = {};
+    const void* rec2[NCLASS] = {};
     for (int k = 0; k < NCLASS; ++k) {
         ClassData& C = B.cls[k];
         if (!C.npairs) continue;
-        const int* cnt = counts + (size_t)k * G.nsub;
-        const int* ids_k = ids + cls_off[k];
-        const unsigned* skips_k = skips + cls_off[k];
-        const int zrows = first ? zrow : 0;
-        if (nop == 2 && md_fuse(C.la, C.lb) && (c.fuse_mask >> k & 1u)) {
-            // both operators in one pass over the tasks
-            const void* rc = op_consts(C, {omegas[0], omegas[1]}, f32, s);
-            MdMasks mk{masks, mopstride, zrow, first};
-            launch_md(s, C, G, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k, skips_k,
-                      rc, boys, zrows, mk);
-        } else {
-            for (int op = 0; op < nop; ++op) {
-                const void* rc = op_consts(C, {omegas[op]}, f32, s);
-                MdMasks mk{tracked ? masks + mopstride * op : nullptr, mopstride, zrow, first};
-                launch_md(s, C, G, f32, g64, 1, F, Gb + gsz * opstride * op, gstride, opstride,
-                          ncopy, cnt, ids_k, skips_k, rc, boys, zrows, mk);
-            }
+        if (nop == 2 && md_fuse(C.la, C.lb) && (c.fuse_mask >> k & 1u))
+            rec2[k] = noalloc ? op_records_find(C, omegas, 2, f32)
+                              : op_records(C, {omegas[0], omegas[1]}, f32, s, true);
+        else
+            for (int op = 0; op < nop; ++op)
+                rec1[k][op] = noalloc ? op_records_find(C, omegas + op, 1, f32)
+                                      : op_records(C, {omegas[op]}, f32, s, true);
+    }
+    double* QDT = (double*)(arena + P.qdt);
+    transpose(s, nao, d_QD, QDT);
+    zero_bytes(s, work, sizeof(unsigned long long) * 2 * NCLASS);
+    const int zrow = nao + MD_SPARE;
+    const bool tracked = zrow <= 32 * MD_MASK_WORDS;
+    const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
+    const int* d_sh = B.d_shao.as<int>();
+    size_t ntask = 0;
+    for (auto& C : B.cls) ntask += (size_t)C.npairs * G.nsub;
+    for (int ci = 0; ci < P.nchunk; ++ci) {
+        const int s0 = ci * P.csub, ns = std::min(P.csub, G.nsub - s0);
+        const int Ppad = ns * SUB;
+        // the chunk's points as a grid of their own (views into the arena)
+        Grid Gc;
+        const Grid* gv = &G;
+        if (P.nchunk > 1) {
+            Gc.nsub = ns;
+            Gc.Ppad = Ppad;
+            Gc.npts = std::min(Ppad, G.npts - s0 * SUB);
+            Gc.d_xyz.borrow(arena + P.xyz, 8 * 3 * (size_t)Ppad);
+            Gc.d_w.borrow(arena + P.w, 8 * (size_t)Ppad);
+            Gc.d_sub.borrow(arena + P.sub, sizeof(float4) * ns);
+            for (int d = 0; d < 3; ++d)
+                WB_HIP(hipMemcpyAsync(Gc.d_xyz.as<double>() + (size_t)d * Ppad,
+                                      G.d_xyz.as<double>() + (size_t)d * G.Ppad + (size_t)s0 * SUB,
+                                      8 * (size_t)Ppad, hipMemcpyDeviceToDevice, s));
+            WB_HIP(hipMemcpyAsync(Gc.d_w.p, G.d_w.as<double>() + (size_t)s0 * SUB,
+                                  8 * (size_t)Ppad, hipMemcpyDeviceToDevice, s));
+            WB_HIP(hipMemcpyAsync(Gc.d_sub.p, G.d_sub.as<float4>() + s0, sizeof(float4) * ns,
+                                  hipMemcpyDeviceToDevice, s));
+            gv = &Gc;
         }
-        first = false;
+        double* X = (double*)(arena + P.x);
+        launch_ao_eval(s, B, *gv, 0, X);
+        mark(PH_AO);
+        // F and G carry MD_SPARE spare rows after the AO rows for the padding
+        // positions of the exchange kernels' B blocks
+        void* F = arena + P.f;
+        gemm_nn_fast(s, nao, Ppad, nao, QDT, X, F, f32);
+        mark(PH_FGEMM);
+        // screening and task lists
+        float* fmax = (float*)(arena + P.fmax);
+        launch_fmax(s, F, f32, gv->d_w.as<double>(), nao, Ppad, ns, d_sh, d_sh + B.nshell,
+                    B.nshell, fmax);
+        int* counts = (int*)(arena + P.counts);
+        zero_bytes(s, counts, sizeof(int) * (size_t)NCLASS * ns);
+        int* ids = (int*)(arena + P.ids);
+        unsigned* skips = (unsigned*)(arena + P.skips);
+        size_t toff = 0;
+        size_t cls_off[NCLASS];
+        TaskLaunch tl;
+        for (int k = 0; k < NCLASS; ++k) {
+            cls_off[k] = toff;
+            const ClassData& C = B.cls[k];
+            if (C.npairs) {
+                TaskClass& t = tl.c[tl.n++];
+                t.scr = C.d_scr.as<PairScreen>();
+                t.groups = C.d_groups.as<PairGroup>();
+                t.pbc = C.d_pbc.as<float4>();
+                t.counts = counts + (size_t)k * ns;
+                t.ids = ids + toff;
+                t.skips = skips + toff;
+                t.work = work + 2 * k;
+                t.ngroups = C.ngroups;
+                t.bb = md_bb(C.la, C.lb);
+            }
+            toff += (size_t)C.ngroups * ns;
+        }
+        launch_tasks(s, tl, *gv, fmax, B.nshell, (float)tol);
+        // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),
+        // one zero row (read for a shell's first update in a region)
+        const size_t gstride = (size_t)(zrow + 1) * Ppad;
+        const size_t opstride = gstride * ncopy;          // elements between operators' G
+        const size_t gsz = g64 ? 8 : 4;
+        char* Gb = arena + P.g;
+        mark(PH_SCREEN);
+        // G is never cleared: written-shell masks per (copy, sub-block) region
+        // send each shell's first update to the zero row (rows < 2048; larger
+        // bases clear G in the first class launch instead)
+        const size_t mopstride = (size_t)ncopy * ns * MD_MASK_WORDS;
+        unsigned* masks = tracked ? (unsigned*)(arena + P.mask) : nullptr;
+        bool first = true;
+        for (int k = 0; k < NCLASS; ++k) {
+            ClassData& C = B.cls[k];
+            if (!C.npairs) continue;
+            const int* cnt = counts + (size_t)k * ns;
+            const int* ids_k = ids + cls_off[k];
+            const unsigned* skips_k = skips + cls_off[k];
+            const int zrows = first ? zrow : 0;
+            if (rec2[k]) {
+                // both operators in one pass over the tasks
+                MdMasks mk{masks, mopstride, zrow, first};
+                launch_md(s, C, *gv, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k,
+                          skips_k, rec2[k], boys, zrows, mk);
+            } else {
+                for (int op = 0; op < nop; ++op) {
+                    MdMasks mk{tracked ? masks + mopstride * op : nullptr, mopstride, zrow, first};
+                    launch_md(s, C, *gv, f32, g64, 1, F, Gb + gsz * opstride * op, gstride,
+                              opstride, ncopy, cnt, ids_k, skips_k, rec1[k][op], boys, zrows, mk);
+                }
+            }
+            first = false;
+        }
+        if (first) {                                    // no shell pair at all
+            if (tracked) zero_bytes(s, masks, sizeof(unsigned) * mopstride * nop);
+            else zero_bytes(s, Gb, gsz * opstride * nop);
+        }
+        mark(PH_MD);
+        for (int op = 0; op < nop; ++op) {
+            gemm_nt_core(s, nao, nao, Ppad, P.kchunk, X, Gb + gsz * opstride * op, !g64, ncopy,
+                         gstride, gv->d_w.as<double>(), d_K + (size_t)op * nn, ci > 0,
+                         (double*)(arena + P.cp), (double*)(arena + P.fold),
+                         tracked ? masks + mopstride * op : nullptr, B.d_row0.as<int>(), ns);
+            mark(PH_KGEMM);
+        }
     }
-    if (first) {                                    // no shell pair at all
-        if (tracked) zero_bytes(s, masks, sizeof(unsigned) * mopstride * nop);
-        else zero_bytes(s, Gb, gsz * opstride * nop);
-    }
-    c.mark(PH_MD);
-    for (int op = 0; op < nop; ++op) {
-        gemm_nt_fast(s, nao, nao, Ppad, X, Gb + gsz * opstride * op, !g64, ncopy, gstride,
-                     G.d_w.as<double>(), d_K + (size_t)op * nao * nao, c.ws_Kp,
-                     tracked ? masks + mopstride * op : nullptr, B.d_row0.as<int>(), G.nsub);
-        c.mark(PH_KGEMM);
-    }
-    collect(c);
+    if (!noalloc) c.tasks_total = (long long)ntask;
+    if (prof) collect(c);
+}
+
+// the K build with the context's workspaces (the arena sized by the
+// context's budget)
+static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
+                        const double* omegas, double tol, int prec, double* d_K)
+{
+    const bool f32 = prec == WB_PREC_MIXED;
+    const KPlan P = k_plan_budget(B, G, nop, f32, !f32 || c.g64, c.ncopy, c.k_budget);
+    char* arena = (char*)c.ws_karena.ensure(P.total);
+    auto* work = (unsigned long long*)c.ws_work.ensure(sizeof(unsigned long long) * 2 * NCLASS);
+    k_build_arena(c, B, G, d_QD, nop, omegas, tol, prec, d_K, P, arena, work, false);
 }
 
 // kept tasks (sub-block x shell pair) of the last screening, all classes
@@ -898,6 +1038,30 @@ int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy)
     })
 }
 
+int wb_ctx_set_k_budget(wb_ctx_t ctx, long long bytes)
+{
+    WB_TRY({
+        if (!ctx || bytes < 0) throw Error(WB_INVALID, "null context or negative budget");
+        ((Ctx*)ctx)->k_budget = (size_t)bytes;
+    })
+}
+
+int wb_k_plan(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, int nop, int precision,
+              long long* out3)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !basis || !grid || !out3) throw Error(WB_INVALID, "null argument");
+        if (nop < 1 || nop > 8) throw Error(WB_INVALID, "nop out of range");
+        const bool f32 = precision == WB_PREC_MIXED;
+        const KPlan P = k_plan_budget(*(Basis*)basis, *(Grid*)grid, nop, f32, !f32 || c->g64,
+                                      c->ncopy, c->k_budget);
+        out3[0] = (long long)P.total;
+        out3[1] = P.nchunk;
+        out3[2] = P.csub;
+    })
+}
+
 int wb_ctx_set_g64(wb_ctx_t ctx, int on)
 {
     WB_TRY({ ((Ctx*)ctx)->g64 = on != 0; })
@@ -1068,7 +1232,7 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
             for (int n : {1, 2, 4, 16}) {
                 const size_t gbytes = (size_t)n * (b->nao + MD_SPARE + 1) * g->Ppad * nop *
                                       (g64 ? 8 : 4);
-                if (gbytes > (free_b + c->ws_G.bytes) / 4) continue;
+                if (gbytes > (free_b + c->ws_karena.bytes) / 4) continue;
                 const float t = time_cfg(n, best_mask, 0);
                 if (t < best_t) { best_t = t; best_n = n; }
             }
@@ -1090,8 +1254,7 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
             c->gemm_tiles = best_tiles;
             // the largest configuration tried sized G: let it regrow to the kept one
             WB_HIP(hipStreamSynchronize(c->stream));
-            c->ws_G.release();
-            c->ws_mask.release();
+            c->ws_karena.release();
             if (report) {
                 report[0] = best_n;
                 report[1] = (double)best_mask;
@@ -1281,23 +1444,9 @@ static void job_needs(const Ctx& w, const wb_cycle_job& j, int nop, int prec,
     auto at_least = [](size_t* n, int b, size_t v) { n[b] = std::max(n[b], v); };
     if (j.kgrid) {
         const Grid& G = *(const Grid*)j.kgrid;
-        const size_t P = G.Ppad, nsub = G.nsub;
-        size_t nitem = 0;
-        for (const auto& C : B.cls) nitem += (size_t)C.ngroups * nsub;
-        const size_t zrow = nao + MD_SPARE;
-        const bool g64 = !f32 || w.g64;
-        at_least(need, WS_X, 8 * nao * P);
-        at_least(need, WS_D, 8 * nn);
-        at_least(need, WS_F, (f32 ? 4 : 8) * (nao + MD_SPARE) * P);
-        at_least(need, WS_FMAX, 4 * nsub * B.nshell);
-        at_least(need, WS_COUNTS, 4 * NCLASS * nsub);
+        at_least(need, WS_KARENA, k_plan_budget(B, G, nop, f32, !f32 || w.g64, w.ncopy,
+                                                w.k_budget).total);
         at_least(need, WS_WORK, 8 * 2 * NCLASS);
-        at_least(need, WS_TASKS, 4 * std::max<size_t>(nitem, 1));
-        at_least(need, WS_SKIPS, 4 * std::max<size_t>(nitem, 1));
-        at_least(need, WS_G, (g64 ? 8 : 4) * (zrow + 1) * P * w.ncopy * nop);
-        if (zrow <= 32 * MD_MASK_WORDS)
-            at_least(need, WS_MASK, 4 * (size_t)w.ncopy * nsub * MD_MASK_WORDS * nop);
-        at_least(need, WS_KP, gemm_nt_ws_bytes((int)nao, (int)nao, (int)P, true));
         at_least(need, WS_K, 8 * nn * (1 + nop));
     }
     if (j.nlcgrid) {
@@ -1338,6 +1487,7 @@ static Ctx& batch_lane(Ctx& c, size_t i)
     l.fuse_mask = c.fuse_mask;
     l.gemm_tiles = c.gemm_tiles;
     l.g64 = c.g64;
+    l.k_budget = c.k_budget;
     return l;
 }
 
@@ -1598,6 +1748,7 @@ static Ctx& side_ctx(Ctx& c)
     c.side->ncopy = c.ncopy;
     c.side->fuse_mask = c.fuse_mask;
     c.side->g64 = c.g64;
+    c.side->k_budget = c.k_budget;
     return *c.side;
 }
 
@@ -1679,6 +1830,7 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
         w.fuse_mask = c->fuse_mask;
         w.gemm_tiles = c->gemm_tiles;
         w.g64 = c->g64;
+        w.k_budget = c->k_budget;
         std::vector<float> t32;
         std::vector<double> t64;
         boys_tables(t32, t64);

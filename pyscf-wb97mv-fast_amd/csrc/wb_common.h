@@ -34,22 +34,32 @@ void set_last_error(const std::string& msg);
 inline int class_id(int la, int lb) { return la * (la + 1) / 2 + lb; }   // la >= lb
 inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
-// Device buffer owned by the library.
+// Device buffer owned by the library, or a view of memory owned elsewhere
+// (borrow(): never freed, never grown).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = true;
+    }
+    void borrow(void* q, size_t n) {
+        release();
+        p = q;
+        bytes = n;
+        owned = false;
     }
     // grow-only (re)allocation; contents are not preserved
     void* ensure(size_t n) {
         if (n > bytes) {
+            if (!owned) throw Error(WB_INVALID, "borrowed device buffer too small");
             release();
             if (n) WB_HIP(hipMalloc(&p, n));
             bytes = n;

@@ -122,16 +122,20 @@ struct Grid {
 
 // --------------------------------------------------------------- context ---
 // the context's grow-only workspaces, in a fixed order (Ctx::bufs)
-enum WsBuf { WS_X, WS_XV, WS_F, WS_G, WS_TASKS, WS_COUNTS, WS_FMAX, WS_KP, WS_K, WS_D, WS_T,
-             WS_RHO, WS_VV, WS_B, WS_RED, WS_IO, WS_MASK, WS_SKIPS, WS_WORK, WS_COUNT };
+enum WsBuf { WS_X, WS_XV, WS_KARENA, WS_KP, WS_K, WS_T, WS_RHO, WS_VV, WS_B, WS_RED, WS_IO,
+             WS_WORK, WS_COUNT };
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     DevBuf d_b32, d_b64;               // Boys tables
-    // workspaces (grow-only)
-    DevBuf ws_X, ws_Xv, ws_F, ws_G, ws_tasks, ws_counts, ws_fmax, ws_Kp, ws_K, ws_D;
-    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io, ws_mask, ws_skips;
+    // workspaces (grow-only); ws_karena: every buffer of a K build
+    // (k_plan), in one allocation
+    DevBuf ws_X, ws_Xv, ws_karena, ws_Kp, ws_K;
+    DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io;
+    // device bytes a K build may take for its workspace (0: no limit; the
+    // grid is then processed in one chunk)
+    size_t k_budget = 0;
     // statistics of the last K build
     long long tasks_total = 0, tasks_kept = 0;
     float ms_last = 0.f;
@@ -165,9 +169,8 @@ struct Ctx {
     // (the caller makes sure no work is in flight)
     size_t ws_bytes() const;
     void trim();
-    DevBuf* bufs[WS_COUNT] = {&ws_X, &ws_Xv, &ws_F, &ws_G, &ws_tasks, &ws_counts, &ws_fmax,
-                              &ws_Kp, &ws_K, &ws_D, &ws_T, &ws_rho, &ws_vv, &ws_B,
-                              &ws_red, &ws_io, &ws_mask, &ws_skips, &ws_work};
+    DevBuf* bufs[WS_COUNT] = {&ws_X, &ws_Xv, &ws_karena, &ws_Kp, &ws_K, &ws_T, &ws_rho,
+                              &ws_vv, &ws_B, &ws_red, &ws_io, &ws_work};
 };
 
 enum Phase { PH_AO = 0, PH_FGEMM, PH_SCREEN, PH_MD, PH_KGEMM, PH_NLC_AO, PH_NLC_RHO,
@@ -299,6 +302,19 @@ struct VvLayout { size_t cnt, off, pt, pk, idx, aux, ts, F, U, W, sym, total; };
 VvLayout vv_layout(int nsub, int Ppad, bool f64);
 // workspace bytes gemm_nt_fast takes (partials, plus the folded operand)
 size_t gemm_nt_ws_bytes(int M, int N, int K, bool fold);
+// the point-axis split gemm_nt_fast uses for K points: returns the number of
+// splits, kchunk = points per split (a multiple of 64)
+int nt_split(int K, int& kchunk);
+// whether gemm_nt_core needs the folded operand buffer Bf (N x K doubles)
+bool gemm_nt_folds(bool b_f32, int nb, const double* w, const void* B, const unsigned* masks);
+// gemm_nt_fast on caller-provided buffers with a given split: C = (accumulate
+// ? C : 0) + sum over splits z (in order) of A[:, split z] Bsum[:, split z]^T,
+// Cp = ceil(K / kchunk) x M x N partials, Bf the folded operand (or null when
+// gemm_nt_folds says none is needed).  A build over chunks of whole splits
+// that accumulates them in order gives the one-call result bit for bit.
+void gemm_nt_core(hipStream_t s, int M, int N, int K, int kchunk, const double* A, const void* B,
+                  bool b_f32, int nb, size_t bstride, const double* w, double* C, bool accumulate,
+                  double* Cp, double* Bf, const unsigned* masks, const int* row0, int nsub);
 struct Ctx;
 void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
                const double* origin, const double* rho, double b, double C, double tol, bool f64,

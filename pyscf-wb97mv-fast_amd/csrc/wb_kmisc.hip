@@ -385,12 +385,14 @@ void gemm_nt_kernel(int M, int N, int K, int kchunk, const double* __restrict__ 
             }
 }
 
+// C = (accumulate ? C : 0) + Cp[0] + Cp[1] + ... in split order: a build
+// cut into chunks of whole splits adds the same partials in the same order
 __global__ void split_reduce_kernel(int MN, int nsplit, const double* __restrict__ Cp,
-                                    double* __restrict__ C)
+                                    double* __restrict__ C, int accumulate)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= MN) return;
-    double v = 0.0;
+    double v = accumulate ? C[i] : 0.0;
     for (int z = 0; z < nsplit; ++z) v += Cp[(size_t)z * MN + i];
     C[i] = v;
 }
@@ -413,7 +415,7 @@ void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A, const v
                            (const double*)B, nb, bstride, w, Cp);
     WB_HIP(hipGetLastError());
     hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
-                       nsplit, Cp, C);
+                       nsplit, Cp, C, 0);
     WB_HIP(hipGetLastError());
 }
 
@@ -736,12 +738,14 @@ void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const dou
     WB_HIP(hipGetLastError());
 }
 
-// split of the K GEMM over the point axis: <= 96 chunks of a multiple of T_K
-static int nt_split(int K, int& kchunk)
+// split of the K GEMM over the point axis: <= 96 chunks of a multiple of
+// 64 points (whole sub-blocks, so that a K build cut into chunks of whole
+// splits sums the same partials)
+int nt_split(int K, int& kchunk)
 {
     int nsplit = std::max(1, std::min(96, K / (T_K * 64)));
     kchunk = (K + nsplit - 1) / nsplit;
-    kchunk = (kchunk + T_K - 1) / T_K * T_K;
+    kchunk = (kchunk + SUB - 1) / SUB * SUB;
     return (K + kchunk - 1) / kchunk;
 }
 
@@ -763,19 +767,36 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
         gemm_nt_reduce(s, M, N, K, A, B, b_f32, nb, bstride, w, C, ws);
         return;
     }
+    int kchunk = 0;
+    const int nsplit = nt_split(K, kchunk);
+    const bool fold = gemm_nt_folds(b_f32, nb, w, B, masks);
+    const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
+    char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * (size_t)N * K : 0));
+    gemm_nt_core(s, M, N, K, kchunk, A, B, b_f32, nb, bstride, w, C, false, (double*)base,
+                 fold ? (double*)(base + part_bytes) : nullptr, masks, row0, nsub);
+}
+
+bool gemm_nt_folds(bool b_f32, int nb, const double* w, const void* B, const unsigned* masks)
+{
+    return b_f32 || nb != 1 || w != nullptr || !aligned16(B) || masks;
+}
+
+void gemm_nt_core(hipStream_t s, int M, int N, int K, int kchunk, const double* A, const void* B,
+                  bool b_f32, int nb, size_t bstride, const double* w, double* C, bool accumulate,
+                  double* Cp, double* Bf, const unsigned* masks, const int* row0, int nsub)
+{
+    if (K % 2 || !aligned16(A) || kchunk % T_K)
+        throw Error(WB_INVALID, "split GEMM: K even, A 16-byte aligned, splits of whole slices");
+    if (masks && (K != nsub * SUB || !w || bstride % 4 || !aligned16(B) || !aligned16(w)))
+        throw Error(WB_INVALID, "masked fold: B must be sub-block copies of 16-byte rows");
     // fold the copies / weights into one FP64 operand when needed
     const double* Bd = (const double*)B;
     const size_t nbk = (size_t)N * K;
     const int wm = wave_tiles(M), wn = wave_tiles(N);
     const int mt = (M + 32 * wm - 1) / (32 * wm), ntile = mt * ((N + 32 * wn - 1) / (32 * wn));
-    int kchunk = 0;
-    const int nsplit = nt_split(K, kchunk);
-    const bool fold = b_f32 || nb != 1 || w != nullptr || !aligned16(B) || masks;
-    const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
-    char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * nbk : 0));
-    double* Cp = (double*)base;
-    if (fold) {
-        double* Bf = (double*)(base + part_bytes);
+    const int nsplit = (K + kchunk - 1) / kchunk;
+    if (gemm_nt_folds(b_f32, nb, w, B, masks)) {
+        if (!Bf) throw Error(WB_INVALID, "split GEMM: folded operand buffer missing");
         if (masks) {
             const unsigned blocks = (unsigned)((nbk / 4 + 255) / 256);
             if (b_f32)
@@ -785,17 +806,16 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
                 hipLaunchKernelGGL(fold_masked_kernel<double>, dim3(blocks), dim3(256), 0, s, nbk, K,
                                    (const double*)B, nb, bstride, w, Bf, masks, row0, nsub);
             WB_HIP(hipGetLastError());
-        }
-        const bool vec = K % 4 == 0 && bstride % 4 == 0 && aligned16(B);
-        const unsigned blocks = (unsigned)((vec ? (nbk / 4 + 255) / 256 : (nbk + 255) / 256));
+        } else {
+            const bool vec = K % 4 == 0 && bstride % 4 == 0 && aligned16(B);
+            const unsigned blocks = (unsigned)((vec ? (nbk / 4 + 255) / 256 : (nbk + 255) / 256));
 #define WB_FOLD(T, V) hipLaunchKernelGGL((fold_copies_kernel<T, V>), dim3(blocks), dim3(256), 0, s, \
                                          nbk, K, (const T*)B, nb, bstride, w, Bf)
-        if (!masks) {
             if (b_f32) { if (vec) WB_FOLD(float, true); else WB_FOLD(float, false); }
             else { if (vec) WB_FOLD(double, true); else WB_FOLD(double, false); }
+#undef WB_FOLD
             WB_HIP(hipGetLastError());
         }
-#undef WB_FOLD
         Bd = Bf;
     }
     dim3 grid(ntile * nsplit);
@@ -808,7 +828,7 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
 #undef WB_NT
     WB_HIP(hipGetLastError());
     hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
-                       nsplit, Cp, C);
+                       nsplit, Cp, C, accumulate ? 1 : 0);
     WB_HIP(hipGetLastError());
 }
 

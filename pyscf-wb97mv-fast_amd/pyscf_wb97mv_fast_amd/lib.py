@@ -45,6 +45,8 @@ _SIGS = {
     'wb_ctx_destroy': (_i, [_vp]),
     'wb_ctx_set_copies': (_i, [_vp, _i]),
     'wb_ctx_set_g64': (_i, [_vp, _i]),
+    'wb_ctx_set_k_budget': (_i, [_vp, ctypes.c_longlong]),
+    'wb_k_plan': (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     'wb_sync': (_i, [_vp]),
     'wb_last_ms': (ctypes.c_float, [_vp]),
     'wb_ctx_set_profile': (_i, [_vp, _i]),
@@ -173,6 +175,17 @@ class Context:
     def trim(self):
         """Release the workspaces (they regrow on the next call)."""
         check(load().wb_ctx_trim(self.h))
+
+    def set_k_budget(self, nbytes):
+        """Workspace budget of this context's K builds (0: no limit); a
+        build over budget runs over chunks of the grid, bit for bit the same."""
+        check(load().wb_ctx_set_k_budget(self.h, int(nbytes)))
+
+    def k_plan(self, basis, grid, nop=2, precision='mixed'):
+        """(workspace bytes, chunks, sub-blocks per chunk) of a K build."""
+        out = numpy.zeros(3, dtype=numpy.int64)
+        check(load().wb_k_plan(self.h, basis.h, grid.h, int(nop), PREC[precision], ptr(out)))
+        return tuple(int(v) for v in out)
 
     def tuning(self):
         """(G copies, fused-class mask, GEMM tiles) the context runs with."""

@@ -51,6 +51,7 @@ class Request:
     t_start: float = field(default=None, repr=False)
     t_end: float = field(default=None, repr=False)
     est_bytes: int = field(default=0, repr=False)     # device bytes the job needs from empty
+    k_budget: int = field(default=0, repr=False)      # K workspace budget set to fit (0: none)
     _seq: int = field(default=0, repr=False)          # arrival number
     _shared: bool = field(default=False, repr=False)  # started while other jobs ran
 
@@ -219,11 +220,36 @@ class CycleServer:
         return basis, kg, vg
 
     def _need(self, ctx, req, state):
-        """(bytes the slot holds after the job, bytes the job needs from empty)."""
+        """(bytes the slot holds after the job, bytes the job needs from empty).
+        A job whose K build alone would not fit the budget gets a K workspace
+        budget on its slot (wb_ctx_set_k_budget): its K build then runs over
+        chunks of the grid, with the same result, in as little as the budget
+        leaves; the slot's budget is cleared for the next job."""
         if self._runner is not None:
             held = int(getattr(ctx, 'mem_bytes', 0) or 0)
             return max(held, req.est_bytes), req.est_bytes
         basis, kg, vg = state
+        ctx.set_k_budget(0)
+        after, alone = self._job_bytes(ctx, req, basis, kg, vg)
+        if alone > self.budget and kg is not None:
+            arena = ctx.k_plan(basis, kg, len(self.omegas), self.precision)[0]
+            kb = arena - (alone - self.budget)
+            while kb > 0:
+                ctx.set_k_budget(kb)
+                try:
+                    after, alone = self._job_bytes(ctx, req, basis, kg, vg)
+                except GpuOOM:                   # below one split's workspace
+                    break
+                if alone <= self.budget:
+                    req.k_budget = kb
+                    break
+                kb -= max(alone - self.budget, arena // 64)
+            else:
+                ctx.set_k_budget(0)
+        req.est_bytes = int(alone)
+        return int(after), int(alone)
+
+    def _job_bytes(self, ctx, req, basis, kg, vg):
         job = lib.CycleJob()
         job.basis = basis.h.value
         job.flags = lib.JOB_SEQUENTIAL if req._shared else 0
@@ -233,7 +259,6 @@ class CycleServer:
         lib.check(lib.load().wb_job_bytes(ctx.h, ctypes.byref(job), len(self.omegas),
                                           lib.PREC[self.precision], ctypes.byref(after),
                                           ctypes.byref(alone)))
-        req.est_bytes = int(alone.value)
         return int(after.value), int(alone.value)
 
     def _run(self, ctx, req, state):

@@ -58,7 +58,10 @@ class SyntheticSGX:
         self._pjs_data = PJSData(self.mol, self.grids, direct_scf_tol, Q, self._sym)
 
     def copy(self):
-        return copy.copy(self)
+        """A shallow copy, as PySCF's; the copy counts its own J builds."""
+        twin = copy.copy(self)
+        twin.j_calls = 0
+        return twin
 
     def get_jk(self, dm, hermi=1, vhfopt=None, with_j=True, with_k=True,
                direct_scf_tol=1e-13, omega=None):
