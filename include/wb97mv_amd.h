@@ -145,6 +145,23 @@ int wb_k_build(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* qd,
 int wb_k_build_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_qd,
                    int nop, const double* omegas, double tile_tol, int precision,
                    double* d_k);
+/* wb_k_build_dev without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  No counterpart in the reference.  Same inputs,
+ * outputs and result (bit for bit, with the context's copy count and G
+ * precision); every buffer it needs comes from one fixed 256 MiB device
+ * array per GPU, held from the library's first context on that GPU until
+ * the process ends, and the grid runs over as many chunks as the array
+ * needs (K accumulated in the one-chunk order).  A molecule whose smallest
+ * chunk does not fit is WB_UNSUPPORTED, with the bytes it needs.  The
+ * primitive-pair records of the operator set must already be on the device
+ * (any earlier wb_k_build / wb_k_build_dev / wb_cycle* call with the same
+ * omegas on this basis made them); otherwise WB_INVALID.  All calls on one
+ * GPU share its array: calls that overlap on one GPU race.  wb_k_stats and
+ * wb_k_work keep describing the last allocating K build. */
+int wb_k_build_noalloc(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_qd,
+                       int nop, const double* omegas, double tile_tol, int precision,
+                       double* d_k);
 /* out2 = {tasks considered, tasks kept} of the last K build on grid */
 int wb_k_stats(wb_ctx_t ctx, wb_grid_t grid, long long* out2);
 

@@ -849,6 +849,61 @@ static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop
     if (prof) collect(c);
 }
 
+// ------------------------------------------- allocation-free K build ----
+// Device scratch of wb_k_build_noalloc, a fixed maximum of 256 MiB.  Every
+// GPU the library runs on holds its own copy of this array until the
+// process ends.  In host code the array's name is only a host-side handle:
+// its device address, per device, comes from hipGetSymbolAddress with that
+// device current (resolve_scratch, called once per device when a context is
+// created there), and only that address reaches kernels and copies.  All
+// calls on one GPU share its array: a call clears what must start at zero
+// (its counters, task counts and written-shell masks) itself, and calls that
+// overlap on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+constexpr size_t NOALLOC_WORK_BYTES = 256;          // the per-class counters, first
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[MAX_DEVICES] = {};
+
+static void resolve_scratch(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    WB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(WB_DEVICE_ERROR, "no device address for the scratch array");
+    g_scratch_ptr[device] = (char*)p;
+}
+
+static char* scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES || !g_scratch_ptr[device])
+        throw Error(WB_INVALID, "no scratch on this device: create the context with wb_ctx_create");
+    return g_scratch_ptr[device];
+}
+
+static void k_build_noalloc(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
+                            const double* omegas, double tol, int prec, double* d_K)
+{
+    const bool f32 = prec == WB_PREC_MIXED;
+    const size_t cap = NOALLOC_SCRATCH_BYTES - NOALLOC_WORK_BYTES;
+    KPlan P;
+    try {
+        P = k_plan_budget(B, G, nop, f32, !f32 || c.g64, c.ncopy, cap);
+    } catch (const Error& e) {
+        if (e.code != WB_OOM) throw;
+        throw Error(WB_UNSUPPORTED, std::string("allocation-free K build: ") + e.what() +
+                                        " exceeds the fixed 256 MiB scratch (fewer G copies, "
+                                        "or wb_k_build_dev, which allocates)");
+    }
+    char* base = scratch_of(c.device);
+    k_build_arena(c, B, G, d_QD, nop, omegas, tol, prec, d_K, P, base + NOALLOC_WORK_BYTES,
+                  (unsigned long long*)base, true);
+}
+
 // the K build with the context's workspaces (the arena sized by the
 // context's budget)
 static void k_build_dev(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop,
@@ -937,6 +992,7 @@ int wb_ctx_create(int device, void* stream, wb_ctx_t* out)
         auto* c = new Ctx();
         c->device = device;
         WB_HIP(hipSetDevice(device));
+        resolve_scratch(device);
         if (stream) {
             c->stream = (hipStream_t)stream;
         } else {
@@ -1148,6 +1204,22 @@ int wb_k_build_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double*
             throw Error(WB_INVALID, "unknown precision");
         WB_HIP(hipSetDevice(c->device));
         k_build_dev(*c, *(Basis*)basis, *(Grid*)grid, d_qd, nop, omegas, tile_tol, precision, d_k);
+    })
+}
+
+int wb_k_build_noalloc(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* d_qd,
+                       int nop, const double* omegas, double tile_tol, int precision,
+                       double* d_k)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !basis || !grid || !d_qd || !omegas || !d_k) throw Error(WB_INVALID, "null argument");
+        if (nop < 1 || nop > 8) throw Error(WB_INVALID, "nop out of range");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        WB_HIP(hipSetDevice(c->device));
+        k_build_noalloc(*c, *(Basis*)basis, *(Grid*)grid, d_qd, nop, omegas, tile_tol, precision,
+                        d_k);
     })
 }
 

@@ -60,6 +60,7 @@ _SIGS = {
     'wb_grid_destroy': (_i, [_vp]),
     'wb_k_build': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _i, _vp]),
     'wb_k_build_dev': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _i, _vp]),
+    'wb_k_build_noalloc': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _i, _vp]),
     'wb_k_stats': (_i, [_vp, _vp, _vp]),
     'wb_nlc_vxc': (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
     'wb_nlc_vxc_dev': (_i, [_vp, _vp, _vp, _vp, _d, _d, _d, _i, _vp, _vp]),
