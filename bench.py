@@ -209,43 +209,89 @@ def bench_batch(args):
         'mixed_vs_fp64': t_64 / t_bl}), flush=True)
 
 
+def _serve_jobs(count, max_waters, max_heavy, seed):
+    """count SCF jobs: water clusters (n ~ U{8..max_waters}, def2-SVP) and
+    drug-like chains (U{10..max_heavy} heavy atoms, def2-SVP or def2-TZVP),
+    seeded; level-0 (coarse) and level-1 (fine) grids with the cell
+    partition on the GPU; extended-Hueckel densities."""
+    from pyscf_wb97mv_fast_amd import lib
+    from pyscf_wb97mv_fast_amd.basis import pack_mol
+    from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
+    rng = numpy.random.default_rng(seed)
+    pctx = lib.Context(0)
+    out = []
+    while len(out) < count:
+        i = len(out)
+        try:
+            if rng.random() < 0.5:
+                n = int(rng.integers(8, max_waters + 1))
+                mol = mole.Mole(geometry.water_cluster(n, seed=seed + i), 'def2-svp')
+                name = '(H2O)%d def2-SVP' % n
+            else:
+                n = int(rng.integers(10, max_heavy + 1))
+                basis = 'def2-tzvp' if rng.random() < 0.5 else 'def2-svp'
+                mol = mole.Mole(geometry.drug_like(n, seed=seed + i), basis)
+                name = 'drug-like %d heavy atoms %s' % (n, basis)
+        except RuntimeError:                        # a jammed random walk: draw again
+            continue
+        g0 = grids.Grids(mol, level=0, device=pctx).build()
+        g1 = grids.Grids(mol, level=1, device=pctx).build()
+        out.append(dict(name=name, mol=mol, pack=pack_mol(mol), coarse=(g0.coords, g0.weights),
+                        fine=(g1.coords, g1.weights), dm=guess.huckel_density(mol)))
+    return out
+
+
 def bench_serve(args):
-    """Seeded Poisson arrivals of SCF-cycle jobs (water clusters of
-    U{4..--batch-max} molecules) through the continuous-batching loop."""
-    from pyscf_wb97mv_fast_amd.serving import CycleServer, Request
+    """Seeded Poisson arrivals of SCF jobs run to convergence through the
+    continuous-batching loop: each job level-0 grids (mixed), level-1 grids
+    (mixed), level-1 grids in FP64 until |dE| < 1e-9 Ha and |g| < 1e-6, with
+    the model SCF step; the same schedule in FP64 throughout beside it."""
+    from pyscf_wb97mv_fast_amd.serving import CycleServer, Request, Stage, staged
+    from pyscf_wb97mv_fast_amd.systems import scf
     t = time.perf_counter()
-    mols = _clusters(args.serve, args.batch_max, 23)
-    gaps = numpy.random.default_rng(5).exponential(1.0 / args.rate, len(mols))
-    log('serve workload: %d jobs, setup %.1f s' % (len(mols), time.perf_counter() - t))
+    jobs = _serve_jobs(args.serve, args.serve_waters, args.serve_heavy, 23)
+    gaps = numpy.random.default_rng(5).exponential(1.0 / args.rate, len(jobs))
+    log('serve workload: %d jobs (%s), setup %.1f s' % (
+        len(jobs), ', '.join(j['name'] for j in jobs), time.perf_counter() - t))
     out = {}
-    for precision in ('mixed', 'fp64'):
-        srv = CycleServer(slots=args.lanes, hbm_budget=200e9, precision=precision)
-        # no warm-up: the first jobs on each slot include its workspace allocation
+    for form in ('mixed', 'fp64'):
+        srv = CycleServer(slots=args.lanes, hbm_budget=200e9)
         t0 = time.perf_counter()
         futs = []
-        for gap, m in zip(gaps, mols):
+        for gap, j in zip(gaps, jobs):
             time.sleep(float(gap))
-            futs.append(srv.submit(Request(pack=m[1], kgrid=(m[2], m[3]),
-                                           nlcgrid=(m[2], m[3]), dm=m[4], cycles=args.cycles)))
+            st = staged((j['coarse'], j['coarse']), (j['fine'], j['fine']))
+            if form == 'fp64':
+                st = [Stage(x.kgrid, x.nlcgrid, 'fp64', x.tau_e, x.tau_g, x.max_cycles) for x in st]
+            futs.append(srv.submit(Request(pack=j['pack'], dm=j['dm'], scf=scf.ModelSCF(j['mol']),
+                                           stages=st, conv_tol=1e-9, conv_tol_grad=1e-6,
+                                           max_cycles=100)))
         res = [f.result() for f in futs]
         srv.close()
-        st = srv.stats()
-        st['wall_s'] = time.perf_counter() - t0
-        st['energies'] = [float(r['nlc'][1]) for r in res]
-        log('%s: %d jobs, %.2f jobs/s, p50 %.3f s, p99 %.3f s, at most %d at once'
-            % (precision, st['jobs'], st['jobs_per_s'], st['p50_s'], st['p99_s'],
-               st['max_active']))
-        out[precision] = st
-    dnl = max(abs(a - b) for a, b in zip(out['mixed'].pop('energies'), out['fp64'].pop('energies')))
+        stt = srv.stats()
+        stt['wall_s'] = time.perf_counter() - t0
+        stt['converged'] = sum(bool(r['converged']) for r in res)
+        stt['cycles'] = [r['cycles'] for r in res]
+        stt['energies'] = [r['e_tot'] for r in res]
+        log('%s: %d jobs (%d converged), %.2f jobs/s, time to converged energy p50 %.2f s, '
+            'p99 %.2f s, at most %d at once' % (form, stt['jobs'], stt['converged'],
+                                                stt['jobs_per_s'], stt['p50_s'], stt['p99_s'],
+                                                stt['max_active']))
+        out[form] = stt
+    de = max(abs(a - b) for a, b in zip(out['mixed'].pop('energies'), out['fp64'].pop('energies')))
     print(json.dumps({
-        'metric': 'served SCF-cycle jobs/s and p50/p99 time to result', 'value':
+        'metric': 'served SCF jobs/s and p50/p99 time to converged energy', 'value':
         out['mixed']['jobs_per_s'], 'unit': 'jobs/s', 'n_gpus': 1, 'higher_is_better': True,
         'data': 'synthetic', 'config': {
-            'workload': '%d jobs, Poisson arrivals %.1f/s (seed 5), water clusters n ~ '
-                        'U{4..%d}, def2-SVP, level-1 grids, %d K(1/r)+K(erf)+VV10 cycles each'
-                        % (len(mols), args.rate, args.batch_max, args.cycles),
-            'slots': args.lanes, 'hbm_budget_bytes': 200e9},
-        'mixed': out['mixed'], 'fp64': out['fp64'], 'max_abs_dE_nl_vs_fp64': dnl}), flush=True)
+            'workload': '%d jobs, Poisson arrivals %.1f/s (seed 5): water clusters n ~ U{8..%d} '
+                        'def2-SVP, drug-like chains U{10..%d} heavy atoms def2-SVP/TZVP; '
+                        'level-0 -> level-1 grids (mixed) -> level-1 FP64 tail to |dE| < 1e-9 '
+                        'Ha; model SCF step (extended-Hueckel h + device K and VV10)'
+                        % (len(jobs), args.rate, args.serve_waters, args.serve_heavy),
+            'slots': args.lanes, 'hbm_budget_bytes': 200e9,
+            'jobs': [j['name'] for j in jobs]},
+        'mixed': out['mixed'], 'fp64': out['fp64'],
+        'max_abs_final_dE_mixed_vs_fp64': de}), flush=True)
 
 
 def main():
@@ -275,10 +321,14 @@ def main():
                     help='set-up mode: shell-pair construction and the grid-cell partition '
                          'for (H2O)8/27/64; prints its own JSON line')
     ap.add_argument('--serve', type=int, default=0, metavar='N',
-                    help='serving mode: N jobs with seeded Poisson arrivals (--rate per '
-                         's) through the continuous-batching loop; prints its own JSON line')
+                    help='serving mode: N SCF jobs with seeded Poisson arrivals (--rate per '
+                         's) run to convergence through the continuous-batching loop; prints '
+                         'its own JSON line')
     ap.add_argument('--rate', type=float, default=2.0)
-    ap.add_argument('--cycles', type=int, default=3)
+    ap.add_argument('--serve-waters', type=int, default=32,
+                    help='largest water cluster of the serving mix (8..N)')
+    ap.add_argument('--serve-heavy', type=int, default=20,
+                    help='largest drug-like chain of the serving mix (10..N heavy atoms)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '

@@ -71,8 +71,9 @@ int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy);
  * {1, 2, 4, 8, 16} (those whose G fits a quarter of the free device memory),
  * then, class by class, fused vs two-pass exchange kernels for the classes
  * with a fused two-operator form, then the FP64 GEMMs' MFMA tiles per wave
- * side (3 or 4, against the padding rule's choice), and keeps the fastest
- * on the context.  report (optional, 6): best copies, fuse mask, best ms, ms
+ * side (3 or 4, against the padding rule's choice), then the waves per
+ * sub-block of each class (wb_ctx_set_class_waves; halved while faster),
+ * and keeps the fastest on the context.  report (optional, 6): best copies, fuse mask, best ms, ms
  * of the default (8 copies, all fused, tiles by padding), configurations
  * timed, GEMM tiles (0: by padding).  Results stay within the mixed form's
  * tolerance of the default's (the copies change the FP32 summation order).
@@ -83,6 +84,15 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
                     double* report);
 int wb_ctx_get_tuning(wb_ctx_t ctx, int* ncopy, unsigned* fuse_mask, int* gemm_tiles);
 int wb_ctx_set_tuning(wb_ctx_t ctx, int ncopy, unsigned fuse_mask, int gemm_tiles);
+/* Waves per 64-point sub-block of each (la, lb) class's exchange kernel
+ * (class c = la(la+1)/2 + lb; 10 values): a sub-block's task list is cut
+ * into this many chunks, run by separate waves, each writing its own G copy
+ * (0: the copy count, the default; at most the copy count; the first class
+ * with pairs always uses every copy).  wb_ctx_autotune picks them, class by
+ * class, after the copy count; NULL restores the default.  No counterpart in
+ * the reference (one launch shape per class, gpu/sgx_k.py:764-773). */
+int wb_ctx_get_class_waves(wb_ctx_t ctx, int* waves10);
+int wb_ctx_set_class_waves(wb_ctx_t ctx, const int* waves10);
 /* Workspace budget of the context's K builds in bytes (0, the default: no
  * limit).  A K build whose workspace (AO values, F, the G copies, task
  * lists, the K GEMM's partials) would exceed it runs over chunks of the

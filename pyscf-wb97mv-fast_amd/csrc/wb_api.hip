@@ -432,6 +432,10 @@ static void upload_basis(Basis& B, hipStream_t s)
     for (int i = 0; i < B.nshell; ++i)
         for (int m = 0; m < 2 * B.shells[i].l + 1; ++m) row0[B.shells[i].ao0 + m] = B.shells[i].ao0;
     upload(B.d_row0, row0.data(), sizeof(int) * row0.size(), s);
+    std::vector<int> rowsh(B.nao);
+    for (int i = 0; i < B.nshell; ++i)
+        for (int m = 0; m < 2 * B.shells[i].l + 1; ++m) rowsh[B.shells[i].ao0 + m] = i;
+    upload(B.d_rowsh, rowsh.data(), sizeof(int) * rowsh.size(), s);
     for (auto& C : B.cls) {
         if (!C.npairs) continue;
         upload(C.d_pairs, C.pairs.data(), sizeof(PairRec) * C.npairs, s);
@@ -735,7 +739,6 @@ static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop
     const int zrow = nao + MD_SPARE;
     const bool tracked = zrow <= 32 * MD_MASK_WORDS;
     const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
-    const int* d_sh = B.d_shao.as<int>();
     size_t ntask = 0;
     for (auto& C : B.cls) ntask += (size_t)C.npairs * G.nsub;
     for (int ci = 0; ci < P.nchunk; ++ci) {
@@ -771,7 +774,7 @@ static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop
         mark(PH_FGEMM);
         // screening and task lists
         float* fmax = (float*)(arena + P.fmax);
-        launch_fmax(s, F, f32, gv->d_w.as<double>(), nao, Ppad, ns, d_sh, d_sh + B.nshell,
+        launch_fmax(s, F, f32, gv->d_w.as<double>(), nao, Ppad, ns, B.d_rowsh.as<int>(),
                     B.nshell, fmax);
         int* counts = (int*)(arena + P.counts);
         zero_bytes(s, counts, sizeof(int) * (size_t)NCLASS * ns);
@@ -818,16 +821,19 @@ static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop
             const int* ids_k = ids + cls_off[k];
             const unsigned* skips_k = skips + cls_off[k];
             const int zrows = first ? zrow : 0;
+            // waves per sub-block of this class (the first class launched
+            // initializes every copy's region, so it uses them all)
+            const int nch = (first || c.cls_waves[k] <= 0) ? ncopy : std::min(c.cls_waves[k], ncopy);
             if (rec2[k]) {
                 // both operators in one pass over the tasks
                 MdMasks mk{masks, mopstride, zrow, first};
-                launch_md(s, C, *gv, f32, g64, 2, F, Gb, gstride, opstride, ncopy, cnt, ids_k,
+                launch_md(s, C, *gv, f32, g64, 2, F, Gb, gstride, opstride, nch, cnt, ids_k,
                           skips_k, rec2[k], boys, zrows, mk);
             } else {
                 for (int op = 0; op < nop; ++op) {
                     MdMasks mk{tracked ? masks + mopstride * op : nullptr, mopstride, zrow, first};
                     launch_md(s, C, *gv, f32, g64, 1, F, Gb + gsz * opstride * op, gstride,
-                              opstride, ncopy, cnt, ids_k, skips_k, rec1[k][op], boys, zrows, mk);
+                              opstride, nch, cnt, ids_k, skips_k, rec1[k][op], boys, zrows, mk);
                 }
             }
             first = false;
@@ -1094,6 +1100,28 @@ int wb_ctx_set_copies(wb_ctx_t ctx, int ncopy)
     })
 }
 
+int wb_ctx_get_class_waves(wb_ctx_t ctx, int* waves10)
+{
+    WB_TRY({
+        if (!ctx || !waves10) throw Error(WB_INVALID, "null argument");
+        std::copy(((Ctx*)ctx)->cls_waves, ((Ctx*)ctx)->cls_waves + NCLASS, waves10);
+    })
+}
+
+int wb_ctx_set_class_waves(wb_ctx_t ctx, const int* waves10)
+{
+    WB_TRY({
+        if (!ctx) throw Error(WB_INVALID, "null context");
+        int w[NCLASS] = {0};
+        if (waves10)
+            for (int k = 0; k < NCLASS; ++k) {
+                if (waves10[k] < 0 || waves10[k] > 64) throw Error(WB_INVALID, "waves out of range");
+                w[k] = waves10[k];
+            }
+        std::copy(w, w + NCLASS, ((Ctx*)ctx)->cls_waves);
+    })
+}
+
 int wb_ctx_set_k_budget(wb_ctx_t ctx, long long bytes)
 {
     WB_TRY({
@@ -1270,10 +1298,14 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
         WB_HIP(hipEventCreate(&e0));
         WB_HIP(hipEventCreate(&e1));
         int trials = 0;
+        bool waves_set = false;                // the per-class search below sets them
+        int waves0[NCLASS];
+        std::copy(c->cls_waves, c->cls_waves + NCLASS, waves0);
         auto time_cfg = [&](int ncopy, unsigned mask, int tiles) {
             c->ncopy = ncopy;
             c->fuse_mask = mask;
             c->gemm_tiles = tiles;
+            if (!waves_set) std::fill(c->cls_waves, c->cls_waves + NCLASS, 0);
             double* d = (double*)c->ws_K.ensure(sizeof(double) * nn * (1 + nop));
             WB_HIP(hipMemcpyAsync(d, qd, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
             k_build_dev(*c, *b, *g, d, nop, omegas, tile_tol, precision, d + nn);
@@ -1321,6 +1353,21 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
                 const float t = time_cfg(best_n, best_mask, tl);
                 if (t < best_t) { best_t = t; best_tiles = tl; }
             }
+            // waves per sub-block, class by class (all but the first class
+            // launched, which initializes every copy): halve while faster
+            waves_set = true;
+            std::fill(c->cls_waves, c->cls_waves + NCLASS, 0);
+            bool first_cls = true;
+            for (int k = 0; k < NCLASS; ++k) {
+                if (!b->cls[k].npairs) continue;
+                if (first_cls) { first_cls = false; continue; }
+                for (int wv = best_n / 2; wv >= 1; wv /= 2) {
+                    c->cls_waves[k] = wv;
+                    const float t = time_cfg(best_n, best_mask, best_tiles);
+                    if (t < best_t) best_t = t;
+                    else { c->cls_waves[k] = wv * 2 >= best_n ? 0 : wv * 2; break; }
+                }
+            }
             c->ncopy = best_n;
             c->fuse_mask = best_mask;
             c->gemm_tiles = best_tiles;
@@ -1339,6 +1386,7 @@ int wb_ctx_autotune(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double
             c->ncopy = ncopy0;
             c->fuse_mask = mask0;
             c->gemm_tiles = tiles0;
+            std::copy(waves0, waves0 + NCLASS, c->cls_waves);
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             throw;
@@ -1560,6 +1608,7 @@ static Ctx& batch_lane(Ctx& c, size_t i)
     l.gemm_tiles = c.gemm_tiles;
     l.g64 = c.g64;
     l.k_budget = c.k_budget;
+    std::copy(c.cls_waves, c.cls_waves + NCLASS, l.cls_waves);
     return l;
 }
 
@@ -1821,6 +1870,7 @@ static Ctx& side_ctx(Ctx& c)
     c.side->fuse_mask = c.fuse_mask;
     c.side->g64 = c.g64;
     c.side->k_budget = c.k_budget;
+    std::copy(c.cls_waves, c.cls_waves + NCLASS, c.side->cls_waves);
     return *c.side;
 }
 
@@ -1903,6 +1953,7 @@ int wb_cycle_graph_create(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const
         w.gemm_tiles = c->gemm_tiles;
         w.g64 = c->g64;
         w.k_budget = c->k_budget;
+        std::copy(c->cls_waves, c->cls_waves + NCLASS, w.cls_waves);
         std::vector<float> t32;
         std::vector<double> t64;
         boys_tables(t32, t64);

@@ -104,6 +104,7 @@ struct Basis {
     std::vector<double> exps, coefs;
     DevBuf d_shells, d_exps, d_coefs, d_shao;   // d_shao: ao0[nshell], nao[nshell]
     DevBuf d_row0;                              // int [nao]: first AO of the AO's shell
+    DevBuf d_rowsh;                             // int [nao]: the AO's shell
     ClassData cls[NCLASS];
     double pair_cut = 1e-14;
     long long npairs_total = 0;
@@ -145,6 +146,11 @@ struct Ctx {
     unsigned fuse_mask = ~0u;
     // MFMA tiles per wave side of the K build's GEMMs (0: by padding)
     int gemm_tiles = 0;
+    // waves per sub-block of each (la, lb) class's exchange kernel: the
+    // sub-block's task list is cut into this many chunks, each written to its
+    // own G copy (0: ncopy, the most; at most ncopy; the first class launched
+    // always uses ncopy, since it initializes every copy's region)
+    int cls_waves[NCLASS] = {0};
     bool g64 = false;          // mixed form: G accumulated in FP64 (default FP32)
     // per-class work of the last K build: kept tasks, kept primitive pairs
     DevBuf ws_work;
@@ -210,9 +216,10 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
                   bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
                   const unsigned* masks = nullptr, const int* row0 = nullptr, int nsub = 0);
 
+// fmax[s][sh] = max over sub-block s of w_g sum_{mu in sh} |F[mu][g]|;
+// row_shell[mu] = the shell of AO row mu (shells in row order)
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
-                 int Ppad, int nsub, const int* sh_ao0, const int* sh_nao, int nshell,
-                 float* fmax);
+                 int Ppad, int nsub, const int* row_shell, int nshell, float* fmax);
 // one class's screening inputs and task-list outputs
 struct TaskClass {
     const PairScreen* scr;
@@ -253,8 +260,10 @@ struct MdMasks {
     int zrow = 0;
     bool first = false;
 };
+// nchunk (<= the number of G copies): waves per sub-block, chunk j of a
+// sub-block's task list writing copy j
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
-               const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
+               const void* F, void* G, size_t gstride, size_t opstride, int nchunk,
                const int* counts, const int* ids, const unsigned* skips, const void* recs,
                const void* boys, int zero_rows, const MdMasks& mk);
 

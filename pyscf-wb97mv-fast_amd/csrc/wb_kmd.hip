@@ -412,22 +412,22 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
 
 template <int LA, int LB, int NOP, typename real, typename gacc>
 static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
-                        void* G, size_t gstride, size_t opstride, int ncopy, const int* counts,
+                        void* G, size_t gstride, size_t opstride, int nchunk, const int* counts,
                         const int* ids, const unsigned* skips, const void* recs,
                         const void* boys, int zero_rows, const MdMasks& mk)
 {
-    const int workers = g.nsub * ncopy;
+    const int workers = g.nsub * nchunk;
     const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
     hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES),
                        0, s, g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, opstride,
                        counts, ids, skips, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
-                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, ncopy, zero_rows,
+                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, nchunk, zero_rows,
                        mk.m, mk.opstride, mk.zrow, mk.first ? 1 : 0);
     WB_HIP(hipGetLastError());
 }
 
 void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
-               const void* F, void* G, size_t gstride, size_t opstride, int ncopy,
+               const void* F, void* G, size_t gstride, size_t opstride, int nchunk,
                const int* counts, const int* ids, const unsigned* skips, const void* recs,
                const void* boys, int zero_rows, const MdMasks& mk)
 {
@@ -436,11 +436,11 @@ void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool 
         throw Error(WB_INVALID, "operator count not available for this shell-pair class");
 #define WB_LAUNCH(XA, XB, NOPV)                                                                      \
     if (f32 && !g64)                                                                                 \
-        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, skips, recs, boys, zero_rows, mk);  \
+        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, nchunk, counts, ids, skips, recs, boys, zero_rows, mk);  \
     else if (f32)                                                                                    \
-        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, skips, recs, boys, zero_rows, mk); \
+        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, nchunk, counts, ids, skips, recs, boys, zero_rows, mk); \
     else                                                                                             \
-        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, ncopy, counts, ids, skips, recs, boys, zero_rows, mk);
+        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, nchunk, counts, ids, skips, recs, boys, zero_rows, mk);
 #define WB_CASE(XA, XB)                                                                    \
     if (c.la == XA && c.lb == XB) {                                                        \
         if constexpr (md_fuse(XA, XB)) {                                                   \
@@ -494,38 +494,72 @@ void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
 
 // ------------------------------------------------------------ screening ----
 // fmax[s][sh] = max_{g in s} w_g * sum_{mu in sh} |F[mu][g]|
+// One wave per sub-block, one point per lane.  The AO rows are walked in
+// order (independent coalesced loads, so many are in flight); each finished
+// shell's per-point value goes to an LDS tile of 64 shells x 64 points, and
+// then every lane takes the maximum of one shell's row of the tile: one
+// LDS transpose per 64 shells instead of a wave reduction per shell.
 template <typename real>
 __global__ __launch_bounds__(256)
 void fmax_kernel(const real* __restrict__ F, const double* __restrict__ w, int Ppad,
-                 int nsub, const int* __restrict__ sh_ao0, const int* __restrict__ sh_nao,
-                 int nshell, float* __restrict__ fmax)
+                 int nsub, const int* __restrict__ row_shell, int nao, int nshell,
+                 float* __restrict__ fmax)
 {
+    __shared__ float tile[4][64][65];                  // [wave][shell][point], padded
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int s = blockIdx.x * 4 + wave;
-    if (s >= nsub) return;
+    if (s >= nsub) return;                             // no block-wide barrier below
     const int lane = threadIdx.x & 63;
     const int g = s * SUB + lane;
     const double wg = w[g];
-    for (int sh = 0; sh < nshell; ++sh) {
-        const int a0 = sh_ao0[sh], n = sh_nao[sh];
-        double v = 0.0;
-        for (int m = 0; m < n; ++m) v += fabs((double)F[(size_t)(a0 + m) * Ppad + g]);
-        float f = (float)(v * wg);
-        f = wave_max(f);
-        if (lane == 0) fmax[(size_t)s * nshell + sh] = f * 1.0001f;
+    float (*t)[65] = tile[wave];
+    int sh0 = 0;                                       // first shell of the current tile
+    double acc = 0.0;
+    constexpr int RB = 16;                             // rows loaded ahead, independent
+    for (int r0 = 0; r0 < nao; r0 += RB) {
+      float v[RB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+          v[i] = r0 + i < nao ? fabsf((float)F[(size_t)(r0 + i) * Ppad + g]) : 0.f;
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const int r = r0 + i;
+        if (r >= nao) break;                           // uniform
+        acc += (double)v[i];
+        const int sh = row_shell[r];                   // uniform
+        const bool last = r + 1 == nao || row_shell[r + 1] != sh;
+        if (last) {
+            t[sh - sh0][lane] = (float)(acc * wg);
+            acc = 0.0;
+            if (sh - sh0 == 63 || sh + 1 == nshell) {
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const int nb = sh - sh0 + 1;
+                if (lane < nb) {
+                    float m = 0.f;
+#pragma unroll 8
+                    for (int p = 0; p < 64; ++p) m = fmaxf(m, t[lane][p]);
+                    fmax[(size_t)s * nshell + sh0 + lane] = m * 1.0001f;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                sh0 = sh + 1;
+            }
+        }
+      }
     }
 }
 
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao, int Ppad,
-                 int nsub, const int* sh_ao0, const int* sh_nao, int nshell, float* fmax)
+                 int nsub, const int* row_shell, int nshell, float* fmax)
 {
     const int blocks = (nsub + 3) / 4;
     if (f32)
         hipLaunchKernelGGL(fmax_kernel<float>, dim3(blocks), dim3(256), 0, s,
-                           (const float*)F, w, Ppad, nsub, sh_ao0, sh_nao, nshell, fmax);
+                           (const float*)F, w, Ppad, nsub, row_shell, nao, nshell, fmax);
     else
         hipLaunchKernelGGL(fmax_kernel<double>, dim3(blocks), dim3(256), 0, s,
-                           (const double*)F, w, Ppad, nsub, sh_ao0, sh_nao, nshell, fmax);
+                           (const double*)F, w, Ppad, nsub, row_shell, nao, nshell, fmax);
     WB_HIP(hipGetLastError());
 }
 

@@ -1,10 +1,11 @@
 """Autotuner of the K build's run-time choices (wb_ctx_autotune).
 
 The library times, on the caller's own molecule and grid, the number of G
-partial copies the exchange kernels scatter into (fewer copies: less memory
-and a cheaper fold; more: fewer colliding atomic adds), per (la, lb) class
-with a fused two-operator kernel fused vs two passes, and the FP64 GEMMs'
-MFMA tiles per wave side, and keeps the fastest on the context.  ``Tuner`` caches the choice per size class
+partial copies the exchange kernels write into (fewer copies: less memory
+and a cheaper fold; more: more waves per sub-block), per (la, lb) class
+with a fused two-operator kernel fused vs two passes, the FP64 GEMMs' MFMA
+tiles per wave side, and the waves per sub-block of each (la, lb) class's
+exchange kernel, and keeps the fastest on the context.  ``Tuner`` caches the choice per size class
 (AOs and grid points rounded up to powers of two, operator count,
 precision), so molecules of similar size reuse one tuning run; the cache
 can be saved as JSON and loaded by a later process.  The reference runs
@@ -30,7 +31,25 @@ def autotune(ctx, basis, grid, qd, omegas=(0.0, 0.3), tile_tol=1e-11, precision=
                                          lib.ptr(om), float(tile_tol), lib.PREC[precision],
                                          int(reps), lib.ptr(rep)))
     return {'ncopy': int(rep[0]), 'fuse_mask': int(rep[1]), 'best_ms': float(rep[2]),
-            'default_ms': float(rep[3]), 'configs': int(rep[4]), 'gemm_tiles': int(rep[5])}
+            'default_ms': float(rep[3]), 'configs': int(rep[4]), 'gemm_tiles': int(rep[5]),
+            'class_waves': get_class_waves(ctx)}
+
+
+def get_class_waves(ctx):
+    """Waves per sub-block of each (la, lb) class (0: the copy count)."""
+    w = numpy.zeros(10, dtype=numpy.int32)
+    lib.check(lib.load().wb_ctx_get_class_waves(ctx.h, lib.ptr(w)))
+    return [int(v) for v in w]
+
+
+def set_class_waves(ctx, waves=None):
+    if waves is None:
+        lib.check(lib.load().wb_ctx_set_class_waves(ctx.h, None))
+        return
+    w = numpy.ascontiguousarray(waves, dtype=numpy.int32)
+    if w.shape != (10,):
+        raise ValueError('one value per (la, lb) class: 10')
+    lib.check(lib.load().wb_ctx_set_class_waves(ctx.h, lib.ptr(w)))
 
 
 def get_tuning(ctx):
@@ -72,6 +91,7 @@ class Tuner:
             self.table[k] = e
         else:
             set_tuning(ctx, e['ncopy'], e['fuse_mask'], e.get('gemm_tiles', 0))
+            set_class_waves(ctx, e.get('class_waves'))
         return k, e
 
     def save(self, path):

@@ -80,6 +80,8 @@ _SIGS = {
     'wb_ctx_get_tuning': (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_uint),
                                ctypes.POINTER(_i)]),
     'wb_ctx_set_tuning': (_i, [_vp, _i, ctypes.c_uint, _i]),
+    'wb_ctx_get_class_waves': (_i, [_vp, _vp]),
+    'wb_ctx_set_class_waves': (_i, [_vp, _vp]),
     'wb_test_gemm': (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'wb_test_pair_bounds': (_i, [_i, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     'wb_xc_rho': (_i, [_vp, _vp, _vp, _vp, _i, _vp]),

@@ -1,27 +1,40 @@
-"""Continuous-batching request loop for SCF-cycle jobs on one GPU.
+"""Continuous-batching request loop for SCF jobs on one GPU.
 
-A request is one molecule's job: its basis and grids (host arrays), an FP64
-density, and a number of cycles; each cycle runs the device K build for the
-given operators and the VV10 term, then an optional host ``update(dm, k,
-nlc, vmat) -> dm`` (the SCF step the caller owns; PySCF's in the reference,
-staging/schedule.py:99-452).  Requests arrive at any time; ``slots`` worker
-threads, each with its own library context (HIP stream and workspaces),
-take them in arrival order as soon as a slot is free; a taken job starts
-when the device bytes its slot will hold afterwards (exact, wb_job_bytes)
-plus what the other slots hold fit the HBM budget, and no job that arrived
-earlier is still waiting for memory (so jobs start in arrival order).  Requests that arrive
-while others run are admitted into free slots, their device work overlaps
-with the running jobs', and every job runs exactly the single-molecule code
-on a context of its own, so its results do not depend on the mix.  Each
-job's basis, grids and density stay on the device for all its cycles and
-are released when it ends.  Slots keep their grow-only workspaces between
-jobs; a waiting job makes its own slot and the idle slots release theirs,
-so it starts once the busy slots finish, and a job that needs more than the
-whole budget is refused with GpuOOM.
+A request is one molecule's job: its basis and grids (host arrays) and an
+FP64 density.  It either runs a fixed number of cycles -- each cycle the
+device K build for the given operators and the VV10 term, then an optional
+host ``update(dm, k, nlc, vmat) -> dm`` -- or, given an SCF step ``scf(dm,
+k, nlc, vmat) -> (dm_next, energy, gradient_norm)`` (PySCF's Fock build and
+diagonalization in real use; the reference's staged SCF,
+staging/schedule.py:99-452), runs to convergence through ``stages``: grids
+and precisions in order, coarse to fine, ending with an FP64 tail, the next
+stage taken when the energy change and the orbital gradient fall below the
+stage's thresholds or after its maximum number of cycles (the switching
+rules of staging/schedule.py:289-350, :368-394), with the step's history
+reset at each switch; the job is converged when, in the last stage, the
+energy changes by less than conv_tol and the gradient norm is below
+conv_tol_grad.
 
-Latency is time from submission to the job's last cycle (p50/p99 over the
-finished jobs), beside throughput in jobs/s.  The reference has no serving
-path.
+Requests arrive at any time; ``slots`` worker threads, each with its own
+library context (HIP stream and workspaces), take them in arrival order as
+soon as a slot is free; a taken job starts when the device bytes its slot
+will hold afterwards (exact, wb_job_bytes, the most demanding stage) plus
+what the other slots hold fit the HBM budget, and no job that arrived
+earlier is still waiting for memory (so jobs start in arrival order).  A job
+whose K build alone would not fit gets a K workspace budget on its slot (the
+build runs over grid chunks, same result).  Requests that arrive while
+others run are admitted into free slots, their device work overlaps with the
+running jobs', and every job runs exactly the single-molecule code on a
+context of its own, so its results do not depend on the mix.  Each job's
+basis and every stage's grids stay on the device for the whole job and are
+released when it ends.  Slots keep their grow-only workspaces between jobs;
+a waiting job makes its own slot and the idle slots release theirs, so it
+starts once the busy slots finish, and a job that needs more than the whole
+budget is refused with GpuOOM.
+
+Latency is time from submission to the job's last cycle -- for SCF jobs the
+time to the converged energy -- (p50/p99 over the finished jobs), beside
+throughput in jobs/s.  The reference has no serving path.
 """
 import collections
 import ctypes
@@ -37,6 +50,30 @@ from .errors import GpuOOM
 
 
 @dataclass
+class Stage:
+    """One stage of a staged SCF job: its grids, the precision of the
+    integral and VV10 arithmetic, and when to move on (|dE| < tau_e and
+    gradient norm < tau_g, or max_cycles cycles in the stage)."""
+    kgrid: tuple = None                 # (coords, weights) of the SGX grid
+    nlcgrid: tuple = None               # (coords, weights) of the VV10 grid
+    precision: str = 'mixed'
+    tau_e: float = 1e-6
+    tau_g: float = 1e-3
+    max_cycles: int = 8
+
+
+def staged(coarse, fine, fp64_tail=True, tau=(1e-5, 1e-7), tau_g=(1e-2, 1e-4), max_cycles=8):
+    """The coarse -> fine schedule with an FP64 tail: coarse grids (mixed),
+    fine grids (mixed), fine grids in FP64 to convergence.  coarse and fine
+    are (kgrid, nlcgrid) pairs of (coords, weights)."""
+    out = [Stage(coarse[0], coarse[1], 'mixed', tau[0], tau_g[0], max_cycles),
+           Stage(fine[0], fine[1], 'mixed', tau[1], tau_g[1], max_cycles)]
+    if fp64_tail:
+        out.append(Stage(fine[0], fine[1], 'fp64', 0.0, 0.0, 10 ** 6))
+    return out
+
+
+@dataclass
 class Request:
     pack: object                        # basis.pack_mol(mol)
     kgrid: tuple = None                 # (coords (n, 3), weights (n,)) of the SGX grid
@@ -46,6 +83,13 @@ class Request:
     cycles: int = 1
     update: object = None               # host SCF step: update(dm, k, nlc, vmat) -> dm
     tag: object = None
+    # run to convergence: scf(dm, k, nlc, vmat) -> (dm, energy, gradient norm)
+    # through stages (kgrid / nlcgrid above are then unused)
+    scf: object = None
+    stages: list = None
+    conv_tol: float = 1e-8
+    conv_tol_grad: float = 1e-5
+    max_cycles: int = 100
     # filled by the server
     t_submit: float = field(default=None, repr=False)
     t_start: float = field(default=None, repr=False)
@@ -210,34 +254,60 @@ class CycleServer:
                 fut.set_exception(err)
 
     # -- device work -----------------------------------------------------------
+    def _stages(self, req):
+        if req.scf is not None:
+            if not req.stages:
+                raise ValueError('an SCF job needs its stages')
+            return list(req.stages)
+        return [Stage(req.kgrid, req.nlcgrid, self.precision)]
+
     def _prepare(self, ctx, req):
-        """The job's device state: basis, grids (resident for all its cycles)."""
+        """The job's device state: its basis and every stage's grids (one
+        device grid per distinct grid), resident for the whole job."""
         if self._runner is not None:
             return None
         basis = lib.DeviceBasis(ctx, req.pack)
-        kg = lib.DeviceGrid(ctx, *req.kgrid) if req.kgrid is not None else None
-        vg = lib.DeviceGrid(ctx, *req.nlcgrid) if req.nlcgrid is not None else None
-        return basis, kg, vg
+        made = {}
+
+        def grid(g):
+            if g is None:
+                return None
+            key = (id(g[0]), id(g[1]))
+            if key not in made:
+                made[key] = lib.DeviceGrid(ctx, *g)
+            return made[key]
+        return basis, [(grid(st.kgrid), grid(st.nlcgrid), st.precision)
+                       for st in self._stages(req)]
 
     def _need(self, ctx, req, state):
-        """(bytes the slot holds after the job, bytes the job needs from empty).
-        A job whose K build alone would not fit the budget gets a K workspace
-        budget on its slot (wb_ctx_set_k_budget): its K build then runs over
-        chunks of the grid, with the same result, in as little as the budget
-        leaves; the slot's budget is cleared for the next job."""
+        """(bytes the slot holds after the job, bytes the job needs from empty),
+        over its most demanding stage.  A job whose K build alone would not fit
+        the budget gets a K workspace budget on its slot (wb_ctx_set_k_budget):
+        its K builds then run over chunks of the grid, with the same result, in
+        as little as the budget leaves; the slot's budget is cleared for the
+        next job."""
         if self._runner is not None:
             held = int(getattr(ctx, 'mem_bytes', 0) or 0)
             return max(held, req.est_bytes), req.est_bytes
-        basis, kg, vg = state
+        basis, stages = state
+
+        def worst():
+            after, alone, big = 0, -1, None
+            for kg, vg, prec in stages:
+                a, b = self._job_bytes(ctx, req, basis, kg, vg, prec)
+                after = max(after, a)
+                if b > alone:
+                    alone, big = b, (kg, prec)
+            return after, alone, big
         ctx.set_k_budget(0)
-        after, alone = self._job_bytes(ctx, req, basis, kg, vg)
-        if alone > self.budget and kg is not None:
-            arena = ctx.k_plan(basis, kg, len(self.omegas), self.precision)[0]
+        after, alone, big = worst()
+        if alone > self.budget and big is not None and big[0] is not None:
+            arena = ctx.k_plan(basis, big[0], len(self.omegas), big[1])[0]
             kb = arena - (alone - self.budget)
             while kb > 0:
                 ctx.set_k_budget(kb)
                 try:
-                    after, alone = self._job_bytes(ctx, req, basis, kg, vg)
+                    after, alone, _ = worst()
                 except GpuOOM:                   # below one split's workspace
                     break
                 if alone <= self.budget:
@@ -249,7 +319,7 @@ class CycleServer:
         req.est_bytes = int(alone)
         return int(after), int(alone)
 
-    def _job_bytes(self, ctx, req, basis, kg, vg):
+    def _job_bytes(self, ctx, req, basis, kg, vg, precision):
         job = lib.CycleJob()
         job.basis = basis.h.value
         job.flags = lib.JOB_SEQUENTIAL if req._shared else 0
@@ -257,7 +327,7 @@ class CycleServer:
         job.nlcgrid = vg.h.value if vg is not None else None
         after, alone = ctypes.c_longlong(), ctypes.c_longlong()
         lib.check(lib.load().wb_job_bytes(ctx.h, ctypes.byref(job), len(self.omegas),
-                                          lib.PREC[self.precision], ctypes.byref(after),
+                                          lib.PREC[precision], ctypes.byref(after),
                                           ctypes.byref(alone)))
         return int(after.value), int(alone.value)
 
@@ -266,34 +336,70 @@ class CycleServer:
             return self._runner(ctx, req)
         return self._run_device(ctx, req, state)
 
-    def _run_device(self, ctx, req, state=None):
-        L = lib.load()
-        basis, kg, vg = state if state is not None else self._prepare(ctx, req)
+    def _cycle(self, ctx, req, basis, kg, vg, precision, dm):
+        """One cycle's device work (K build and VV10 concurrent on the device,
+        wb_cycle, when the job started alone; one after the other when it
+        shares the device): (k, nlc, vmat)."""
         nao = basis.nao
         om = lib.f64(self.omegas)
+        job = lib.CycleJob()
+        job.basis = basis.h.value
+        job.flags = lib.JOB_SEQUENTIAL if req._shared else 0
+        k = nlc = vmat = None
+        if kg is not None:
+            qd = lib.f64(dm if req.q is None else req.q @ dm)
+            k = numpy.empty((len(om), nao, nao))
+            job.kgrid, job.qd, job.k = kg.h.value, qd.ctypes.data, k.ctypes.data
+        if vg is not None:
+            nlc, vmat = numpy.empty(2), numpy.empty((nao, nao))
+            job.nlcgrid, job.dm = vg.h.value, dm.ctypes.data
+            job.nlc, job.vmat = nlc.ctypes.data, vmat.ctypes.data
+        lib.check(lib.load().wb_cycle(ctx.h, ctypes.byref(job), len(om), lib.ptr(om),
+                                      self.k_tol, self.nlc[0], self.nlc[1], self.vv_tol,
+                                      lib.PREC[precision]))
+        return k, nlc, vmat
+
+    def _run_device(self, ctx, req, state=None):
+        basis, stages = state if state is not None else self._prepare(ctx, req)
         dm = lib.f64(req.dm)
-        out = {}
-        for _ in range(max(1, int(req.cycles))):
-            # K build and VV10 concurrent on the device (wb_cycle) when the job
-            # started alone, one after the other when it shares the device
-            job = lib.CycleJob()
-            job.basis = basis.h.value
-            job.flags = lib.JOB_SEQUENTIAL if req._shared else 0
-            if kg is not None:
-                qd = lib.f64(dm if req.q is None else req.q @ dm)
-                k = numpy.empty((len(om), nao, nao))
-                job.kgrid, job.qd, job.k = kg.h.value, qd.ctypes.data, k.ctypes.data
-            if vg is not None:
-                o2, vmat = numpy.empty(2), numpy.empty((nao, nao))
-                job.nlcgrid, job.dm = vg.h.value, dm.ctypes.data
-                job.nlc, job.vmat = o2.ctypes.data, vmat.ctypes.data
-            lib.check(L.wb_cycle(ctx.h, ctypes.byref(job), len(om), lib.ptr(om), self.k_tol,
-                                 self.nlc[0], self.nlc[1], self.vv_tol, lib.PREC[self.precision]))
-            if kg is not None:
-                out['k'] = k
-            if vg is not None:
-                out['nlc'], out['vmat'] = o2, vmat
-            if req.update is not None:
-                dm = lib.f64(req.update(dm, out.get('k'), out.get('nlc'), out.get('vmat')))
-        out['dm'] = dm
-        return out
+        if req.scf is None:
+            kg, vg, prec = stages[0]
+            out = {}
+            for _ in range(max(1, int(req.cycles))):
+                k, nlc, vmat = self._cycle(ctx, req, basis, kg, vg, prec, dm)
+                if k is not None:
+                    out['k'] = k
+                if nlc is not None:
+                    out['nlc'], out['vmat'] = nlc, vmat
+                if req.update is not None:
+                    dm = lib.f64(req.update(dm, out.get('k'), out.get('nlc'), out.get('vmat')))
+            out['dm'] = dm
+            return out
+        # staged SCF to convergence
+        specs = self._stages(req)
+        si, in_stage, e_last = 0, 0, None
+        history = []
+        converged = False
+        e = None
+        for cycle in range(int(req.max_cycles)):
+            kg, vg, prec = stages[si]
+            k, nlc, vmat = self._cycle(ctx, req, basis, kg, vg, prec, dm)
+            dm_next, e, gnorm = req.scf(dm, k, nlc, vmat)
+            de = float('inf') if e_last is None else abs(e - e_last)
+            history.append({'cycle': cycle, 'stage': si, 'precision': prec, 'e': float(e),
+                            'de': de, 'gnorm': float(gnorm), 't': time.perf_counter()})
+            e_last = e
+            in_stage += 1
+            last = si == len(stages) - 1
+            if last and de < req.conv_tol and gnorm < req.conv_tol_grad:
+                converged = True
+                break
+            st = specs[si]
+            if not last and ((de < st.tau_e and gnorm < st.tau_g) or in_stage >= st.max_cycles):
+                si, in_stage, e_last = si + 1, 0, None       # energies of another grid differ
+                reset = getattr(req.scf, 'reset', None)
+                if reset is not None:
+                    reset()
+            dm = lib.f64(dm_next)
+        return {'e_tot': float(e), 'dm': dm, 'converged': converged, 'cycles': len(history),
+                'history': history, 'k': k, 'nlc': nlc, 'vmat': vmat}

@@ -124,13 +124,22 @@ def _shell_energies(mol):
     return numpy.asarray(diag) / 27.211386
 
 
-def huckel_density(mol, k=1.75):
+def huckel_hamiltonian(mol, k=1.75):
+    """(H, S): the extended-Hueckel Hamiltonian and the analytic overlap."""
     S = overlap_matrix(mol)
     h = _shell_energies(mol)
     H = k * S * (h[:, None] + h[None, :]) * 0.5
     numpy.fill_diagonal(H, h)
+    return H, S
+
+
+def huckel_density(mol, k=1.75):
+    H, S = huckel_hamiltonian(mol, k)
+    return huckel_density_from(H, S, mol.nelectron // 2)
+
+
+def huckel_density_from(H, S, nocc):
     e, C = scipy.linalg.eigh(H, S)
-    nocc = mol.nelectron // 2
     Cocc = C[:, :nocc]
     D = 2.0 * Cocc @ Cocc.T
     return 0.5 * (D + D.T)

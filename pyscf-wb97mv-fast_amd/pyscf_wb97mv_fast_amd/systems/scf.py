@@ -1,0 +1,117 @@
+"""A model SCF step for running jobs to convergence without PySCF.
+
+In real use the SCF driver is PySCF's (the reference's staged SCF calls
+get_veff, then diagonalizes with DIIS, staging/schedule.py:99-452); the
+device supplies K and the VV10 term.  Offline, ``ModelSCF`` stands in for
+that driver with a closed-shell energy functional built from the device
+terms and the extended-Hueckel one-electron Hamiltonian h of the molecule
+(systems.guess):
+
+    E(D) = tr(D h) + x/4 tr(D K_x[D]) + E_nl[D],
+    K_x = hyb K(1/r) + (alpha - hyb) K(erf(w r)/r)      (omega-B97M-V's mix)
+    F = h + x/2 K_x + V_nl,
+
+K symmetrized as get_k_only returns it, x = x_scale.  A real functional
+balances exchange with a Coulomb term the model does not have; the full
+exchange (x = -1) alone makes E concave in D on real grids and the
+iteration oscillates between near-degenerate orbitals (measured on water
+clusters and a def2-TZVP chain at level-1 grids; x = -0.5 still fails on
+the chain), so the default is x = -0.1, which converges every test
+molecule in 8-24 cycles.  Each call takes this cycle's device results,
+returns the next density (Roothaan step on the DIIS-extrapolated Fock
+matrix in the canonically orthogonalized basis, virtual orbitals optionally
+level-shifted, the lowest nelec/2 orbitals doubly occupied), the energy of
+the density that went in, and the norm of the orbital gradient
+F D S - S D F.  It is a model: it has no Coulomb, nuclear or semilocal XC
+terms, and its energies are not omega-B97M-V energies; what it exercises is
+the device path inside an SCF loop that runs to a fixed point.
+"""
+import numpy
+import scipy.linalg
+
+from . import guess
+
+
+class DIIS:
+    """Pulay extrapolation of Fock matrices with error vectors e = FDS - SDF
+    (orthogonal basis), at most ``space`` of them."""
+
+    def __init__(self, space=8):
+        self.space = space
+        self.reset()
+
+    def reset(self):
+        self.fs, self.es = [], []
+
+    def update(self, f, e):
+        self.fs.append(f)
+        self.es.append(e)
+        if len(self.fs) > self.space:
+            self.fs.pop(0)
+            self.es.pop(0)
+        n = len(self.fs)
+        if n < 2:
+            return f
+        B = numpy.empty((n + 1, n + 1))
+        B[-1, :] = B[:, -1] = -1.0
+        B[-1, -1] = 0.0
+        for i in range(n):
+            for j in range(i + 1):
+                B[i, j] = B[j, i] = numpy.vdot(self.es[i], self.es[j])
+        rhs = numpy.zeros(n + 1)
+        rhs[-1] = -1.0
+        try:
+            c = numpy.linalg.solve(B, rhs)[:n]
+        except numpy.linalg.LinAlgError:
+            return f
+        return sum(ci * fi for ci, fi in zip(c, self.fs))
+
+
+class ModelSCF:
+    def __init__(self, mol, rsh=(0.3, 1.0, 0.15), diis_space=8, lindep=1e-8, shift=0.0,
+                 x_scale=-0.1):
+        self.shift = float(shift)
+        self.x = float(x_scale)
+        self.h, self.S = guess.huckel_hamiltonian(mol)
+        self.omega, self.alpha, self.hyb = rsh
+        self.nocc = mol.nelectron // 2
+        e, U = scipy.linalg.eigh(self.S)
+        keep = e > lindep
+        self.X = U[:, keep] / numpy.sqrt(e[keep])          # X^T S X = 1
+        self.diis = DIIS(diis_space)
+
+    def guess(self):
+        return guess.huckel_density_from(self.h, self.S, self.nocc)
+
+    def fock(self, dm, k, vmat):
+        kk = 0.5 * (k + k.transpose(0, 2, 1))
+        kx = self.hyb * kk[0] + (self.alpha - self.hyb) * kk[1]
+        return self.h + 0.5 * self.x * kx + (vmat if vmat is not None else 0.0), kx
+
+    def energy(self, dm, kx, nlc):
+        e = (float(numpy.einsum('ij,ji->', dm, self.h))
+             + 0.25 * self.x * float(numpy.einsum('ij,ji->', dm, kx)))
+        return e + (float(nlc[1]) if nlc is not None else 0.0)
+
+    def reset(self):
+        """Forget the extrapolation history (at a change of grid or precision)."""
+        self.diis.reset()
+
+    def __call__(self, dm, k, nlc, vmat):
+        F, kx = self.fock(dm, k, vmat)
+        e_tot = self.energy(dm, kx, nlc)
+        X = self.X
+        fds = F @ dm @ self.S
+        g = X.T @ (fds - fds.T) @ X                     # S D F = (F D S)^T
+        gnorm = float(numpy.linalg.norm(g))
+        Fo = self.diis.update(X.T @ F @ X, g)
+        if self.shift:
+            # level shift: + shift (1 - P), P the occupied projector in this basis
+            P = 0.5 * X.T @ self.S @ dm @ self.S @ X
+            Fo = Fo + self.shift * (numpy.eye(len(P)) - P)
+        # only the occupied orbitals are needed
+        w, C = scipy.linalg.eigh(Fo, subset_by_index=[0, self.nocc - 1], overwrite_a=True,
+                                 check_finite=False)
+        Cocc = X @ C
+        dm_new = 2.0 * Cocc @ Cocc.T
+        return 0.5 * (dm_new + dm_new.T), e_tot, gnorm

@@ -1,7 +1,9 @@
 """The K-build autotuner (wb_ctx_autotune, gpu/autotune.py): it keeps the
 fastest configuration it timed (never slower than the default it also
 timed), the tuned K stays within the mixed form's error of the FP64 K, and
-the per-size-class cache reapplies a tuning without timing again."""
+the per-size-class cache reapplies a tuning without timing again.  The
+waves per sub-block of each (la, lb) class change only the FP64 summation
+order of the G copies."""
 import numpy
 import pytest
 
@@ -47,8 +49,10 @@ def test_autotune_keeps_fastest_and_accuracy(case):
     err_tuned = numpy.abs(k_tuned - k64).max()
     print('max|dK| default %.3e tuned %.3e' % (err_def, err_tuned), flush=True)
     assert err_tuned <= 2 * err_def + 1e-10
+    assert len(rep['class_waves']) == 10 and rep['class_waves'] == autotune.get_class_waves(ctx)
     # restoring the default configuration restores the default K bit for bit
     autotune.set_tuning(ctx, 8, 0xffffffff, 0)
+    autotune.set_class_waves(ctx, None)
     assert numpy.array_equal(_k(ctx, b, gr, dm), k_def)
     # either GEMM tile shape gives K within rounding of the default
     for tiles in (3, 4):
@@ -77,3 +81,22 @@ def test_tuner_cache_reuses_the_size_class(case, tmp_path):
     t.save(str(path))
     t2 = autotune.Tuner().load(str(path))
     assert t2.table == t.table
+
+
+def test_class_waves_change_only_the_summation_order(case):
+    pk, g, dm = case
+    ctx = lib.Context(0)
+    b = lib.DeviceBasis(ctx, pk)
+    gr = lib.DeviceGrid(ctx, g.coords, g.weights)
+    ref = _k(ctx, b, gr, dm, 'fp64')
+    for waves in ([1] * 10, [0, 2, 4, 1, 8, 3, 1, 2, 1, 64]):
+        autotune.set_class_waves(ctx, waves)
+        assert autotune.get_class_waves(ctx) == waves
+        k = _k(ctx, b, gr, dm, 'fp64')
+        print(waves, 'max|dK| %.2e' % numpy.abs(k - ref).max(), flush=True)
+        assert numpy.abs(k - ref).max() <= 1e-12 * numpy.abs(ref).max()
+    autotune.set_class_waves(ctx, None)
+    assert autotune.get_class_waves(ctx) == [0] * 10
+    assert numpy.array_equal(_k(ctx, b, gr, dm, 'fp64'), ref)
+    with pytest.raises(GpuError):
+        autotune.set_class_waves(ctx, [65] + [0] * 9)
