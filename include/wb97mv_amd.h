@@ -288,6 +288,26 @@ int wb_cycle_batch(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
                    const double* omegas, double k_tol, double nlc_b, double nlc_C,
                    double vv_tol, int precision, int lanes, double* job_ms);
 
+/* Cycles of many molecules with the exchange kernels of all their K builds
+ * in one launch per shell-pair class: each molecule's set-up (AO values, F,
+ * screening, task lists) runs into a workspace block of its own, then one
+ * persistent launch per (la, lb) class drains a device work queue of the
+ * (molecule, sub-block, chunk) items of all of them, then each molecule's K
+ * GEMMs run.  The VV10 terms run meanwhile on the context's second stream.
+ * Molecules are taken largest first in groups whose workspaces fit the
+ * context's K budget (wb_ctx_set_k_budget; 64 GiB when none is set) and
+ * the device memory free at the call; one that fits no group runs its own
+ * K build, tiled to those limits.  The K arena is sized once per call for
+ * all its groups and kept (wb_ctx_trim releases it).  Each job's outputs equal
+ * wb_k_build's and wb_nlc_vxc's for it alone, bit for bit.  job_ms
+ * (optional, njob): each job's completion time in ms from the start of the
+ * call.  The reference has no batch path (staging/schedule.py:99-452 runs
+ * one molecule per SCF); a job is one get_k_only call per operator
+ * (gpu/sgx_k.py:663-751) and one nr_nlc_vxc call (gpu/vv10.py:268-342). */
+int wb_cycle_queue(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
+                   const double* omegas, double k_tol, double nlc_b, double nlc_C,
+                   double vv_tol, int precision, double* job_ms);
+
 /* VV10 kernel sums on given points (coords n x 3): F = -1.5 sum T, U, W as
  * libdft's VXC_vv10nlc returns them (gpu/vv10.py:167-240). */
 int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, const double* W0,
@@ -316,6 +336,16 @@ int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
  * reference reads the finished projector, sgx._pjs_data.
  * _overlap_correction_matrix, gpu/sgx_k.py:698-704). */
 int wb_grid_overlap(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, double* out);
+
+/* The overlap-fitted projector of COSX (PJS): Q = S_g^{-1} S_an (nao x nao,
+ * host arrays), S_g the grid overlap (as wb_grid_overlap) and S_an the
+ * analytic overlap (PySCF's int1e_ovlp), by LU with partial pivoting on the
+ * device (rocSOLVER), as PySCF builds sgx._pjs_data._overlap_correction_matrix
+ * with scipy.linalg.solve(S_g, S) on the host (read by the reference at
+ * gpu/sgx_k.py:698-704).  A singular S_g (the grid cannot represent the
+ * basis) is WB_UNSUPPORTED. */
+int wb_pjs_projector(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* S_an,
+                     double* Q);
 
 /* Boys function F_0..F_nmax (nmax <= 6) at n values T (host arrays), out
  * (nmax + 1, n), by the device routines the exchange kernels use: FP32

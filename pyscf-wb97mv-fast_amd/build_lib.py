@@ -47,7 +47,9 @@ def build(verbose=False, out=OUT, defines=()):
     with concurrent.futures.ThreadPoolExecutor(len(UNITS)) as ex:
         objs = list(ex.map(compile_unit, UNITS))
     tmp = out + '.tmp'
-    subprocess.check_call([HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + objs)
+    subprocess.check_call([HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + objs +
+                          ['-L/opt/rocm/lib', '-lrocsolver', '-lrocblas',
+                           '-Wl,-rpath,/opt/rocm/lib'])
     os.replace(tmp, out)
     return out
 

@@ -14,6 +14,8 @@
 #include <cstring>
 #include <numeric>
 #include "wb_boys.h"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include "wb_internal.h"
 #include <memory>
 #include <atomic>
@@ -432,10 +434,7 @@ static void upload_basis(Basis& B, hipStream_t s)
     for (int i = 0; i < B.nshell; ++i)
         for (int m = 0; m < 2 * B.shells[i].l + 1; ++m) row0[B.shells[i].ao0 + m] = B.shells[i].ao0;
     upload(B.d_row0, row0.data(), sizeof(int) * row0.size(), s);
-    std::vector<int> rowsh(B.nao);
-    for (int i = 0; i < B.nshell; ++i)
-        for (int m = 0; m < 2 * B.shells[i].l + 1; ++m) rowsh[B.shells[i].ao0 + m] = i;
-    upload(B.d_rowsh, rowsh.data(), sizeof(int) * rowsh.size(), s);
+
     for (auto& C : B.cls) {
         if (!C.npairs) continue;
         upload(C.d_pairs, C.pairs.data(), sizeof(PairRec) * C.npairs, s);
@@ -697,6 +696,206 @@ static const void* op_records(ClassData& C, const std::vector<double>& omegas, b
                                                                        (int)omegas.size(), f32);
 }
 
+// operators: 1/r (omega = 0) and erf(omega r)/r (omega > 0); the
+// short-range erfc operator of PySCF's omega < 0 is outside this path
+static void k_check_omegas(int nop, const double* omegas)
+{
+    for (int op = 0; op < nop; ++op)
+        if (!(omegas[op] >= 0.0 && omegas[op] < 1e30))
+            throw Error(WB_UNSUPPORTED, "omega must be 0 (1/r) or positive (erf); the "
+                                        "short-range erfc operator (omega < 0) is not supported");
+}
+
+// primitive-pair records of every class: both operators in one record set
+// where the class runs them fused, else one set per operator (building them
+// may allocate and synchronize; noalloc: looked up only)
+struct KRecs {
+    const void* rec1[NCLASS][8] = {};
+    const void* rec2[NCLASS] = {};
+};
+static void k_records(Ctx& c, Basis& B, int nop, const double* omegas, bool f32, bool noalloc,
+                      KRecs& R)
+{
+    hipStream_t s = c.stream;
+    for (int k = 0; k < NCLASS; ++k) {
+        ClassData& C = B.cls[k];
+        if (!C.npairs) continue;
+        if (nop == 2 && md_fuse(C.la, C.lb) && (c.fuse_mask >> k & 1u))
+            R.rec2[k] = noalloc ? op_records_find(C, omegas, 2, f32)
+                                : op_records(C, {omegas[0], omegas[1]}, f32, s, true);
+        else
+            for (int op = 0; op < nop; ++op)
+                R.rec1[k][op] = noalloc ? op_records_find(C, omegas + op, 1, f32)
+                                        : op_records(C, {omegas[op]}, f32, s, true);
+    }
+}
+
+// One chunk of one molecule's K build between its phases: its points (the
+// grid itself, or views into the arena), AO values X, F, task lists, G
+// copies and written-shell masks
+struct KChunk {
+    Grid Gc;
+    const Grid* gv = nullptr;
+    int ns = 0, Ppad = 0, zrow = 0;
+    bool tracked = false, first = true;
+    double* X = nullptr;
+    void* F = nullptr;
+    int* counts = nullptr;
+    int* ids = nullptr;
+    unsigned* skips = nullptr;
+    size_t cls_off[NCLASS] = {};
+    char* Gb = nullptr;
+    unsigned* masks = nullptr;
+    size_t gstride = 0, opstride = 0, gsz = 0, mopstride = 0;
+};
+
+// chunk ci of the build planned by P on arena: its points, X, F = QD^T X,
+// the screening bounds and the task lists (work: the per-class counters,
+// 2 NCLASS x u64; QDT: QD transposed)
+template <typename Mark>
+static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char* arena,
+                          unsigned long long* work, const double* QDT, int ci, bool f32, bool g64,
+                          double tol, KChunk& K, Mark mark)
+{
+    hipStream_t s = c.stream;
+    const int nao = B.nao, ncopy = c.ncopy;
+    const int s0 = ci * P.csub, ns = std::min(P.csub, G.nsub - s0);
+    const int Ppad = ns * SUB;
+    K.ns = ns;
+    K.Ppad = Ppad;
+    K.zrow = nao + MD_SPARE;
+    K.tracked = K.zrow <= 32 * MD_MASK_WORDS;
+    K.first = true;
+    // the chunk's points as a grid of their own (views into the arena)
+    K.gv = &G;
+    if (P.nchunk > 1) {
+        Grid& Gc = K.Gc;
+        Gc.nsub = ns;
+        Gc.Ppad = Ppad;
+        Gc.npts = std::min(Ppad, G.npts - s0 * SUB);
+        Gc.d_xyz.borrow(arena + P.xyz, 8 * 3 * (size_t)Ppad);
+        Gc.d_w.borrow(arena + P.w, 8 * (size_t)Ppad);
+        Gc.d_sub.borrow(arena + P.sub, sizeof(float4) * ns);
+        for (int d = 0; d < 3; ++d)
+            WB_HIP(hipMemcpyAsync(Gc.d_xyz.as<double>() + (size_t)d * Ppad,
+                                  G.d_xyz.as<double>() + (size_t)d * G.Ppad + (size_t)s0 * SUB,
+                                  8 * (size_t)Ppad, hipMemcpyDeviceToDevice, s));
+        WB_HIP(hipMemcpyAsync(Gc.d_w.p, G.d_w.as<double>() + (size_t)s0 * SUB,
+                              8 * (size_t)Ppad, hipMemcpyDeviceToDevice, s));
+        WB_HIP(hipMemcpyAsync(Gc.d_sub.p, G.d_sub.as<float4>() + s0, sizeof(float4) * ns,
+                              hipMemcpyDeviceToDevice, s));
+        K.gv = &Gc;
+    }
+    K.X = (double*)(arena + P.x);
+    launch_ao_eval(s, B, *K.gv, 0, K.X);
+    mark(PH_AO);
+    // F and G carry MD_SPARE spare rows after the AO rows for the padding
+    // positions of the exchange kernels' B blocks
+    K.F = arena + P.f;
+    gemm_nn_fast(s, nao, Ppad, nao, QDT, K.X, K.F, f32);
+    mark(PH_FGEMM);
+    // screening and task lists
+    float* fmax = (float*)(arena + P.fmax);
+    launch_fmax(s, K.F, f32, K.gv->d_w.as<double>(), nao, Ppad, ns, B.d_shao.as<int>(),
+                B.d_shao.as<int>() + B.nshell, B.nshell, fmax);
+    K.counts = (int*)(arena + P.counts);
+    zero_bytes(s, K.counts, sizeof(int) * (size_t)NCLASS * ns);
+    K.ids = (int*)(arena + P.ids);
+    K.skips = (unsigned*)(arena + P.skips);
+    size_t toff = 0;
+    TaskLaunch tl;
+    for (int k = 0; k < NCLASS; ++k) {
+        K.cls_off[k] = toff;
+        const ClassData& C = B.cls[k];
+        if (C.npairs) {
+            TaskClass& t = tl.c[tl.n++];
+            t.scr = C.d_scr.as<PairScreen>();
+            t.groups = C.d_groups.as<PairGroup>();
+            t.pbc = C.d_pbc.as<float4>();
+            t.counts = K.counts + (size_t)k * ns;
+            t.ids = K.ids + toff;
+            t.skips = K.skips + toff;
+            t.work = work + 2 * k;
+            t.ngroups = C.ngroups;
+            t.bb = md_bb(C.la, C.lb);
+        }
+        toff += (size_t)C.ngroups * ns;
+    }
+    launch_tasks(s, tl, *K.gv, fmax, B.nshell, (float)tol);
+    // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),
+    // one zero row (read for a shell's first update in a region)
+    K.gstride = (size_t)(K.zrow + 1) * Ppad;
+    K.opstride = K.gstride * ncopy;                   // elements between operators' G
+    K.gsz = g64 ? 8 : 4;
+    K.Gb = arena + P.g;
+    // G is never cleared: written-shell masks per (copy, sub-block) region
+    // send each shell's first update to the zero row (rows < 2048; larger
+    // bases clear G in the first class launch instead)
+    K.mopstride = (size_t)ncopy * ns * MD_MASK_WORDS;
+    K.masks = K.tracked ? (unsigned*)(arena + P.mask) : nullptr;
+    mark(PH_SCREEN);
+}
+
+// the exchange-kernel shares of class k for the chunk, in the order the
+// single-molecule build launches them: one share for both operators (the
+// return value: the operator count of each share, 2) or one per operator
+// (1); none without pairs.  Marks the chunk's first class.
+static int k_class_jobs(const Ctx& c, const Basis& B, KChunk& K, const KRecs& R, int k, int nop,
+                        MdJob* out, int& njob)
+{
+    njob = 0;
+    const ClassData& C = B.cls[k];
+    if (!C.npairs) return 1;
+    const int ns = K.ns, ncopy = c.ncopy;
+    const int* cnt = K.counts + (size_t)k * ns;
+    const int* ids_k = K.ids + K.cls_off[k];
+    const unsigned* skips_k = K.skips + K.cls_off[k];
+    const int zrows = K.first ? K.zrow : 0;
+    // waves per sub-block of this class (the first class launched
+    // initializes every copy's region, so it uses them all)
+    const int nch = (K.first || c.cls_waves[k] <= 0) ? ncopy : std::min(c.cls_waves[k], ncopy);
+    int nopv = 1;
+    if (R.rec2[k]) {
+        // both operators in one pass over the tasks
+        MdMasks mk{K.masks, K.mopstride, K.zrow, K.first};
+        out[njob++] = md_job(C, *K.gv, K.F, K.Gb, K.gstride, K.opstride, nch, cnt, ids_k, skips_k,
+                             R.rec2[k], zrows, mk);
+        nopv = 2;
+    } else {
+        for (int op = 0; op < nop; ++op) {
+            MdMasks mk{K.tracked ? K.masks + K.mopstride * op : nullptr, K.mopstride, K.zrow,
+                       K.first};
+            out[njob++] = md_job(C, *K.gv, K.F, K.Gb + K.gsz * K.opstride * op, K.gstride,
+                                 K.opstride, nch, cnt, ids_k, skips_k, R.rec1[k][op], zrows, mk);
+        }
+    }
+    K.first = false;
+    return nopv;
+}
+
+// after the class launches: G of a chunk without any shell pair cleared,
+// then K (+)= X^T (w o G) per operator
+template <typename Mark>
+static void k_chunk_finish(Ctx& c, Basis& B, const KPlan& P, char* arena, const KChunk& K,
+                           int nop, bool g64, double* d_K, bool accumulate, Mark mark)
+{
+    hipStream_t s = c.stream;
+    const int nao = B.nao;
+    const size_t nn = (size_t)nao * nao;
+    if (K.first) {                                    // no shell pair at all
+        if (K.tracked) zero_bytes(s, K.masks, sizeof(unsigned) * K.mopstride * nop);
+        else zero_bytes(s, K.Gb, K.gsz * K.opstride * nop);
+    }
+    mark(PH_MD);
+    for (int op = 0; op < nop; ++op) {
+        gemm_nt_core(s, nao, nao, K.Ppad, P.kchunk, K.X, K.Gb + K.gsz * K.opstride * op, !g64,
+                     c.ncopy, K.gstride, K.gv->d_w.as<double>(), d_K + (size_t)op * nn,
+                     accumulate, (double*)(arena + P.cp), (double*)(arena + P.fold),
+                     K.tracked ? K.masks + K.mopstride * op : nullptr, B.d_row0.as<int>(), K.ns);
+        mark(PH_KGEMM);
+    }
+}
+
 // The K build on the arena (P.total bytes at arena; work: the per-class
 // counters, 2 NCLASS x u64).  noalloc: no allocating call at all (records
 // looked up, no profiling events).
@@ -704,152 +903,33 @@ static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop
                           const double* omegas, double tol, int prec, double* d_K,
                           const KPlan& P, char* arena, unsigned long long* work, bool noalloc)
 {
-    // operators: 1/r (omega = 0) and erf(omega r)/r (omega > 0); the
-    // short-range erfc operator of PySCF's omega < 0 is outside this path
-    for (int op = 0; op < nop; ++op)
-        if (!(omegas[op] >= 0.0 && omegas[op] < 1e30))
-            throw Error(WB_UNSUPPORTED, "omega must be 0 (1/r) or positive (erf); the "
-                                        "short-range erfc operator (omega < 0) is not supported");
+    k_check_omegas(nop, omegas);
     const bool f32 = (prec == WB_PREC_MIXED);
     const bool g64 = !f32 || c.g64;
-    const int nao = B.nao, ncopy = c.ncopy;
-    const size_t nn = (size_t)nao * nao;
     hipStream_t s = c.stream;
     const bool prof = c.profile && !noalloc;
     GemmTilesScope tiles(c.gemm_tiles);
     if (prof) c.start();
     auto mark = [&](int ph) { if (prof) c.mark(ph); };
-    // every class's records first (building them may allocate and synchronize)
-    const void* rec1[NCLASS][8] = {};
-    const void* rec2[NCLASS] = {};
-    for (int k = 0; k < NCLASS; ++k) {
-        ClassData& C = B.cls[k];
-        if (!C.npairs) continue;
-        if (nop == 2 && md_fuse(C.la, C.lb) && (c.fuse_mask >> k & 1u))
-            rec2[k] = noalloc ? op_records_find(C, omegas, 2, f32)
-                              : op_records(C, {omegas[0], omegas[1]}, f32, s, true);
-        else
-            for (int op = 0; op < nop; ++op)
-                rec1[k][op] = noalloc ? op_records_find(C, omegas + op, 1, f32)
-                                      : op_records(C, {omegas[op]}, f32, s, true);
-    }
+    KRecs R;
+    k_records(c, B, nop, omegas, f32, noalloc, R);     // first: may allocate and synchronize
     double* QDT = (double*)(arena + P.qdt);
-    transpose(s, nao, d_QD, QDT);
+    transpose(s, B.nao, d_QD, QDT);
     zero_bytes(s, work, sizeof(unsigned long long) * 2 * NCLASS);
-    const int zrow = nao + MD_SPARE;
-    const bool tracked = zrow <= 32 * MD_MASK_WORDS;
     const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
     size_t ntask = 0;
     for (auto& C : B.cls) ntask += (size_t)C.npairs * G.nsub;
     for (int ci = 0; ci < P.nchunk; ++ci) {
-        const int s0 = ci * P.csub, ns = std::min(P.csub, G.nsub - s0);
-        const int Ppad = ns * SUB;
-        // the chunk's points as a grid of their own (views into the arena)
-        Grid Gc;
-        const Grid* gv = &G;
-        if (P.nchunk > 1) {
-            Gc.nsub = ns;
-            Gc.Ppad = Ppad;
-            Gc.npts = std::min(Ppad, G.npts - s0 * SUB);
-            Gc.d_xyz.borrow(arena + P.xyz, 8 * 3 * (size_t)Ppad);
-            Gc.d_w.borrow(arena + P.w, 8 * (size_t)Ppad);
-            Gc.d_sub.borrow(arena + P.sub, sizeof(float4) * ns);
-            for (int d = 0; d < 3; ++d)
-                WB_HIP(hipMemcpyAsync(Gc.d_xyz.as<double>() + (size_t)d * Ppad,
-                                      G.d_xyz.as<double>() + (size_t)d * G.Ppad + (size_t)s0 * SUB,
-                                      8 * (size_t)Ppad, hipMemcpyDeviceToDevice, s));
-            WB_HIP(hipMemcpyAsync(Gc.d_w.p, G.d_w.as<double>() + (size_t)s0 * SUB,
-                                  8 * (size_t)Ppad, hipMemcpyDeviceToDevice, s));
-            WB_HIP(hipMemcpyAsync(Gc.d_sub.p, G.d_sub.as<float4>() + s0, sizeof(float4) * ns,
-                                  hipMemcpyDeviceToDevice, s));
-            gv = &Gc;
-        }
-        double* X = (double*)(arena + P.x);
-        launch_ao_eval(s, B, *gv, 0, X);
-        mark(PH_AO);
-        // F and G carry MD_SPARE spare rows after the AO rows for the padding
-        // positions of the exchange kernels' B blocks
-        void* F = arena + P.f;
-        gemm_nn_fast(s, nao, Ppad, nao, QDT, X, F, f32);
-        mark(PH_FGEMM);
-        // screening and task lists
-        float* fmax = (float*)(arena + P.fmax);
-        launch_fmax(s, F, f32, gv->d_w.as<double>(), nao, Ppad, ns, B.d_rowsh.as<int>(),
-                    B.nshell, fmax);
-        int* counts = (int*)(arena + P.counts);
-        zero_bytes(s, counts, sizeof(int) * (size_t)NCLASS * ns);
-        int* ids = (int*)(arena + P.ids);
-        unsigned* skips = (unsigned*)(arena + P.skips);
-        size_t toff = 0;
-        size_t cls_off[NCLASS];
-        TaskLaunch tl;
+        KChunk K;
+        k_chunk_setup(c, B, G, P, arena, work, QDT, ci, f32, g64, tol, K, mark);
         for (int k = 0; k < NCLASS; ++k) {
-            cls_off[k] = toff;
-            const ClassData& C = B.cls[k];
-            if (C.npairs) {
-                TaskClass& t = tl.c[tl.n++];
-                t.scr = C.d_scr.as<PairScreen>();
-                t.groups = C.d_groups.as<PairGroup>();
-                t.pbc = C.d_pbc.as<float4>();
-                t.counts = counts + (size_t)k * ns;
-                t.ids = ids + toff;
-                t.skips = skips + toff;
-                t.work = work + 2 * k;
-                t.ngroups = C.ngroups;
-                t.bb = md_bb(C.la, C.lb);
-            }
-            toff += (size_t)C.ngroups * ns;
+            MdJob js[8];
+            int nj = 0;
+            const int nopv = k_class_jobs(c, B, K, R, k, nop, js, nj);
+            for (int i = 0; i < nj; ++i)
+                launch_md(s, B.cls[k].la, B.cls[k].lb, f32, g64, nopv, js[i], boys);
         }
-        launch_tasks(s, tl, *gv, fmax, B.nshell, (float)tol);
-        // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),
-        // one zero row (read for a shell's first update in a region)
-        const size_t gstride = (size_t)(zrow + 1) * Ppad;
-        const size_t opstride = gstride * ncopy;          // elements between operators' G
-        const size_t gsz = g64 ? 8 : 4;
-        char* Gb = arena + P.g;
-        mark(PH_SCREEN);
-        // G is never cleared: written-shell masks per (copy, sub-block) region
-        // send each shell's first update to the zero row (rows < 2048; larger
-        // bases clear G in the first class launch instead)
-        const size_t mopstride = (size_t)ncopy * ns * MD_MASK_WORDS;
-        unsigned* masks = tracked ? (unsigned*)(arena + P.mask) : nullptr;
-        bool first = true;
-        for (int k = 0; k < NCLASS; ++k) {
-            ClassData& C = B.cls[k];
-            if (!C.npairs) continue;
-            const int* cnt = counts + (size_t)k * ns;
-            const int* ids_k = ids + cls_off[k];
-            const unsigned* skips_k = skips + cls_off[k];
-            const int zrows = first ? zrow : 0;
-            // waves per sub-block of this class (the first class launched
-            // initializes every copy's region, so it uses them all)
-            const int nch = (first || c.cls_waves[k] <= 0) ? ncopy : std::min(c.cls_waves[k], ncopy);
-            if (rec2[k]) {
-                // both operators in one pass over the tasks
-                MdMasks mk{masks, mopstride, zrow, first};
-                launch_md(s, C, *gv, f32, g64, 2, F, Gb, gstride, opstride, nch, cnt, ids_k,
-                          skips_k, rec2[k], boys, zrows, mk);
-            } else {
-                for (int op = 0; op < nop; ++op) {
-                    MdMasks mk{tracked ? masks + mopstride * op : nullptr, mopstride, zrow, first};
-                    launch_md(s, C, *gv, f32, g64, 1, F, Gb + gsz * opstride * op, gstride,
-                              opstride, nch, cnt, ids_k, skips_k, rec1[k][op], boys, zrows, mk);
-                }
-            }
-            first = false;
-        }
-        if (first) {                                    // no shell pair at all
-            if (tracked) zero_bytes(s, masks, sizeof(unsigned) * mopstride * nop);
-            else zero_bytes(s, Gb, gsz * opstride * nop);
-        }
-        mark(PH_MD);
-        for (int op = 0; op < nop; ++op) {
-            gemm_nt_core(s, nao, nao, Ppad, P.kchunk, X, Gb + gsz * opstride * op, !g64, ncopy,
-                         gstride, gv->d_w.as<double>(), d_K + (size_t)op * nn, ci > 0,
-                         (double*)(arena + P.cp), (double*)(arena + P.fold),
-                         tracked ? masks + mopstride * op : nullptr, B.d_row0.as<int>(), ns);
-            mark(PH_KGEMM);
-        }
+        k_chunk_finish(c, B, P, arena, K, nop, g64, d_K, ci > 0, mark);
     }
     if (!noalloc) c.tasks_total = (long long)ntask;
     if (prof) collect(c);
@@ -1912,6 +1992,302 @@ int wb_cycle_dev(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t kgrid, const double* 
     })
 }
 
+// ------------------------------------------------------- cycle queue ---
+// K builds of many molecules with the exchange kernels of all of them in one
+// launch per shell-pair class (launch_md_queue): every molecule's set-up
+// (AO values, F, screening, task lists) runs on the context's stream into an
+// arena of its own, then one queue launch per class drains the (molecule,
+// sub-block, chunk) work items of all of them, then each molecule's K GEMMs
+// run.  Each molecule's buffers and arithmetic are those of its own
+// k_build_arena, so its K equals wb_k_build's bit for bit.
+struct QueueJob {
+    int idx = 0;                        // the caller's job index
+    Basis* B = nullptr;
+    const Grid* G = nullptr;
+    KPlan P;
+    size_t arena = 0, io = 0;           // offsets in the group's arena: the plan's, QD and K's
+};
+
+static size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// work buffer of a group of n jobs: per job the per-class counters, one
+// queue counter per class, then the work-item descriptors (at most nop per
+// job and class) and their prefix sums
+static size_t queue_work_bytes(size_t n, int nop)
+{
+    return align256(8 * (2 * NCLASS * n + NCLASS)) + align256(sizeof(MdJob) * NCLASS * n * nop) +
+           align256(8 * NCLASS * (n * nop + 1));
+}
+
+// one group: every job's one-chunk plan, QD and K at its offsets in the
+// context's K arena
+static void k_build_queue_group(Ctx& c, const std::vector<QueueJob>& g, const wb_cycle_job* jobs,
+                                int nop, const double* omegas, double tol, int prec)
+{
+    const bool f32 = prec == WB_PREC_MIXED;
+    const bool g64 = !f32 || c.g64;
+    hipStream_t s = c.stream;
+    const size_t n = g.size();
+    GemmTilesScope tiles(c.gemm_tiles);
+    std::vector<KRecs> R(n);
+    for (size_t i = 0; i < n; ++i)           // first: building records may allocate and synchronize
+        k_records(c, *g[i].B, nop, omegas, f32, false, R[i]);
+    size_t arena_b = 0;
+    for (const auto& q : g)
+        arena_b = std::max({arena_b, q.arena + q.P.total,
+                            q.io + 8 * (size_t)q.B->nao * q.B->nao * (1 + nop)});
+    char* arena = (char*)c.ws_karena.ensure(arena_b);
+    char* io = arena;
+    char* wbuf = (char*)c.ws_work.ensure(queue_work_bytes(n, nop));
+    auto* work = (unsigned long long*)wbuf;
+    unsigned long long* heads = work + 2 * NCLASS * n;
+    MdJob* d_desc = (MdJob*)(wbuf + align256(8 * (2 * NCLASS * n + NCLASS)));
+    auto* d_first = (long long*)((char*)d_desc + align256(sizeof(MdJob) * NCLASS * n * nop));
+    zero_bytes(s, work, 8 * (2 * NCLASS * n + NCLASS));
+    // every molecule's QD up first (a copy from pageable memory can wait for
+    // the stream), then every molecule's set-up
+    for (size_t i = 0; i < n; ++i)
+        WB_HIP(hipMemcpyAsync(io + g[i].io, jobs[g[i].idx].qd, 8 * (size_t)g[i].B->nao * g[i].B->nao,
+                              hipMemcpyHostToDevice, s));
+    std::vector<std::unique_ptr<KChunk>> K(n);
+    for (size_t i = 0; i < n; ++i) {
+        K[i].reset(new KChunk);
+        const Basis& B = *g[i].B;
+        double* qd = (double*)(io + g[i].io);
+        char* a = arena + g[i].arena;
+        double* QDT = (double*)(a + g[i].P.qdt);
+        transpose(s, B.nao, qd, QDT);
+        k_chunk_setup(c, *g[i].B, *g[i].G, g[i].P, a, work + 2 * NCLASS * i, QDT, 0, f32, g64,
+                      tol, *K[i], [](int) {});
+    }
+    // the class launches: every molecule's shares of class k in job order
+    // (each molecule's in the order its own build launches them)
+    std::vector<MdJob> desc;
+    std::vector<long long> first;
+    struct Launch { int la, lb, nopv, njob; size_t d0, f0; long long total; };
+    std::vector<Launch> launches;
+    for (int k = 0; k < NCLASS; ++k) {
+        Launch L{0, 0, 0, 0, desc.size(), first.size(), 0};
+        for (size_t i = 0; i < n; ++i) {
+            MdJob js[8];
+            int nj = 0;
+            const int nopv = k_class_jobs(c, *g[i].B, *K[i], R[i], k, nop, js, nj);
+            if (!nj) continue;
+            if (L.njob && L.nopv != nopv)
+                throw Error(WB_DEVICE_ERROR, "queue launch: operator counts differ within a class");
+            L.la = g[i].B->cls[k].la;
+            L.lb = g[i].B->cls[k].lb;
+            L.nopv = nopv;
+            for (int t = 0; t < nj; ++t) {
+                first.push_back(L.total);
+                desc.push_back(js[t]);
+                L.total += md_items(js[t]);
+                ++L.njob;
+            }
+        }
+        if (!L.njob) {
+            first.resize(L.f0);
+            continue;
+        }
+        first.push_back(L.total);
+        launches.push_back(L);
+    }
+    if (desc.size() > NCLASS * n * (size_t)nop || first.size() > NCLASS * (n * nop + 1))
+        throw Error(WB_DEVICE_ERROR, "queue launch: more work-item descriptors than planned");
+    // the host vectors live until the group's closing synchronization
+    if (!desc.empty()) {
+        WB_HIP(hipMemcpyAsync(d_desc, desc.data(), sizeof(MdJob) * desc.size(),
+                              hipMemcpyHostToDevice, s));
+        WB_HIP(hipMemcpyAsync(d_first, first.data(), sizeof(long long) * first.size(),
+                              hipMemcpyHostToDevice, s));
+    }
+    const void* boys = f32 ? c.d_b32.p : c.d_b64.p;
+    for (size_t l = 0; l < launches.size(); ++l) {
+        const Launch& L = launches[l];
+        launch_md_queue(s, L.la, L.lb, f32, g64, L.nopv, d_desc + L.d0, d_first + L.f0, L.njob,
+                        L.total, heads + l, boys);
+    }
+    // every molecule's K GEMMs, then every K down
+    for (size_t i = 0; i < n; ++i) {
+        const size_t nn = (size_t)g[i].B->nao * g[i].B->nao;
+        k_chunk_finish(c, *g[i].B, g[i].P, arena + g[i].arena, *K[i], nop, g64,
+                       (double*)(io + g[i].io) + nn, false, [](int) {});
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const size_t nn = (size_t)g[i].B->nao * g[i].B->nao;
+        WB_HIP(hipMemcpyAsync(jobs[g[i].idx].k, (double*)(io + g[i].io) + nn, 8 * nn * nop,
+                              hipMemcpyDeviceToHost, s));
+    }
+    WB_HIP(hipStreamSynchronize(s));
+}
+
+// the arena block of one job in a queue group: its one-chunk plan, then QD
+// and K
+static size_t queue_job_bytes(const KPlan& P, const Basis& B, int nop)
+{
+    return align256(P.total) + align256(8 * (size_t)B.nao * B.nao * (1 + nop));
+}
+
+int wb_cycle_queue(wb_ctx_t ctx, int njob, const wb_cycle_job* jobs, int nop,
+                   const double* omegas, double k_tol, double b, double C, double vv_tol,
+                   int precision, double* job_ms)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c) throw Error(WB_INVALID, "null context");
+        if (njob < 0 || (njob > 0 && !jobs)) throw Error(WB_INVALID, "bad job list");
+        if (precision != WB_PREC_MIXED && precision != WB_PREC_FP64)
+            throw Error(WB_INVALID, "unknown precision");
+        bool anyk = false, anyv = false;
+        for (int i = 0; i < njob; ++i) {
+            const wb_cycle_job& j = jobs[i];
+            if (!j.basis) throw Error(WB_INVALID, "job " + std::to_string(i) + ": null basis");
+            if (j.kgrid && (!j.qd || !j.k))
+                throw Error(WB_INVALID, "job " + std::to_string(i) + ": K build needs qd and k");
+            if (j.nlcgrid && (!j.dm || !j.nlc || !j.vmat))
+                throw Error(WB_INVALID, "job " + std::to_string(i) + ": VV10 needs dm, nlc, vmat");
+            anyk |= j.kgrid != nullptr;
+            anyv |= j.nlcgrid != nullptr;
+        }
+        if (anyk && (nop < 1 || nop > 8 || !omegas)) throw Error(WB_INVALID, "nop out of range");
+        if (anyk) k_check_omegas(nop, omegas);
+        WB_HIP(hipSetDevice(c->device));
+        WB_HIP(hipStreamSynchronize(c->stream));     // work queued on the caller's context first
+        const bool f32 = precision == WB_PREC_MIXED;
+        const bool g64 = !f32 || c->g64;
+        // largest K build first (nao^2 x points)
+        std::vector<int> order(njob);
+        std::vector<double> est(njob, 0.0);
+        for (int i = 0; i < njob; ++i) {
+            order[i] = i;
+            if (jobs[i].kgrid)
+                est[i] = (double)((const Basis*)jobs[i].basis)->nao *
+                         ((const Basis*)jobs[i].basis)->nao * ((const Grid*)jobs[i].kgrid)->Ppad;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return est[x] > est[y]; });
+        const auto t0 = std::chrono::steady_clock::now();
+        auto now_ms = [&] {
+            return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+        };
+        std::vector<double> k_ms(njob, 0.0), v_ms(njob, 0.0);
+        // the VV10 terms: the side context, on a host thread of their own, one
+        // job after another, its workspaces sized for the largest first (so
+        // they are not reallocated while the K groups run)
+        Ctx* sd = anyv ? &side_ctx(*c) : nullptr;
+        if (sd) {
+            size_t vneed[WS_COUNT] = {0};
+            for (int i = 0; i < njob; ++i) {
+                if (!jobs[i].nlcgrid) continue;
+                wb_cycle_job v = jobs[i];
+                v.kgrid = nullptr;
+                size_t need[WS_COUNT], side[WS_COUNT];
+                job_needs(*sd, v, nop, precision, need, side);
+                for (int w = 0; w < WS_COUNT; ++w) vneed[w] = std::max(vneed[w], need[w]);
+            }
+            for (int w = 0; w < WS_COUNT; ++w) sd->bufs[w]->ensure(vneed[w]);
+        }
+        // K groups within the device memory free now plus the K arena's own
+        // (less a reserve) and within the context's K budget, or 64 GiB
+        // without one; a job too large for a group runs its own K build,
+        // tiled to the device's budget.  The plan comes first, so that the
+        // arena is sized once for every group of the call (and for the next
+        // call with the same jobs).
+        size_t free_b = 0, total_b = 0;
+        WB_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t reserve = std::min<size_t>(total_b / 32, (size_t)8 << 30);
+        const size_t held = c->ws_karena.bytes + c->ws_work.bytes;
+        const size_t dev_budget = free_b + held > reserve ? free_b + held - reserve : 0;
+        const size_t budget = std::min(dev_budget, c->k_budget ? c->k_budget : (size_t)64 << 30);
+        const size_t alone_budget = c->k_budget ? std::min(c->k_budget, dev_budget) : dev_budget;
+        struct Step { std::vector<QueueJob> grp; int alone = -1; size_t bytes = 0; };
+        std::vector<Step> plan;
+        size_t arena_max = 0, nmax = 0;
+        for (int i : order) {
+            if (!jobs[i].kgrid) continue;
+            QueueJob q;
+            q.idx = i;
+            q.B = (Basis*)jobs[i].basis;
+            q.G = (const Grid*)jobs[i].kgrid;
+            q.P = k_plan(*q.B, *q.G, nop, f32, g64, c->ncopy, q.G->nsub);
+            const size_t jb = queue_job_bytes(q.P, *q.B, nop);
+            if (jb + queue_work_bytes(1, nop) > budget) {
+                Step st;
+                st.alone = i;
+                st.bytes = k_plan_budget(*q.B, *q.G, nop, f32, g64, c->ncopy, alone_budget).total;
+                arena_max = std::max(arena_max, st.bytes);
+                plan.push_back(std::move(st));
+                continue;
+            }
+            if (plan.empty() || plan.back().alone >= 0 ||
+                plan.back().bytes + jb + queue_work_bytes(plan.back().grp.size() + 1, nop) > budget)
+                plan.emplace_back();
+            Step& st = plan.back();
+            q.arena = st.bytes;
+            q.io = st.bytes + align256(q.P.total);
+            st.bytes += jb;
+            st.grp.push_back(q);
+            arena_max = std::max(arena_max, st.bytes);
+            nmax = std::max(nmax, st.grp.size());
+        }
+        c->ws_karena.ensure(arena_max);
+        if (nmax) c->ws_work.ensure(queue_work_bytes(nmax, nop));
+        std::string verr;
+        int vcode = WB_OK;
+        std::thread vt;
+        if (sd) {
+            vt = std::thread([&] {
+                try {
+                    WB_HIP(hipSetDevice(sd->device));
+                    for (int i : order) {
+                        if (!jobs[i].nlcgrid) continue;
+                        wb_cycle_job v = jobs[i];
+                        v.kgrid = nullptr;
+                        run_cycle_job(*sd, v, nop, omegas, k_tol, b, C, vv_tol, precision);
+                        v_ms[i] = now_ms();
+                    }
+                } catch (const Error& e) {
+                    vcode = e.code;
+                    verr = e.what();
+                } catch (const std::exception& e) {
+                    vcode = WB_DEVICE_ERROR;
+                    verr = e.what();
+                }
+            });
+        }
+        try {
+            for (const Step& st : plan) {
+                if (st.alone < 0) {
+                    k_build_queue_group(*c, st.grp, jobs, nop, omegas, k_tol, precision);
+                    const double t = now_ms();
+                    for (const auto& q : st.grp) k_ms[q.idx] = t;
+                    continue;
+                }
+                wb_cycle_job kj = jobs[st.alone];
+                kj.nlcgrid = nullptr;
+                const size_t kb = c->k_budget;
+                c->k_budget = alone_budget;
+                try {
+                    run_cycle_job(*c, kj, nop, omegas, k_tol, b, C, vv_tol, precision);
+                } catch (...) {
+                    c->k_budget = kb;
+                    throw;
+                }
+                c->k_budget = kb;
+                k_ms[st.alone] = now_ms();
+            }
+        } catch (...) {
+            if (vt.joinable()) vt.join();
+            throw;
+        }
+        if (vt.joinable()) vt.join();
+        if (vcode != WB_OK) throw Error(vcode, "cycle queue, VV10: " + verr);
+        if (job_ms)
+            for (int i = 0; i < njob; ++i) job_ms[i] = std::max(k_ms[i], v_ms[i]);
+    })
+}
+
 // ------------------------------------------------------- cycle graphs ---
 // One SCF cycle's device work recorded once as a HIP graph.  The graph owns
 // its workspaces (a private context on the owner's stream): an eager call on
@@ -2158,6 +2534,18 @@ int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, i
     })
 }
 
+// S_g = X^T diag(w) X on the context (nao x nao, device, into ws_K)
+static double* grid_overlap_dev(Ctx& c, Basis& b, Grid& g)
+{
+    const int nao = b.nao, Ppad = g.Ppad;
+    double* X = (double*)c.ws_X.ensure(sizeof(double) * nao * (size_t)Ppad);
+    launch_ao_eval(c.stream, b, g, 0, X);
+    // padding points carry weight 0
+    double* S = (double*)c.ws_K.ensure(sizeof(double) * 3 * nao * (size_t)nao);
+    gemm_nt_fast(c.stream, nao, nao, Ppad, X, X, false, 1, 0, g.d_w.as<double>(), S, c.ws_Kp);
+    return S;
+}
+
 int wb_grid_overlap(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, double* out)
 {
     WB_TRY({
@@ -2166,16 +2554,50 @@ int wb_grid_overlap(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, double* out)
         auto* g = (Grid*)grid;
         if (!c || !b || !g || !out) throw Error(WB_INVALID, "null argument");
         WB_HIP(hipSetDevice(c->device));
-        const int nao = b->nao, Ppad = g->Ppad;
-        double* X = (double*)c->ws_X.ensure(sizeof(double) * nao * (size_t)Ppad);
-        launch_ao_eval(c->stream, *b, *g, 0, X);
-        // padding points carry weight 0
-        double* S = (double*)c->ws_K.ensure(sizeof(double) * nao * (size_t)nao);
-        gemm_nt_fast(c->stream, nao, nao, Ppad, X, X, false, 1, 0, g->d_w.as<double>(), S,
-                     c->ws_Kp);
+        const int nao = b->nao;
+        double* S = grid_overlap_dev(*c, *b, *g);
         WB_HIP(hipMemcpyAsync(out, S, sizeof(double) * nao * (size_t)nao, hipMemcpyDeviceToHost,
                               c->stream));
         WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_pjs_projector(wb_ctx_t ctx, wb_basis_t basis, wb_grid_t grid, const double* S_an,
+                     double* Q)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        auto* b = (Basis*)basis;
+        auto* g = (Grid*)grid;
+        if (!c || !b || !g || !S_an || !Q) throw Error(WB_INVALID, "null argument");
+        WB_HIP(hipSetDevice(c->device));
+        const int nao = b->nao;
+        const size_t nn = (size_t)nao * nao;
+        double* Sg = grid_overlap_dev(*c, *b, *g);        // ws_K: [Sg | rhs | out]
+        double* rhs = Sg + nn;
+        double* out = Sg + 2 * nn;
+        WB_HIP(hipMemcpyAsync(rhs, S_an, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+        // LU with partial pivoting (PySCF's scipy.linalg.solve(S_g, S)); both
+        // matrices symmetric, so row- and column-major storage agree on input;
+        // the column-major solution X, read row-major, is X^T
+        int* ipiv = (int*)c->ws_io.ensure(sizeof(int) * (nao + 1));
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success)
+            throw Error(WB_DEVICE_ERROR, "rocblas_create_handle failed");
+        struct Guard { rocblas_handle h; ~Guard() { rocblas_destroy_handle(h); } } guard{h};
+        rocblas_set_stream(h, c->stream);
+        if (rocsolver_dgetrf(h, nao, nao, Sg, nao, ipiv, ipiv + nao) != rocblas_status_success ||
+            rocsolver_dgetrs(h, rocblas_operation_none, nao, nao, Sg, nao, ipiv, rhs, nao) !=
+                rocblas_status_success)
+            throw Error(WB_DEVICE_ERROR, "rocSOLVER LU solve failed");
+        int info = 0;
+        WB_HIP(hipMemcpyAsync(&info, ipiv + nao, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        transpose(c->stream, nao, rhs, out);
+        WB_HIP(hipMemcpyAsync(Q, out, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+        WB_HIP(hipStreamSynchronize(c->stream));
+        if (info != 0)
+            throw Error(WB_UNSUPPORTED, "grid overlap matrix is singular (LU pivot " +
+                                            std::to_string(info) + "): the grid cannot fit the overlap");
     })
 }
 

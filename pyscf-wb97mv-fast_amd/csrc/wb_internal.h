@@ -104,7 +104,6 @@ struct Basis {
     std::vector<double> exps, coefs;
     DevBuf d_shells, d_exps, d_coefs, d_shao;   // d_shao: ao0[nshell], nao[nshell]
     DevBuf d_row0;                              // int [nao]: first AO of the AO's shell
-    DevBuf d_rowsh;                             // int [nao]: the AO's shell
     ClassData cls[NCLASS];
     double pair_cut = 1e-14;
     long long npairs_total = 0;
@@ -216,10 +215,10 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
                   bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
                   const unsigned* masks = nullptr, const int* row0 = nullptr, int nsub = 0);
 
-// fmax[s][sh] = max over sub-block s of w_g sum_{mu in sh} |F[mu][g]|;
-// row_shell[mu] = the shell of AO row mu (shells in row order)
+// fmax[s][sh] = max over sub-block s of w_g sum_{mu in sh} |F[mu][g]|
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
-                 int Ppad, int nsub, const int* row_shell, int nshell, float* fmax);
+                 int Ppad, int nsub, const int* sh_ao0, const int* sh_nao, int nshell,
+                 float* fmax);
 // one class's screening inputs and task-list outputs
 struct TaskClass {
     const PairScreen* scr;
@@ -260,12 +259,42 @@ struct MdMasks {
     int zrow = 0;
     bool first = false;
 };
-// nchunk (<= the number of G copies): waves per sub-block, chunk j of a
-// sub-block's task list writing copy j
-void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
-               const void* F, void* G, size_t gstride, size_t opstride, int nchunk,
-               const int* counts, const int* ids, const unsigned* skips, const void* recs,
-               const void* boys, int zero_rows, const MdMasks& mk);
+// One molecule's share of a class launch of the exchange kernel: its points,
+// F, G copies (operator o at G + o opstride), the class's task lists of its
+// sub-blocks (ids/skips: cap entries per sub-block), pair groups and
+// primitive records.  nchunk (<= the number of G copies): waves per
+// sub-block, chunk j of a sub-block's task list writing copy j.  masks null:
+// the launch clears zero_rows rows of every region first.
+struct MdJob {
+    const double* xyz;
+    const void* F;
+    void* G;
+    size_t gstride, opstride;
+    const int* counts;
+    const int* ids;
+    const unsigned* skips;
+    const PairGroup* groups;
+    const int* bbao;
+    const char* recs;
+    unsigned* masks;
+    size_t mopstride;
+    int cap, Ppad, nsub, nchunk, zero_rows, zrow, first, pad_;
+};
+MdJob md_job(const ClassData& c, const Grid& g, const void* F, void* G, size_t gstride,
+             size_t opstride, int nchunk, const int* counts, const int* ids,
+             const unsigned* skips, const void* recs, int zero_rows, const MdMasks& mk);
+// waves of one molecule's share (its work items in a queue launch)
+inline long long md_items(const MdJob& j) { return (long long)j.nsub * j.nchunk; }
+void launch_md(hipStream_t s, int la, int lb, bool f32, bool g64, int nop, const MdJob& j,
+               const void* boys);
+// One launch of class (la, lb) over njob molecules' shares (jobs, on the
+// device): a grid sized to the device's resident waves takes work items from
+// the counter head (zero at the launch); first[m] (njob + 1 entries, on the
+// device): the first item of job m, first[njob] = total.  Each molecule's G
+// equals the one its own launch_md writes, bit for bit.
+void launch_md_queue(hipStream_t s, int la, int lb, bool f32, bool g64, int nop,
+                     const MdJob* jobs, const long long* first, int njob, long long total,
+                     unsigned long long* head, const void* boys);
 
 // GGA density on the grid without stored AO gradients: rho[0][g] =
 // sum_mu T[mu][g] X[mu][g], rho[1+d][g] = 2 sum_mu T[mu][g] dX_d[mu][g]

@@ -124,19 +124,28 @@ __device__ __forceinline__ void gsph(const real* Gc, real* Gs) {
 
 constexpr int MD_WAVES = 4;
 
-// real: integral arithmetic; gacc: G accumulation (registers and memory);
-// NOP operators at once (G of operator o at G + o opstride)
+// The Boys table of order L staged in LDS by the block (FP32 only)
+template <int L, typename real>
+__device__ __forceinline__ const real* md_stage_boys(const real* __restrict__ boys_g, real* s_tab)
+{
+    if constexpr (sizeof(real) == 4) {
+        constexpr int TAB = B32_NODES * b32_row(L);
+        const real* src = boys_g + b32_off(L);
+        for (int i = threadIdx.x; i < TAB; i += blockDim.x) s_tab[i] = src[i];
+        __syncthreads();
+        return s_tab;
+    }
+    return boys_g;
+}
+
+// One wave's work in one class launch: sub-block s (one point per lane) and
+// chunk j of s's task list, of the molecule a describes (MdJob: its F, G
+// copies, task lists and records).  real: integral arithmetic; gacc: G
+// accumulation (registers and memory); NOP operators at once (G of operator
+// o at G + o opstride).  The single-molecule kernel and the many-molecule
+// queue kernel run exactly this.
 template <int LA, int LB, int NOP, typename real, typename gacc>
-__global__ __launch_bounds__(64 * MD_WAVES)
-void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
-                 gacc* __restrict__ G, size_t gstride, size_t opstride,
-                 const int* __restrict__ counts, const int* __restrict__ ids,
-                 const unsigned* __restrict__ skips, int cap,
-                 const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
-                 const char* __restrict__ recs,
-                 const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk,
-                 int zero_rows, unsigned* __restrict__ masks, size_t mopstride, int zrow,
-                 int first)
+__device__ __forceinline__ void md_wave(const MdJob& a, const real* __restrict__ tab, int s, int j)
 {
     constexpr int L = LA + LB;
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
@@ -147,21 +156,24 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     // (<= 36 entries); the larger f-shell blocks belong to single-primitive
     // shells in the def2 sets and stay in the arithmetic type
     using vacc = typename std::conditional<(NCA * NCB <= 36), double, real>::type;
-    // FP32: this class's Boys table (order L) is staged in LDS
-    constexpr int TAB = F32 ? B32_NODES * b32_row(L) : 1;
-    __shared__ __align__(16) real s_tab[TAB];
-    const real* tab = boys_g;
-    if constexpr (F32) {
-        const real* src = boys_g + b32_off(L);
-        for (int i = threadIdx.x; i < TAB; i += blockDim.x) s_tab[i] = src[i];
-        __syncthreads();
-        tab = s_tab;
-    }
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    (void)NSA; (void)NSB; (void)L; (void)F32;
+    const double* __restrict__ xyz = a.xyz;
+    const real* __restrict__ F = (const real*)a.F;
+    gacc* __restrict__ G = (gacc*)a.G;
+    const size_t gstride = a.gstride, opstride = a.opstride;
+    const int* __restrict__ counts = a.counts;
+    const int* __restrict__ ids = a.ids;
+    const unsigned* __restrict__ skips = a.skips;
+    const int cap = a.cap;
+    const PairGroup* __restrict__ groups = a.groups;
+    const int* __restrict__ bbao = a.bbao;
+    const char* __restrict__ recs = a.recs;
+    const int Ppad = a.Ppad, nsub = a.nsub, nchunk = a.nchunk, zero_rows = a.zero_rows;
+    unsigned* __restrict__ masks = a.masks;
+    const size_t mopstride = a.mopstride;
+    const int zrow = a.zrow, first = a.first;
+    (void)nchunk;
     const int lane = threadIdx.x & 63;
-    const int worker = blockIdx.x * MD_WAVES + wave;
-    if (worker >= nsub * nchunk) return;
-    const int s = worker % nsub, j = worker / nsub;
     gacc* const Gc = G + (size_t)j * gstride;         // operator o: Gc + o opstride
     const int g = s * SUB + lane;
     // This wave alone owns rows x its 64 points of copy j in every class
@@ -410,39 +422,130 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     if (tracked) store_mask();
 }
 
+// one class, one molecule: worker = chunk x nsub + sub-block
 template <int LA, int LB, int NOP, typename real, typename gacc>
-static void md_launch_t(hipStream_t s, const ClassData& c, const Grid& g, const void* F,
-                        void* G, size_t gstride, size_t opstride, int nchunk, const int* counts,
-                        const int* ids, const unsigned* skips, const void* recs,
-                        const void* boys, int zero_rows, const MdMasks& mk)
+__global__ __launch_bounds__(64 * MD_WAVES)
+void md_k_kernel(MdJob a, const real* __restrict__ boys_g)
 {
-    const int workers = g.nsub * nchunk;
-    const int blocks = (workers + MD_WAVES - 1) / MD_WAVES;
-    hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks), dim3(64 * MD_WAVES),
-                       0, s, g.d_xyz.as<double>(), (const real*)F, (gacc*)G, gstride, opstride,
-                       counts, ids, skips, c.ngroups, c.d_groups.as<PairGroup>(), c.d_bbao.as<int>(),
-                       (const char*)recs, (const real*)boys, g.Ppad, g.nsub, nchunk, zero_rows,
-                       mk.m, mk.opstride, mk.zrow, mk.first ? 1 : 0);
+    constexpr int TAB = sizeof(real) == 4 ? B32_NODES * b32_row(LA + LB) : 1;
+    __shared__ __align__(16) real s_tab[TAB];
+    const real* tab = md_stage_boys<LA + LB, real>(boys_g, s_tab);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int worker = blockIdx.x * MD_WAVES + wave;
+    if (worker >= a.nsub * a.nchunk) return;
+    md_wave<LA, LB, NOP, real, gacc>(a, tab, worker % a.nsub, worker / a.nsub);
+}
+
+// one class, many molecules: a persistent grid whose waves take work items
+// from one device counter until all are taken; item i belongs to the
+// molecule m with first[m] <= i < first[m + 1] and is its wave
+// (i - first[m]) of the single-molecule launch above, so every molecule's G
+// is exactly the one its own launch writes.  Every wave leaves the loop once
+// the counter has passed total.
+template <int LA, int LB, int NOP, typename real, typename gacc>
+__global__ __launch_bounds__(64 * MD_WAVES)
+void md_queue_kernel(const MdJob* __restrict__ jobs, const long long* __restrict__ first,
+                     int njob, long long total, unsigned long long* __restrict__ head,
+                     const real* __restrict__ boys_g)
+{
+    constexpr int TAB = sizeof(real) == 4 ? B32_NODES * b32_row(LA + LB) : 1;
+    __shared__ __align__(16) real s_tab[TAB];
+    const real* tab = md_stage_boys<LA + LB, real>(boys_g, s_tab);
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long it = 0;
+        if (lane == 0) it = atomicAdd(head, 1ull);
+        const unsigned lo32 = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)it);
+        const unsigned hi32 = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(it >> 32));
+        const long long item = (long long)(((unsigned long long)hi32 << 32) | lo32);
+        if (item >= total) break;
+        int lo = 0, hi = njob - 1;                    // last job with first[m] <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const MdJob a = jobs[lo];
+        const long long local = item - first[lo];
+        md_wave<LA, LB, NOP, real, gacc>(a, tab, (int)(local % a.nsub), (int)(local / a.nsub));
+    }
+}
+
+MdJob md_job(const ClassData& c, const Grid& g, const void* F, void* G, size_t gstride,
+             size_t opstride, int nchunk, const int* counts, const int* ids,
+             const unsigned* skips, const void* recs, int zero_rows, const MdMasks& mk)
+{
+    MdJob j{};
+    j.xyz = g.d_xyz.as<double>();
+    j.F = F;
+    j.G = G;
+    j.gstride = gstride;
+    j.opstride = opstride;
+    j.counts = counts;
+    j.ids = ids;
+    j.skips = skips;
+    j.groups = c.d_groups.as<PairGroup>();
+    j.bbao = c.d_bbao.as<int>();
+    j.recs = (const char*)recs;
+    j.masks = mk.m;
+    j.mopstride = mk.opstride;
+    j.cap = c.ngroups;
+    j.Ppad = g.Ppad;
+    j.nsub = g.nsub;
+    j.nchunk = nchunk;
+    j.zero_rows = zero_rows;
+    j.zrow = mk.zrow;
+    j.first = mk.first ? 1 : 0;
+    return j;
+}
+
+// resident blocks of a queue kernel on the current device (all its CUs)
+template <typename K>
+static int queue_blocks(K kernel)
+{
+    int dev = 0, cus = 0, per = 0;
+    WB_HIP(hipGetDevice(&dev));
+    WB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    WB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64 * MD_WAVES, 0));
+    return std::max(1, cus) * std::max(1, per);
+}
+
+template <int LA, int LB, int NOP, typename real, typename gacc>
+static void md_launch_t(hipStream_t s, const MdJob* j, const MdJob* jobs, const long long* first,
+                        int njob, long long total, unsigned long long* head, const void* boys)
+{
+    if (j) {
+        const long long workers = md_items(*j);
+        const int blocks = (int)((workers + MD_WAVES - 1) / MD_WAVES);
+        hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks),
+                           dim3(64 * MD_WAVES), 0, s, *j, (const real*)boys);
+    } else {
+        auto kern = md_queue_kernel<LA, LB, NOP, real, gacc>;
+        static const int resident = queue_blocks(kern);     // one device kind per process
+        const long long need = (total + MD_WAVES - 1) / MD_WAVES;
+        const int blocks = (int)std::min<long long>(need, resident);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * MD_WAVES), 0, s, jobs, first, njob,
+                           total, head, (const real*)boys);
+    }
     WB_HIP(hipGetLastError());
 }
 
-void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool g64, int nop,
-               const void* F, void* G, size_t gstride, size_t opstride, int nchunk,
-               const int* counts, const int* ids, const unsigned* skips, const void* recs,
-               const void* boys, int zero_rows, const MdMasks& mk)
+// one class launch, single molecule (j) or queue (jobs ...)
+static void md_dispatch(hipStream_t s, int la, int lb, bool f32, bool g64, int nop,
+                        const MdJob* j, const MdJob* jobs, const long long* first, int njob,
+                        long long total, unsigned long long* head, const void* boys)
 {
-    if (c.npairs == 0) return;
-    if (nop != 1 && !(nop == 2 && md_fuse(c.la, c.lb)))
+    if (nop != 1 && !(nop == 2 && md_fuse(la, lb)))
         throw Error(WB_INVALID, "operator count not available for this shell-pair class");
-#define WB_LAUNCH(XA, XB, NOPV)                                                                      \
-    if (f32 && !g64)                                                                                 \
-        md_launch_t<XA, XB, NOPV, float, float>(s, c, g, F, G, gstride, opstride, nchunk, counts, ids, skips, recs, boys, zero_rows, mk);  \
-    else if (f32)                                                                                    \
-        md_launch_t<XA, XB, NOPV, float, double>(s, c, g, F, G, gstride, opstride, nchunk, counts, ids, skips, recs, boys, zero_rows, mk); \
-    else                                                                                             \
-        md_launch_t<XA, XB, NOPV, double, double>(s, c, g, F, G, gstride, opstride, nchunk, counts, ids, skips, recs, boys, zero_rows, mk);
+#define WB_LAUNCH(XA, XB, NOPV)                                                                    \
+    if (f32 && !g64)                                                                               \
+        md_launch_t<XA, XB, NOPV, float, float>(s, j, jobs, first, njob, total, head, boys);       \
+    else if (f32)                                                                                  \
+        md_launch_t<XA, XB, NOPV, float, double>(s, j, jobs, first, njob, total, head, boys);      \
+    else                                                                                           \
+        md_launch_t<XA, XB, NOPV, double, double>(s, j, jobs, first, njob, total, head, boys);
 #define WB_CASE(XA, XB)                                                                    \
-    if (c.la == XA && c.lb == XB) {                                                        \
+    if (la == XA && lb == XB) {                                                            \
         if constexpr (md_fuse(XA, XB)) {                                                   \
             if (nop == 2) { WB_LAUNCH(XA, XB, 2) return; }                                 \
         }                                                                                  \
@@ -454,6 +557,21 @@ void launch_md(hipStream_t s, const ClassData& c, const Grid& g, bool f32, bool 
 #undef WB_CASE
 #undef WB_LAUNCH
     throw Error(WB_UNSUPPORTED, "shell-pair class beyond (f, f)");
+}
+
+void launch_md(hipStream_t s, int la, int lb, bool f32, bool g64, int nop, const MdJob& j,
+               const void* boys)
+{
+    if (md_items(j) == 0) return;
+    md_dispatch(s, la, lb, f32, g64, nop, &j, nullptr, nullptr, 0, 0, nullptr, boys);
+}
+
+void launch_md_queue(hipStream_t s, int la, int lb, bool f32, bool g64, int nop,
+                     const MdJob* jobs, const long long* first, int njob, long long total,
+                     unsigned long long* head, const void* boys)
+{
+    if (njob == 0 || total == 0) return;
+    md_dispatch(s, la, lb, f32, g64, nop, nullptr, jobs, first, njob, total, head, boys);
 }
 
 // F_0..F_L(T) of the device Boys routines for a list of T (exposed for
@@ -494,72 +612,62 @@ void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
 
 // ------------------------------------------------------------ screening ----
 // fmax[s][sh] = max_{g in s} w_g * sum_{mu in sh} |F[mu][g]|
-// One wave per sub-block, one point per lane.  The AO rows are walked in
-// order (independent coalesced loads, so many are in flight); each finished
-// shell's per-point value goes to an LDS tile of 64 shells x 64 points, and
-// then every lane takes the maximum of one shell's row of the tile: one
-// LDS transpose per 64 shells instead of a wave reduction per shell.
+// The shells of a sub-block are split over FMAX_SPLIT waves (one point per
+// lane); a wave takes 8 shells per pass: all their rows' loads first (they
+// are independent), then 8 interleaved wave reductions.  Measured on
+// (H2O)27 def2-SVP (F 566 MB, FP32): 0.38 ms with one shell and one
+// reduction at a time per sub-block, 0.32 ms this way; what remains is the
+// read of F in 256-byte row pieces.
+constexpr int FMAX_SPLIT = 4;
 template <typename real>
 __global__ __launch_bounds__(256)
 void fmax_kernel(const real* __restrict__ F, const double* __restrict__ w, int Ppad,
-                 int nsub, const int* __restrict__ row_shell, int nao, int nshell,
-                 float* __restrict__ fmax)
+                 int nsub, const int* __restrict__ sh_ao0, const int* __restrict__ sh_nao,
+                 int nshell, float* __restrict__ fmax)
 {
-    __shared__ float tile[4][64][65];                  // [wave][shell][point], padded
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int s = blockIdx.x * 4 + wave;
-    if (s >= nsub) return;                             // no block-wide barrier below
+    const int wid = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= nsub * FMAX_SPLIT) return;
+    const int s = wid / FMAX_SPLIT, q = wid % FMAX_SPLIT;
+    const int shb = (int)((long long)nshell * q / FMAX_SPLIT);
+    const int she = (int)((long long)nshell * (q + 1) / FMAX_SPLIT);
     const int lane = threadIdx.x & 63;
     const int g = s * SUB + lane;
     const double wg = w[g];
-    float (*t)[65] = tile[wave];
-    int sh0 = 0;                                       // first shell of the current tile
-    double acc = 0.0;
-    constexpr int RB = 16;                             // rows loaded ahead, independent
-    for (int r0 = 0; r0 < nao; r0 += RB) {
-      float v[RB];
+    for (int sh0 = shb; sh0 < she; sh0 += 8) {
+        float v[8];
 #pragma unroll
-      for (int i = 0; i < RB; ++i)
-          v[i] = r0 + i < nao ? fabsf((float)F[(size_t)(r0 + i) * Ppad + g]) : 0.f;
-#pragma unroll
-      for (int i = 0; i < RB; ++i) {
-        const int r = r0 + i;
-        if (r >= nao) break;                           // uniform
-        acc += (double)v[i];
-        const int sh = row_shell[r];                   // uniform
-        const bool last = r + 1 == nao || row_shell[r + 1] != sh;
-        if (last) {
-            t[sh - sh0][lane] = (float)(acc * wg);
-            acc = 0.0;
-            if (sh - sh0 == 63 || sh + 1 == nshell) {
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                const int nb = sh - sh0 + 1;
-                if (lane < nb) {
-                    float m = 0.f;
-#pragma unroll 8
-                    for (int p = 0; p < 64; ++p) m = fmaxf(m, t[lane][p]);
-                    fmax[(size_t)s * nshell + sh0 + lane] = m * 1.0001f;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                sh0 = sh + 1;
+        for (int k = 0; k < 8; ++k) {
+            const int sh = sh0 + k;
+            double acc = 0.0;
+            if (sh < she) {                              // uniform
+                const int a0 = sh_ao0[sh], n = sh_nao[sh];
+                for (int m = 0; m < n; ++m) acc += fabs((double)F[(size_t)(a0 + m) * Ppad + g]);
             }
+            v[k] = (float)(acc * wg);
         }
-      }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = fmaxf(v[k], __shfl_xor(v[k], o, 64));
+        if (lane < 8 && sh0 + lane < she) {
+            float r = v[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) r = lane == k ? v[k] : r;
+            fmax[(size_t)s * nshell + sh0 + lane] = r * 1.0001f;
+        }
     }
 }
 
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao, int Ppad,
-                 int nsub, const int* row_shell, int nshell, float* fmax)
+                 int nsub, const int* sh_ao0, const int* sh_nao, int nshell, float* fmax)
 {
-    const int blocks = (nsub + 3) / 4;
+    const int blocks = (nsub * FMAX_SPLIT + 3) / 4;
     if (f32)
         hipLaunchKernelGGL(fmax_kernel<float>, dim3(blocks), dim3(256), 0, s,
-                           (const float*)F, w, Ppad, nsub, row_shell, nao, nshell, fmax);
+                           (const float*)F, w, Ppad, nsub, sh_ao0, sh_nao, nshell, fmax);
     else
         hipLaunchKernelGGL(fmax_kernel<double>, dim3(blocks), dim3(256), 0, s,
-                           (const double*)F, w, Ppad, nsub, row_shell, nao, nshell, fmax);
+                           (const double*)F, w, Ppad, nsub, sh_ao0, sh_nao, nshell, fmax);
     WB_HIP(hipGetLastError());
 }
 

@@ -38,3 +38,18 @@ def grid_overlap(ctx, basis, grid):
     out = numpy.empty((basis.nao, basis.nao))
     lib.check(lib.load().wb_grid_overlap(ctx.h, basis.h, grid.h, lib.ptr(out)))
     return out
+
+
+def pjs_projector(ctx, basis, grid, S):
+    """The overlap-fitted COSX projector Q = S_g^{-1} S on the device: S_g the
+    grid overlap, S the analytic overlap (PySCF's int1e_ovlp), LU with partial
+    pivoting as PySCF's scipy.linalg.solve(S_g, S) (the matrix PySCF keeps as
+    sgx._pjs_data._overlap_correction_matrix and the reference reads at
+    gpu/sgx_k.py:698-704).  K built from Q dm integrates the AO products
+    against the grid as the analytic overlap does."""
+    S = lib.f64(S)
+    if S.shape != (basis.nao, basis.nao):
+        raise ValueError('S must be (nao, nao)')
+    out = numpy.empty((basis.nao, basis.nao))
+    lib.check(lib.load().wb_pjs_projector(ctx.h, basis.h, grid.h, lib.ptr(S), lib.ptr(out)))
+    return out

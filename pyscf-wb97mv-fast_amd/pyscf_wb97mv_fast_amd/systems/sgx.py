@@ -40,10 +40,15 @@ sgx_jk = types.SimpleNamespace(get_k_only=_no_get_k_only)
 
 
 class SyntheticSGX:
-    def __init__(self, mol, grids, Q=None, sym_ovlp=False, j_builder=None):
+    """Q: a fixed PJS projector; fit_ovlp=ctx: the overlap-fitted projector
+    S_g^-1 S built on that device context at _build_pjs (gpu.grids.pjs_projector,
+    as PySCF builds it); neither: the identity."""
+
+    def __init__(self, mol, grids, Q=None, sym_ovlp=False, j_builder=None, fit_ovlp=None):
         self.mol = mol
         self.grids = grids
         self._Q = Q
+        self.fit_ovlp = fit_ovlp
         self._sym = sym_ovlp
         self._pjs_data = None
         self._rsh_df = {}
@@ -55,6 +60,15 @@ class SyntheticSGX:
 
     def _build_pjs(self, direct_scf_tol=1e-13):
         Q = numpy.eye(self.mol.nao) if self._Q is None else self._Q
+        if self._Q is None and self.fit_ovlp is not None:
+            from .. import lib
+            from ..basis import pack_mol
+            from ..gpu.grids import pjs_projector
+            from .guess import overlap_matrix
+            ctx = self.fit_ovlp
+            Q = pjs_projector(ctx, lib.DeviceBasis(ctx, pack_mol(self.mol)),
+                              lib.DeviceGrid(ctx, self.grids.coords, self.grids.weights),
+                              overlap_matrix(self.mol))
         self._pjs_data = PJSData(self.mol, self.grids, direct_scf_tol, Q, self._sym)
 
     def copy(self):

@@ -118,3 +118,50 @@ def test_grid_overlap_vs_quadrature_and_analytic(ctx):
     gf = grids.Grids(mol, level=5, device=ctx).build()
     Sf = dgrids.grid_overlap(ctx, b, lib.DeviceGrid(ctx, gf.coords, gf.weights))
     assert numpy.abs(Sf - guess.overlap_matrix(mol)).max() < 1e-6
+
+
+def test_pjs_projector_on_the_device_matches_the_host_solve():
+    """Q = S_g^-1 S (wb_pjs_projector, rocSOLVER LU) against numpy's solve on
+    the oracle's quadrature overlap, and against the identity in the limit it
+    has: S_g Q = S."""
+    import oracle
+    from pyscf_wb97mv_fast_amd.gpu import grids as dgrids
+    ctx = lib.Context(0)
+    mol = mole.Mole(geometry.water_cluster(3, seed=2), 'def2-svp')
+    g = grids.Grids(mol, level=1, device=ctx).build()
+    b = lib.DeviceBasis(ctx, pack_mol(mol))
+    gr = lib.DeviceGrid(ctx, g.coords, g.weights)
+    S = guess.overlap_matrix(mol)
+    Q = dgrids.pjs_projector(ctx, b, gr, S)
+    X = oracle.ao_eval(oracle.OracleBasis.from_mol(mol), g.coords)[0]
+    Sg = (X * g.weights[:, None]).T @ X
+    ref = numpy.linalg.solve(Sg, S)
+    print('max|dQ| %.2e, max|Q - 1| %.2e' % (numpy.abs(Q - ref).max(), numpy.abs(Q - numpy.eye(len(Q))).max()))
+    assert numpy.allclose(Q, ref, rtol=0, atol=1e-9 * numpy.abs(ref).max())
+    assert numpy.allclose(Sg @ Q, S, atol=1e-9)
+    ctx.close()
+
+
+def test_pjs_projector_brings_the_k_hook_closer_to_the_analytic_exchange():
+    """The overlap-fitted projector built on the device, used through the K
+    hook's PJS contract on the s-only molecule of the closed-form pin: the
+    K with Q is closer to the exact exchange than the K with Q = I."""
+    from pyscf_wb97mv_fast_amd.gpu import sgx_k
+    from pyscf_wb97mv_fast_amd.systems import sgx
+    from test_oracle_pins import s_only_system, ss_eri
+    ctx = lib.Context(0)
+    mol = s_only_system()
+    rng = numpy.random.default_rng(1)
+    D = rng.normal(size=(mol.nao, mol.nao))
+    D = 0.5 * (D + D.T)
+    kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64')
+    g = grids.Grids(mol, level=2).build()
+    for omega in (0.0, 0.3):
+        exact = numpy.einsum('mlns,ls->mn', ss_eri(mol, omega), D)
+        with mol.with_range_coulomb(omega):
+            k_i = sgx_k.get_k_only_gpu(kb, sgx.SyntheticSGX(mol, g), D)
+            k_q = sgx_k.get_k_only_gpu(kb, sgx.SyntheticSGX(mol, g, fit_ovlp=ctx), D)
+        e_i, e_q = numpy.abs(k_i - exact).max(), numpy.abs(k_q - exact).max()
+        print('omega %.1f: max|dK| Q=I %.2e, fitted Q %.2e' % (omega, e_i, e_q))
+        assert e_q < e_i
+    ctx.close()
