@@ -103,9 +103,10 @@ def _clusters(count, nmax, seed):
 
 def bench_setup(args):
     """Per-molecule set-up on the device path: wb_basis_create (shell pairs on
-    the host threads) and wb_grid_partition (cell weights on the GPU) for
-    water clusters of 8, 27 and 64 molecules, level-1 grids; the host numpy
-    partition timed beside it on (H2O)8."""
+    the host threads), wb_grid_partition (cell weights on the GPU) and
+    wb_grid_build (the whole grid on the GPU) for water clusters of 8, 27 and
+    64 molecules, level-1 grids; the host numpy partition timed beside it on
+    (H2O)8."""
     from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.basis import pack_mol
     from pyscf_wb97mv_fast_amd.gpu import grids as dgrids
@@ -124,9 +125,13 @@ def bench_setup(args):
         t = time.perf_counter()
         part = dgrids.ssf_partition(ctx, coords, atoms, owner)
         t_part = time.perf_counter() - t
+        t = time.perf_counter()
+        g = grids.Grids(mol, level=1, device=ctx).build()
+        t_build = time.perf_counter() - t
         row = {'waters': n, 'nao': b.nao, 'shell_pairs': b.npairs, 'primitive_pairs': b.nprims,
                'basis_create_s': round(t_basis, 4), 'raw_points': len(coords),
-               'partition_s': round(t_part, 4)}
+               'partition_s': round(t_part, 4), 'device_grid_build_s': round(t_build, 4),
+               'grid_points': g.size}
         if n == 8:
             t = time.perf_counter()
             ref = grids._ssf_partition(coords, atoms, owner)

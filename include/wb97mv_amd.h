@@ -330,6 +330,23 @@ int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, i
 int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
                       const double* coords, const int* owner, double* out);
 
+/* An atom-centred grid built on the device: the points of radial shell sh
+ * (atom shell_atom[sh], radius shell_r[sh], radial weight shell_w[sh]) are
+ * atom + r u_k for the points (u_k, wa_k) of angular rule shell_rule[sh]
+ * (rule k: rule_npts[k] rows of x, y, z, w in rule_xyzw, rules in order),
+ * weighted shell_w wa_k times the cell partition of wb_grid_partition;
+ * points of weight <= w_cut are dropped.  Written in shell order, then rule
+ * order, to coords (npts x 3) and weights (capacity points at most; *npts =
+ * the count kept).  Products and sums are rounded one by one, so the grid
+ * equals the host's construction of the same points bit for bit.  Replaces
+ * the SGX / XC grid build the reference leaves to PySCF on the host
+ * (with_df.build(level=...), staging/schedule.py:391; grids.build,
+ * gpu/install.py:164). */
+int wb_grid_build(wb_ctx_t ctx, int natm, const double* atoms, int nshell, const int* shell_atom,
+                  const double* shell_r, const double* shell_w, const int* shell_rule, int nrule,
+                  const int* rule_npts, const double* rule_xyzw, double w_cut, long long capacity,
+                  double* coords, double* weights, long long* npts);
+
 /* Overlap matrix on the grid, S_g = sum_g w_g chi_mu(r_g) chi_nu(r_g) (nao x
  * nao, host out), FP64 AO values and FP64 MFMA: the quadrature overlap that
  * COSX's overlap-fitted projector is built from (PySCF's fit_ovlp; the

@@ -2697,6 +2697,91 @@ int wb_grid_partition(wb_ctx_t ctx, int natm, const double* atoms, int npts,
     })
 }
 
+int wb_grid_build(wb_ctx_t ctx, int natm, const double* atoms, int nshell, const int* shell_atom,
+                  const double* shell_r, const double* shell_w, const int* shell_rule, int nrule,
+                  const int* rule_npts, const double* rule_xyzw, double w_cut, long long capacity,
+                  double* coords, double* weights, long long* npts)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || natm < 1 || !atoms || nshell < 0 || nrule < 1 || !rule_npts || !rule_xyzw ||
+            !npts || (nshell > 0 && (!shell_atom || !shell_r || !shell_w || !shell_rule)))
+            throw Error(WB_INVALID, "bad grid build arguments");
+        std::vector<int> rule_off(nrule + 1, 0);
+        for (int k = 0; k < nrule; ++k) {
+            if (rule_npts[k] < 1) throw Error(WB_INVALID, "empty angular rule");
+            rule_off[k + 1] = rule_off[k] + rule_npts[k];
+        }
+        std::vector<long long> first(std::max(nshell, 1) + 1, 0);
+        for (int sh = 0; sh < nshell; ++sh) {
+            if (shell_atom[sh] < 0 || shell_atom[sh] >= natm)
+                throw Error(WB_INVALID, "radial shell atom outside 0..natm-1");
+            if (shell_rule[sh] < 0 || shell_rule[sh] >= nrule)
+                throw Error(WB_INVALID, "radial shell rule outside 0..nrule-1");
+            first[sh + 1] = first[sh] + rule_npts[shell_rule[sh]];
+        }
+        const long long nraw = nshell > 0 ? first[nshell] : 0;
+        if (nraw > INT32_MAX) throw Error(WB_UNSUPPORTED, "more than 2^31 raw grid points");
+        std::vector<double> rab((size_t)natm * natm), irab((size_t)natm * natm);
+        for (int a = 0; a < natm; ++a)
+            for (int b = 0; b < natm; ++b) {
+                const double dx = atoms[3 * a] - atoms[3 * b], dy = atoms[3 * a + 1] - atoms[3 * b + 1],
+                             dz = atoms[3 * a + 2] - atoms[3 * b + 2];
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (a != b && r == 0.0) throw Error(WB_INVALID, "two atoms at one position");
+                rab[(size_t)a * natm + b] = r;
+                irab[(size_t)a * natm + b] = a == b ? 0.0 : 1.0 / r;
+            }
+        WB_HIP(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        DevBuf dA, dR, dI, dF, dSA, dSR, dSW, dSK, dRO, dRX, dX, dO, dWp, dP, dB, dXo, dWo;
+        upload(dA, atoms, sizeof(double) * 3 * natm, s);
+        upload(dR, rab.data(), sizeof(double) * rab.size(), s);
+        upload(dI, irab.data(), sizeof(double) * irab.size(), s);
+        upload(dRO, rule_off.data(), sizeof(int) * rule_off.size(), s);
+        upload(dRX, rule_xyzw, sizeof(double) * 4 * (size_t)rule_off[nrule], s);
+        const size_t n = (size_t)std::max<long long>(nraw, 1);
+        if (nshell > 0) {
+            upload(dF, first.data(), sizeof(long long) * (nshell + 1), s);
+            upload(dSA, shell_atom, sizeof(int) * nshell, s);
+            upload(dSR, shell_r, sizeof(double) * nshell, s);
+            upload(dSW, shell_w, sizeof(double) * nshell, s);
+            upload(dSK, shell_rule, sizeof(int) * nshell, s);
+        }
+        dX.ensure(sizeof(double) * 3 * n);
+        dO.ensure(sizeof(int) * n);
+        dWp.ensure(sizeof(double) * n);
+        dP.ensure(sizeof(double) * n);
+        dB.ensure(sizeof(int) * ((size_t)grid_keep_blocks(nraw) + 1));
+        dXo.ensure(sizeof(double) * 3 * n);
+        dWo.ensure(sizeof(double) * n);
+        launch_grid_points(s, nraw, nshell, dF.as<long long>(), dSA.as<int>(), dSR.as<double>(),
+                           dSW.as<double>(), dSK.as<int>(), dRO.as<int>(), dRX.as<double>(),
+                           dA.as<double>(), dX.as<double>(), dO.as<int>(), dWp.as<double>());
+        launch_ssf_partition(s, (int)nraw, dX.as<double>(), dO.as<int>(), natm, dA.as<double>(),
+                             dR.as<double>(), dI.as<double>(), dP.as<double>());
+        launch_grid_keep(s, nraw, dX.as<double>(), dWp.as<double>(), dP.as<double>(), w_cut,
+                         dB.as<int>(), dXo.as<double>(), dWo.as<double>());
+        int kept = 0;
+        WB_HIP(hipMemcpyAsync(&kept, dB.as<int>() + grid_keep_blocks(nraw), sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        WB_HIP(hipStreamSynchronize(s));
+        *npts = kept;
+        if (kept > capacity)
+            throw Error(WB_INVALID, "grid build: " + std::to_string(kept) +
+                                        " points kept, more than the capacity " +
+                                        std::to_string(capacity));
+        if (kept > 0) {
+            if (!coords || !weights) throw Error(WB_INVALID, "null output");
+            WB_HIP(hipMemcpyAsync(coords, dXo.p, sizeof(double) * 3 * (size_t)kept,
+                                  hipMemcpyDeviceToHost, s));
+            WB_HIP(hipMemcpyAsync(weights, dWo.p, sizeof(double) * (size_t)kept,
+                                  hipMemcpyDeviceToHost, s));
+            WB_HIP(hipStreamSynchronize(s));
+        }
+    })
+}
+
 int wb_test_pair_bounds(int nshell, const int* l, const int* nprim, const double* center,
                         const double* exps, const double* coefs, double pair_cut,
                         long long* counts2, double* pairs, double* prims)

@@ -319,6 +319,18 @@ void launch_xc_weights(hipStream_t s, int Ppad, const double* w, const double* v
 void launch_ssf_partition(hipStream_t s, int npts, const double* xyz, const int* owner, int natm,
                           const double* atoms, const double* rab, const double* inv_rab,
                           double* out);
+// atom-centred raw points (xyz [n][3], owner atom, radial x angular weight)
+// from per-shell radial data and angular rules (x, y, z, w per point)
+void launch_grid_points(hipStream_t s, long long nraw, int nshell, const long long* shell_first,
+                        const int* shell_atom, const double* shell_r, const double* shell_w,
+                        const int* shell_rule, const int* rule_off, const double* rule_xyzw,
+                        const double* atoms, double* xyz, int* owner, double* wpre);
+// the points whose weight wpre x part exceeds w_cut, in order: xyz_out,
+// w_out; bcount: grid_keep_blocks(n) + 1 ints, the kept count at the end
+int grid_keep_blocks(long long n);
+void launch_grid_keep(hipStream_t s, long long n, const double* xyz, const double* wpre,
+                      const double* part, double w_cut, int* bcount, double* xyz_out,
+                      double* w_out);
 
 // VV10
 // VV10 double sum over n = *nptr <= ncap compacted points.  pts: (x, y, z,

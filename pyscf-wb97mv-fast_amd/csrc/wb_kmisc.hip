@@ -1441,6 +1441,140 @@ void launch_ssf_partition(hipStream_t s, int npts, const double* xyz, const int*
     WB_HIP(hipGetLastError());
 }
 
+// Atom-centred points on the device: raw point i lies on radial shell sh
+// (shell_first[sh] <= i < shell_first[sh + 1]) of atom shell_atom[sh], at
+// atom + r u_k with u_k, wa_k the k-th point of the shell's angular rule;
+// its weight before the cell partition is wr wa_k.  Products and sums are
+// rounded one by one (no contraction into fused multiply-adds), as a host
+// build computes them.
+__global__ __launch_bounds__(256) void grid_points_kernel(
+    long long nraw, int nshell, const long long* __restrict__ shell_first,
+    const int* __restrict__ shell_atom, const double* __restrict__ shell_r,
+    const double* __restrict__ shell_w, const int* __restrict__ shell_rule,
+    const int* __restrict__ rule_off, const double* __restrict__ rule_xyzw,
+    const double* __restrict__ atoms, double* __restrict__ xyz, int* __restrict__ owner,
+    double* __restrict__ wpre)
+{
+#pragma clang fp contract(off)
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nraw) return;
+    int lo = 0, hi = nshell - 1;                      // last shell with first <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (shell_first[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const int a = shell_atom[lo];
+    const double r = shell_r[lo];
+    const double* u = rule_xyzw + 4 * ((size_t)rule_off[shell_rule[lo]] + (size_t)(i - shell_first[lo]));
+    xyz[3 * i] = atoms[3 * a] + r * u[0];
+    xyz[3 * i + 1] = atoms[3 * a + 1] + r * u[1];
+    xyz[3 * i + 2] = atoms[3 * a + 2] + r * u[2];
+    owner[i] = a;
+    wpre[i] = shell_w[lo] * u[3];
+}
+
+// Order-preserving compaction of the points whose weight wpre x part is
+// above w_cut, in three passes: kept points per block of 256, an exclusive
+// scan of the block counts (one block), the scatter.
+constexpr int GB_BLOCK = 256;
+__global__ __launch_bounds__(GB_BLOCK) void grid_keep_count_kernel(
+    long long n, const double* __restrict__ wpre, const double* __restrict__ part, double w_cut,
+    int* __restrict__ bcount)
+{
+    __shared__ int s_cnt[GB_BLOCK / 64];
+    const long long i = (long long)blockIdx.x * GB_BLOCK + threadIdx.x;
+    const bool keep = i < n && __dmul_rn(wpre[i], part[i]) > w_cut;
+    const unsigned long long m = __ballot(keep);
+    if ((threadIdx.x & 63) == 0) s_cnt[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < GB_BLOCK / 64; ++k) t += s_cnt[k];
+        bcount[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of nb block counts in place, the total at bcount[nb]
+__global__ __launch_bounds__(1024) void grid_keep_scan_kernel(int nb, int* __restrict__ bcount)
+{
+    __shared__ int s_part[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per, b1 = min(nb, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += bcount[b];
+    s_part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {        // inclusive scan of the thread sums
+        const int v = t >= off ? s_part[t - off] : 0;
+        __syncthreads();
+        s_part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? s_part[t - 1] : 0;
+    for (int b = b0; b < b1; ++b) {
+        const int c = bcount[b];
+        bcount[b] = run;
+        run += c;
+    }
+    if (t == 1023) bcount[nb] = s_part[1023];
+}
+
+__global__ __launch_bounds__(GB_BLOCK) void grid_keep_scatter_kernel(
+    long long n, const double* __restrict__ xyz, const double* __restrict__ wpre,
+    const double* __restrict__ part, double w_cut, const int* __restrict__ boff,
+    double* __restrict__ xyz_out, double* __restrict__ w_out)
+{
+    __shared__ int s_cnt[GB_BLOCK / 64];
+    const long long i = (long long)blockIdx.x * GB_BLOCK + threadIdx.x;
+    const double w = i < n ? __dmul_rn(wpre[i], part[i]) : 0.0;
+    const bool keep = i < n && w > w_cut;
+    const unsigned long long m = __ballot(keep);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) s_cnt[wv] = __popcll(m);
+    __syncthreads();
+    if (!keep) return;
+    int pos = boff[blockIdx.x];
+    for (int k = 0; k < wv; ++k) pos += s_cnt[k];
+    pos += __popcll(m & ((1ull << lane) - 1ull));
+    xyz_out[3 * (size_t)pos] = xyz[3 * i];
+    xyz_out[3 * (size_t)pos + 1] = xyz[3 * i + 1];
+    xyz_out[3 * (size_t)pos + 2] = xyz[3 * i + 2];
+    w_out[pos] = w;
+}
+
+void launch_grid_points(hipStream_t s, long long nraw, int nshell, const long long* shell_first,
+                        const int* shell_atom, const double* shell_r, const double* shell_w,
+                        const int* shell_rule, const int* rule_off, const double* rule_xyzw,
+                        const double* atoms, double* xyz, int* owner, double* wpre)
+{
+    if (nraw > 0)
+        hipLaunchKernelGGL(grid_points_kernel, dim3((unsigned)((nraw + 255) / 256)), dim3(256), 0,
+                           s, nraw, nshell, shell_first, shell_atom, shell_r, shell_w, shell_rule,
+                           rule_off, rule_xyzw, atoms, xyz, owner, wpre);
+    WB_HIP(hipGetLastError());
+}
+
+int grid_keep_blocks(long long n) { return (int)((n + GB_BLOCK - 1) / GB_BLOCK); }
+
+void launch_grid_keep(hipStream_t s, long long n, const double* xyz, const double* wpre,
+                      const double* part, double w_cut, int* bcount /* blocks + 1 */,
+                      double* xyz_out, double* w_out)
+{
+    const int nb = grid_keep_blocks(n);
+    if (nb == 0) {
+        zero_bytes(s, bcount, sizeof(int));
+        return;
+    }
+    hipLaunchKernelGGL(grid_keep_count_kernel, dim3(nb), dim3(GB_BLOCK), 0, s, n, wpre, part,
+                       w_cut, bcount);
+    hipLaunchKernelGGL(grid_keep_scan_kernel, dim3(1), dim3(1024), 0, s, nb, bcount);
+    hipLaunchKernelGGL(grid_keep_scatter_kernel, dim3(nb), dim3(GB_BLOCK), 0, s, n, xyz, wpre,
+                       part, w_cut, bcount, xyz_out, w_out);
+    WB_HIP(hipGetLastError());
+}
+
 // wv = (w v0 / 2, w v1, w v2, w v3, w v4 / 2) from the chain-ruled potential
 // v [5][Ppad]: the operand factors of the semilocal potential matrix
 __global__ void xc_weights_kernel(int Ppad, const double* __restrict__ w,

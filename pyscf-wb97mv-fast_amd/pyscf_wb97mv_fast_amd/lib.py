@@ -70,6 +70,8 @@ _SIGS = {
     'wb_grid_partition': (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp]),
     'wb_grid_overlap': (_i, [_vp, _vp, _vp, _vp]),
     'wb_pjs_projector': (_i, [_vp, _vp, _vp, _vp, _vp]),
+    'wb_grid_build': (_i, [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _d,
+                           ctypes.c_longlong, _vp, _vp, ctypes.POINTER(ctypes.c_longlong)]),
     'wb_cycle_batch': (_i, [_vp, _i, ctypes.POINTER(CycleJob), _i, _vp, _d, _d, _d, _d, _i, _i,
                             _vp]),
     'wb_cycle_queue': (_i, [_vp, _i, ctypes.POINTER(CycleJob), _i, _vp, _d, _d, _d, _d, _i,

@@ -13,8 +13,8 @@ this builds its own grid of the same construction:
   (the NWChem scheme's idea);
 * Stratmann-Scuseria-Frisch partition (Becke cells with compact support);
   points with weight below ``w_cut`` are dropped.  ``Grids(..., device=ctx)``
-  computes the partition on the GPU (gpu/grids.py); the host version here is
-  the default and the test reference.
+  builds the points, the partition and the cut on the GPU (gpu/grids.py);
+  the host version here is the default and the test reference.
 
 Radial/angular sizes per level follow PySCF's ``RAD_GRIDS``/``ANG_ORDER``
 tables for periods 1-3.  ``Grids`` mirrors the attributes the hot path reads
@@ -114,20 +114,13 @@ class Grids:
         self.coords = None
         self.weights = None
 
-    def raw_points(self):
-        """All atom-centred points before the partition: coords (n, 3),
-        radial x angular weights (n,), owner atom (n,)."""
-        mol = self.mol
-        atoms = mol.atom_coords()
-        charges = mol.atom_charges()
-        pts, wts, owner = [], [], []
-        ang_cache = {}
-
-        def ang(deg):
-            if deg not in ang_cache:
-                ang_cache[deg] = angular_product(deg)
-            return ang_cache[deg]
-
+    def shell_tables(self):
+        """The grid's radial shells and angular rules: per shell its atom,
+        radius, radial weight and angular rule index; the rules as
+        ((n, 3) unit vectors, (n,) weights), in first-use order."""
+        charges = self.mol.atom_charges()
+        sh_atom, sh_r, sh_w, sh_rule = [], [], [], []
+        rules, rule_of = [], {}
         for ia, z in enumerate(charges):
             per = _PERIOD[int(z)]
             nrad = _RAD[self.level][per]
@@ -143,20 +136,41 @@ class Grids:
                         d = min(deg, 7)
                     elif ri < 0.5 * rb:
                         d = min(deg, 11)
-                u, wa = ang(d)
-                pts.append(atoms[ia] + ri * u)
-                wts.append(wri * wa)
-                owner.append(numpy.full(len(wa), ia))
+                if d not in rule_of:
+                    rule_of[d] = len(rules)
+                    rules.append(angular_product(d))
+                sh_atom.append(ia)
+                sh_r.append(ri)
+                sh_w.append(wri)
+                sh_rule.append(rule_of[d])
+        return (numpy.asarray(sh_atom, dtype=numpy.int32), numpy.asarray(sh_r, dtype=float),
+                numpy.asarray(sh_w, dtype=float), numpy.asarray(sh_rule, dtype=numpy.int32), rules)
+
+    def raw_points(self):
+        """All atom-centred points before the partition: coords (n, 3),
+        radial x angular weights (n,), owner atom (n,)."""
+        atoms = self.mol.atom_coords()
+        sh_atom, sh_r, sh_w, sh_rule, rules = self.shell_tables()
+        pts, wts, owner = [], [], []
+        for ia, ri, wri, k in zip(sh_atom, sh_r, sh_w, sh_rule):
+            u, wa = rules[k]
+            pts.append(atoms[ia] + ri * u)
+            wts.append(wri * wa)
+            owner.append(numpy.full(len(wa), ia))
         return numpy.concatenate(pts), numpy.concatenate(wts), numpy.concatenate(owner)
 
     def build(self, **kwargs):
+        """coords and weights; with device=ctx the points, the partition and
+        the weight cut are made on the GPU (gpu.grids.build_grid), bit for
+        bit the host's points and cut with the device partition."""
         atoms = self.mol.atom_coords()
-        coords, w, owner = self.raw_points()
         if self.device is not None:
-            from ..gpu.grids import ssf_partition
-            w = w * ssf_partition(self.device, coords, atoms, owner)
-        else:
-            w = w * _ssf_partition(coords, atoms, owner)
+            from ..gpu.grids import build_grid
+            self.coords, self.weights = build_grid(self.device, atoms, *self.shell_tables(),
+                                                   w_cut=self.w_cut)
+            return self
+        coords, w, owner = self.raw_points()
+        w = w * _ssf_partition(coords, atoms, owner)
         keep = w > self.w_cut
         self.coords = numpy.ascontiguousarray(coords[keep])
         self.weights = numpy.ascontiguousarray(w[keep])

@@ -58,6 +58,40 @@ def test_device_grid_matches_host_grid(ctx):
     assert (numpy.abs(gd.weights - gh.weights) <= 1e-13 * wq).all()
 
 
+@pytest.mark.parametrize('case', ['water8_l1', 'drug10_l3'])
+def test_device_built_grid_equals_host_points_with_device_partition(ctx, case):
+    """wb_grid_build: points, partition and cut on the device equal the host's
+    points and cut with the device partition (wb_grid_partition), bit for
+    bit, in the host's order."""
+    if case == 'water8_l1':
+        mol, level = mole.Mole(geometry.water_cluster(8, seed=0), 'def2-svp'), 1
+    else:
+        mol, level = mole.Mole(geometry.drug_like(10, seed=2), 'def2-tzvp'), 3
+    g = grids.Grids(mol, level=level)
+    coords, wq, owner = g.raw_points()
+    w = wq * dgrids.ssf_partition(ctx, coords, mol.atom_coords(), owner)
+    keep = w > g.w_cut
+    gd = grids.Grids(mol, level=level, device=ctx).build()
+    print(case, 'raw', len(coords), 'kept', gd.size, flush=True)
+    assert gd.size == int(keep.sum()) and gd.size < len(coords)
+    assert numpy.array_equal(gd.coords, coords[keep])
+    assert numpy.array_equal(gd.weights, w[keep])
+
+
+def test_device_grid_build_edges(ctx):
+    mol = mole.Mole(geometry.WATER_DIMER, 'def2-svp')
+    t = grids.Grids(mol, level=0).shell_tables()
+    atoms = mol.atom_coords()
+    c, w = dgrids.build_grid(ctx, atoms, *t, w_cut=1e300)          # everything cut
+    assert c.shape == (0, 3) and w.shape == (0,)
+    c, w = dgrids.build_grid(ctx, atoms, t[0][:0], t[1][:0], t[2][:0], t[3][:0], t[4])
+    assert c.shape == (0, 3)
+    bad = (t[0].copy(),) + t[1:]
+    bad[0][0] = len(atoms)
+    with pytest.raises(GpuError):
+        dgrids.build_grid(ctx, atoms, *bad)
+
+
 def test_partition_edge_cases(ctx):
     atoms = numpy.array([[0.0, 0, 0], [1.5, 0, 0]])
     assert dgrids.ssf_partition(ctx, numpy.zeros((0, 3)), atoms, []).shape == (0,)
