@@ -158,6 +158,22 @@ def k_build(basis, coords, weights, QD, omega=0.0, pair_cut=0.0):
     return K
 
 
+def k_build_screened(basis, coords, weights, QD, omega=0.0, pair_cut=0.0, tol=1e-11):
+    """k_build with per-point pair screening (the CPU baseline's cost model of
+    an SGX build's screening; not a rigorous bound): returns (K, pair
+    evaluations kept)."""
+    coords = numpy.ascontiguousarray(coords, dtype=numpy.float64).reshape(-1, 3)
+    weights = numpy.ascontiguousarray(weights, dtype=numpy.float64)
+    QD = numpy.ascontiguousarray(QD, dtype=numpy.float64)
+    K = numpy.empty((basis.nao, basis.nao))
+    kept = ctypes.c_longlong()
+    lib().orc_k_build_screened(basis.cref, ctypes.c_int(len(coords)), _ptr(coords),
+                               _ptr(weights), _ptr(QD), ctypes.c_double(omega),
+                               ctypes.c_double(pair_cut), ctypes.c_double(tol), _ptr(K),
+                               ctypes.byref(kept))
+    return K, int(kept.value)
+
+
 def get_k_only(basis, coords, weights, dm, Q=None, sym_ovlp=False, omega=0.0,
                pair_cut=0.0):
     """sgx_jk.get_k_only's contract: QD = Q dm, K = X^T(w o A(X QD)),

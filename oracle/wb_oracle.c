@@ -363,6 +363,143 @@ void orc_k_build(const orc_basis *bs, int npts, const double *coords,
     free(pa); free(pb);
 }
 
+/* Per-point screened K build, the CPU baseline's form of the (sub-block,
+ * shell pair) screening of an SGX build: at point g the pair (A, B) is
+ * skipped when w_g max(|F_A|, |F_B|) est(A, B, g) < tol.  est is an s-type
+ * magnitude of the pair's ESP block from data kept per pair: c, the centre
+ * of its most diffuse primitive pair, delta, the largest distance of a
+ * primitive-pair centre from c, p_min, beta = sum_ij |c_i c_j|
+ * exp(-mu_ij |AB|^2) 2 pi / p_ij, and the largest |P_ij A|, |P_ij B|:
+ *   est = beta F_0(p_min max(0, |g c| - delta)^2)
+ *         x (1 + |PA| + |g c| + delta)^la (1 + |PB| + |g c| + delta)^lb
+ * (|erf(w r)/r| <= 1/r, so the 1/r form serves both operators; |F_A| the
+ * largest |F| over shell A's AOs).  A cost model for timing, not a rigorous
+ * bound; *kept counts the (point, pair) evaluations done. */
+static double f0_bound(double T)
+{
+    return T < 1e-12 ? 1.0 : 0.5 * sqrt(M_PI / T) * erf(sqrt(T));
+}
+
+typedef struct { double c[3], delta, pmin, beta, ra, rb; } pair_scr;
+
+static void pair_screen(const orc_basis *bs, int sa, int sb, pair_scr *ps)
+{
+    const double *A = bs->center + 3 * sa, *B = bs->center + 3 * sb;
+    const double ab2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1])
+                     + (A[2] - B[2]) * (A[2] - B[2]);
+    ps->pmin = 1e300;
+    ps->beta = ps->delta = ps->ra = ps->rb = 0.0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < bs->nprim[sa]; ++i)
+            for (int j = 0; j < bs->nprim[sb]; ++j) {
+                const double a = bs->exps[bs->prim0[sa] + i], b = bs->exps[bs->prim0[sb] + j];
+                const double p = a + b;
+                double P[3], pa = 0, pb = 0, pc = 0;
+                for (int d = 0; d < 3; ++d) {
+                    P[d] = (a * A[d] + b * B[d]) / p;
+                    pa += (P[d] - A[d]) * (P[d] - A[d]);
+                    pb += (P[d] - B[d]) * (P[d] - B[d]);
+                }
+                if (pass == 0) {                       /* the most diffuse pair's centre */
+                    if (p < ps->pmin) {
+                        ps->pmin = p;
+                        for (int d = 0; d < 3; ++d) ps->c[d] = P[d];
+                    }
+                    continue;
+                }
+                for (int d = 0; d < 3; ++d) pc += (P[d] - ps->c[d]) * (P[d] - ps->c[d]);
+                if (sqrt(pc) > ps->delta) ps->delta = sqrt(pc);
+                if (sqrt(pa) > ps->ra) ps->ra = sqrt(pa);
+                if (sqrt(pb) > ps->rb) ps->rb = sqrt(pb);
+                ps->beta += fabs(bs->coefs[bs->prim0[sa] + i] * bs->coefs[bs->prim0[sb] + j])
+                          * exp(-a * b / p * ab2) * (2.0 * M_PI / p);
+            }
+}
+
+static double pair_est(const pair_scr *ps, int la, int lb, const double *g)
+{
+    const double dx = g[0] - ps->c[0], dy = g[1] - ps->c[1], dz = g[2] - ps->c[2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double e = d > ps->delta ? d - ps->delta : 0.0;
+    double v = ps->beta * f0_bound(ps->pmin * e * e);
+    const double fa = 1.0 + ps->ra + d + ps->delta, fb = 1.0 + ps->rb + d + ps->delta;
+    for (int i = 0; i < la; ++i) v *= fa;
+    for (int i = 0; i < lb; ++i) v *= fb;
+    return v;
+}
+
+void orc_k_build_screened(const orc_basis *bs, int npts, const double *coords,
+                          const double *weights, const double *QD, double omega,
+                          double pair_cut, double tol, double *K, long long *kept)
+{
+    const int nao = bs->nao, ns = bs->nshell;
+    int *pa = malloc(sizeof(int) * ns * (ns + 1) / 2);
+    int *pb = malloc(sizeof(int) * ns * (ns + 1) / 2);
+    int np = 0;
+    for (int sa = 0; sa < ns; ++sa)
+        for (int sb = 0; sb <= sa; ++sb)
+            if (block_bound(bs, sa, sb) > pair_cut) { pa[np] = sa; pb[np] = sb; ++np; }
+    pair_scr *scr = malloc(sizeof(pair_scr) * (np > 0 ? np : 1));
+    for (int k = 0; k < np; ++k) pair_screen(bs, pa[k], pb[k], scr + k);
+    memset(K, 0, sizeof(double) * nao * nao);
+    long long nk = 0;
+#pragma omp parallel reduction(+ : nk)
+    {
+        double *Kp = calloc((size_t)nao * nao, sizeof(double));
+        double *X = malloc(sizeof(double) * nao);
+        double *F = malloc(sizeof(double) * nao);
+        double *G = malloc(sizeof(double) * nao);
+        double *fs = malloc(sizeof(double) * ns);
+        double blk[49];
+#pragma omp for schedule(dynamic, 4)
+        for (int g = 0; g < npts; ++g) {
+            const double *r = coords + 3 * g;
+            const double w = weights[g];
+            ao_point(bs, r, 0, X, 0);
+            for (int l = 0; l < nao; ++l) {
+                double s = 0.0;
+                for (int k = 0; k < nao; ++k) s += X[k] * QD[(size_t)k * nao + l];
+                F[l] = s;
+            }
+            for (int sh = 0; sh < ns; ++sh) {
+                double m = 0.0;
+                for (int i = 0; i < 2 * bs->l[sh] + 1; ++i) {
+                    const double v = fabs(F[bs->ao0[sh] + i]);
+                    if (v > m) m = v;
+                }
+                fs[sh] = fabs(w) * m;
+            }
+            memset(G, 0, sizeof(double) * nao);
+            for (int k = 0; k < np; ++k) {
+                const int sa = pa[k], sb = pb[k];
+                const double fm = fs[sa] > fs[sb] ? fs[sa] : fs[sb];
+                if (fm * pair_est(scr + k, bs->l[sa], bs->l[sb], r) < tol) continue;
+                ++nk;
+                esp_block(bs, sa, sb, r, omega, blk);
+                const int nsa = 2 * bs->l[sa] + 1, nsb = 2 * bs->l[sb] + 1;
+                const int a0 = bs->ao0[sa], b0 = bs->ao0[sb];
+                for (int m = 0; m < nsa; ++m)
+                    for (int n = 0; n < nsb; ++n) {
+                        const double v = blk[m * nsb + n];
+                        G[b0 + n] += v * F[a0 + m];
+                        if (sa != sb) G[a0 + m] += v * F[b0 + n];
+                    }
+            }
+            for (int m = 0; m < nao; ++m) {
+                const double xw = X[m] * w;
+                if (xw == 0.0) continue;
+                double *row = Kp + (size_t)m * nao;
+                for (int n = 0; n < nao; ++n) row[n] += xw * G[n];
+            }
+        }
+#pragma omp critical
+        for (size_t i = 0; i < (size_t)nao * nao; ++i) K[i] += Kp[i];
+        free(Kp); free(X); free(F); free(G); free(fs);
+    }
+    *kept = nk;
+    free(pa); free(pb); free(scr);
+}
+
 /* ------------------------------------------------------------ VV10 ----- */
 void orc_vv10_sum_outer(int n, const double *coords, const double *q, const double *W0,
                         const double *kappa, int nout, const int *outer, double *F,

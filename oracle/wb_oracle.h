@@ -31,6 +31,9 @@ void orc_ao_eval(const orc_basis *bs, int npts, const double *coords, int deriv,
 void orc_k_build(const orc_basis *bs, int npts, const double *coords,
                  const double *weights, const double *QD, double omega,
                  double pair_cut, double *K);
+void orc_k_build_screened(const orc_basis *bs, int npts, const double *coords,
+                          const double *weights, const double *QD, double omega,
+                          double pair_cut, double tol, double *K, long long *kept);
 /* VV10 kernel sums: F = -1.5 sum T, U, W as libdft returns them */
 void orc_vv10_sum(int n, const double *coords, const double *q, const double *W0,
                   const double *kappa, double *F, double *U, double *W);

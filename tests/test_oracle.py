@@ -191,3 +191,27 @@ def test_k_build_vs_dense_contraction(omega):
     G = numpy.einsum('gmn,gm->gn', A, F)
     ref = X.T @ (w[:, None] * G)
     assert numpy.allclose(K, ref, atol=1e-13 * numpy.abs(ref).max())
+
+
+def test_screened_oracle_k_is_the_unscreened_within_the_tolerance():
+    """The CPU baseline's screened K build (per-point pair skipping at the
+    device build's 1e-11) leaves out only what the unscreened build adds
+    below that scale, and skips a large share of the pair evaluations."""
+    import oracle
+    from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
+    mol = mole.Mole(geometry.water_cluster(4, seed=1), 'def2-svp')
+    g = grids.Grids(mol, level=0).build()
+    dm = guess.huckel_density(mol)
+    ob = oracle.OracleBasis.from_mol(mol)
+    sel = numpy.random.default_rng(3).choice(g.size, 64, replace=False)
+    for omega in (0.0, 0.3):
+        K0 = oracle.k_build(ob, g.coords[sel], g.weights[sel], dm, omega, pair_cut=1e-14)
+        K1, kept = oracle.k_build_screened(ob, g.coords[sel], g.weights[sel], dm, omega,
+                                           pair_cut=1e-14, tol=1e-11)
+        npairs = mol.nbas * (mol.nbas + 1) // 2
+        assert 0 < kept < 0.8 * npairs * len(sel), kept
+        assert numpy.abs(K1 - K0).max() <= 1e-8 * numpy.abs(K0).max()
+        # no skipping at tol 0
+        K2, kept2 = oracle.k_build_screened(ob, g.coords[sel], g.weights[sel], dm, omega,
+                                            pair_cut=1e-14, tol=0.0)
+        assert kept2 > kept and numpy.allclose(K2, K0, rtol=0, atol=1e-13 * numpy.abs(K0).max())
