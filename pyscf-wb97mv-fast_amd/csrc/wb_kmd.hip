@@ -139,13 +139,21 @@ __device__ __forceinline__ const real* md_stage_boys(const real* __restrict__ bo
 }
 
 // One wave's work in one class launch: sub-block s (one point per lane) and
-// chunk j of s's task list, of the molecule a describes (MdJob: its F, G
-// copies, task lists and records).  real: integral arithmetic; gacc: G
-// accumulation (registers and memory); NOP operators at once (G of operator
-// o at G + o opstride).  The single-molecule kernel and the many-molecule
-// queue kernel run exactly this.
+// chunk j of s's task list, of one molecule (its F, G copies, task lists and
+// records).  real: integral arithmetic; gacc: G accumulation (registers and
+// memory); NOP operators at once (G of operator o at G + o opstride).  The
+// single-molecule kernel and the many-molecule queue kernel run exactly this.
+// The pointers are __restrict__ parameters (the qualifier on local variables
+// is ignored).
 template <int LA, int LB, int NOP, typename real, typename gacc>
-__device__ __forceinline__ void md_wave(const MdJob& a, const real* __restrict__ tab, int s, int j)
+__device__ __forceinline__ void md_wave(
+    const double* __restrict__ xyz, const real* __restrict__ F, gacc* __restrict__ G,
+    const size_t gstride, const size_t opstride, const int* __restrict__ counts,
+    const int* __restrict__ ids, const unsigned* __restrict__ skips, const int cap,
+    const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
+    const char* __restrict__ recs, const int Ppad, const int nsub, const int nchunk,
+    const int zero_rows, unsigned* __restrict__ masks, const size_t mopstride, const int zrow,
+    const int first, const real* __restrict__ tab, const int s, const int j)
 {
     constexpr int L = LA + LB;
     constexpr int NCA = (LA + 1) * (LA + 2) / 2, NCB = (LB + 1) * (LB + 2) / 2;
@@ -157,22 +165,6 @@ __device__ __forceinline__ void md_wave(const MdJob& a, const real* __restrict__
     // shells in the def2 sets and stay in the arithmetic type
     using vacc = typename std::conditional<(NCA * NCB <= 36), double, real>::type;
     (void)NSA; (void)NSB; (void)L; (void)F32;
-    const double* __restrict__ xyz = a.xyz;
-    const real* __restrict__ F = (const real*)a.F;
-    gacc* __restrict__ G = (gacc*)a.G;
-    const size_t gstride = a.gstride, opstride = a.opstride;
-    const int* __restrict__ counts = a.counts;
-    const int* __restrict__ ids = a.ids;
-    const unsigned* __restrict__ skips = a.skips;
-    const int cap = a.cap;
-    const PairGroup* __restrict__ groups = a.groups;
-    const int* __restrict__ bbao = a.bbao;
-    const char* __restrict__ recs = a.recs;
-    const int Ppad = a.Ppad, nsub = a.nsub, nchunk = a.nchunk, zero_rows = a.zero_rows;
-    unsigned* __restrict__ masks = a.masks;
-    const size_t mopstride = a.mopstride;
-    const int zrow = a.zrow, first = a.first;
-    (void)nchunk;
     const int lane = threadIdx.x & 63;
     gacc* const Gc = G + (size_t)j * gstride;         // operator o: Gc + o opstride
     const int g = s * SUB + lane;
@@ -422,18 +414,44 @@ __device__ __forceinline__ void md_wave(const MdJob& a, const real* __restrict__
     if (tracked) store_mask();
 }
 
-// one class, one molecule: worker = chunk x nsub + sub-block
+// md_wave on the molecule a describes (its pointers passed as the
+// __restrict__ parameters)
+template <int LA, int LB, int NOP, typename real, typename gacc>
+__device__ __forceinline__ void md_wave_job(const MdJob& a, const real* __restrict__ tab, int s,
+                                            int j)
+{
+    md_wave<LA, LB, NOP, real, gacc>(a.xyz, (const real*)a.F, (gacc*)a.G, a.gstride, a.opstride,
+                                     a.counts, a.ids, a.skips, a.cap, a.groups, a.bbao, a.recs,
+                                     a.Ppad, a.nsub, a.nchunk, a.zero_rows, a.masks, a.mopstride,
+                                     a.zrow, a.first, tab, s, j);
+}
+
+// one class, one molecule: worker = chunk x nsub + sub-block.  The pointers
+// are kernel arguments of their own (not fields of a by-value MdJob): with the
+// job as one struct argument the same body took 30-90 more VGPRs per class
+// (the (d,d) kernel one wave per SIMD instead of three) and the exchange
+// phase twice the time.
 template <int LA, int LB, int NOP, typename real, typename gacc>
 __global__ __launch_bounds__(64 * MD_WAVES)
-void md_k_kernel(MdJob a, const real* __restrict__ boys_g)
+void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
+                 gacc* __restrict__ G, size_t gstride, size_t opstride,
+                 const int* __restrict__ counts, const int* __restrict__ ids,
+                 const unsigned* __restrict__ skips, int cap,
+                 const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
+                 const char* __restrict__ recs,
+                 const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk,
+                 int zero_rows, unsigned* __restrict__ masks, size_t mopstride, int zrow,
+                 int first)
 {
     constexpr int TAB = sizeof(real) == 4 ? B32_NODES * b32_row(LA + LB) : 1;
     __shared__ __align__(16) real s_tab[TAB];
     const real* tab = md_stage_boys<LA + LB, real>(boys_g, s_tab);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int worker = blockIdx.x * MD_WAVES + wave;
-    if (worker >= a.nsub * a.nchunk) return;
-    md_wave<LA, LB, NOP, real, gacc>(a, tab, worker % a.nsub, worker / a.nsub);
+    if (worker >= nsub * nchunk) return;
+    md_wave<LA, LB, NOP, real, gacc>(xyz, F, G, gstride, opstride, counts, ids, skips, cap, groups,
+                                     bbao, recs, Ppad, nsub, nchunk, zero_rows, masks, mopstride,
+                                     zrow, first, tab, worker % nsub, worker / nsub);
 }
 
 // one class, many molecules: a persistent grid whose waves take work items
@@ -467,7 +485,7 @@ void md_queue_kernel(const MdJob* __restrict__ jobs, const long long* __restrict
         }
         const MdJob a = jobs[lo];
         const long long local = item - first[lo];
-        md_wave<LA, LB, NOP, real, gacc>(a, tab, (int)(local % a.nsub), (int)(local / a.nsub));
+        md_wave_job<LA, LB, NOP, real, gacc>(a, tab, (int)(local % a.nsub), (int)(local / a.nsub));
     }
 }
 
@@ -517,8 +535,12 @@ static void md_launch_t(hipStream_t s, const MdJob* j, const MdJob* jobs, const 
     if (j) {
         const long long workers = md_items(*j);
         const int blocks = (int)((workers + MD_WAVES - 1) / MD_WAVES);
+        const MdJob& a = *j;
         hipLaunchKernelGGL((md_k_kernel<LA, LB, NOP, real, gacc>), dim3(blocks),
-                           dim3(64 * MD_WAVES), 0, s, *j, (const real*)boys);
+                           dim3(64 * MD_WAVES), 0, s, a.xyz, (const real*)a.F, (gacc*)a.G,
+                           a.gstride, a.opstride, a.counts, a.ids, a.skips, a.cap, a.groups,
+                           a.bbao, a.recs, (const real*)boys, a.Ppad, a.nsub, a.nchunk,
+                           a.zero_rows, a.masks, a.mopstride, a.zrow, a.first);
     } else {
         auto kern = md_queue_kernel<LA, LB, NOP, real, gacc>;
         static const int resident = queue_blocks(kern);     // one device kind per process
