@@ -116,11 +116,10 @@ def gen_prim(la, lb, oc, part=None):
     def eidx(axis, i, j, t):
         return axis * naxis + (i * (lb + 1) + j) * (L + 1) + t
 
-    lines = ['template <typename real, typename vacc>',
+    lines = ['template <typename real, typename vacc, typename EP>',
              '__device__ __forceinline__ void md_prim_%d%d_%s%s(real dx, real dy, real dz,'
              % (la, lb, 'oc' if oc else 'tc', '' if part is None else '_p%d' % part),
-             '    const real* __restrict__ R0, const real* __restrict__ E,'
-             ' vacc* __restrict__ V)',
+             '    const real* __restrict__ R0, EP E, vacc* __restrict__ V)',
              '{']
     rl, R = gen_r(L)
     if part is not None:
@@ -259,23 +258,25 @@ def main(out_path):
              '// E-coefficient products.  The FP32 record holds the ERL leading values',
              '// and then their rounding remainders: E values of one-centre pairs are',
              '// the same for every copy of an element, so their rounding would bias',
-             '// K coherently; each product is formed with one rounding.',
-             'template <typename real, int ERL>',
-             '__device__ __forceinline__ real wb_efmac(const real* __restrict__ E, int k, real x,'
+             '// K coherently; each product is formed with one rounding.  EP: pointer',
+             '// to the record\'s E values, in whatever address space the caller reads',
+             '// them from.',
+             'template <typename real, int ERL, typename EP>',
+             '__device__ __forceinline__ real wb_efmac(EP E, int k, real x,'
              ' real acc) {',
              '    if constexpr (sizeof(real) == 4) return fma(E[k], x, fma(E[k + ERL], x, acc));',
              '    else return fma(E[k], x, acc);',
              '}',
-             'template <typename real, int ERL>',
-             '__device__ __forceinline__ real wb_emulc(const real* __restrict__ E, int k, real x) {',
+             'template <typename real, int ERL, typename EP>',
+             '__device__ __forceinline__ real wb_emulc(EP E, int k, real x) {',
              '    if constexpr (sizeof(real) == 4) return fma(E[k], x, E[k + ERL] * x);',
              '    else return E[k] * x;',
              '}',
-             'template <typename real, int ERL>',
-             '__device__ __forceinline__ real wb_efma(const real* __restrict__ E, int k, real x,'
+             'template <typename real, int ERL, typename EP>',
+             '__device__ __forceinline__ real wb_efma(EP E, int k, real x,'
              ' real acc) { return fma(E[k], x, acc); }',
-             'template <typename real, int ERL>',
-             '__device__ __forceinline__ real wb_emul(const real* __restrict__ E, int k, real x)'
+             'template <typename real, int ERL, typename EP>',
+             '__device__ __forceinline__ real wb_emul(EP E, int k, real x)'
              ' { return E[k] * x; }',
              '// the A Cartesian components of part P of a split class (see gen_md.py)',
              'template <int LA, int LB, int P> struct MdPart;',

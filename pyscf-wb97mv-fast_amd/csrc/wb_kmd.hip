@@ -24,6 +24,22 @@ namespace wb {
 
 template <int L> struct MdDispatch;
 
+// Read-only data that every lane of a wave reads at the same address (task
+// lists, pair groups, block offsets, primitive-pair records) is read through
+// the constant address space: the compiler then loads it with scalar loads
+// into SGPRs whatever it can prove about aliasing.  Without this, such loads
+// become vector loads whenever the pointers do not arrive as no-alias kernel
+// arguments (the many-molecule queue kernel reads them from a job table): the
+// values then occupy VGPRs, 30-90 more per class, and the kernels run at a
+// fraction of their occupancy.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_cptr(const T* p)
+{
+    return (cptr<T>)p;
+}
+
 // Classes whose Cartesian block is formed in two parts of the A components
 // (without z, with z), one pass of the primitive loop each (md_nparts,
 // generated): the large blocks otherwise leave one or two waves per SIMD.
@@ -43,7 +59,7 @@ template <int LA, int LB, int P> struct MdPartA {
 // split class (V holds that part's rows in MdPartA order)
 template <int LA, int LB, bool ONEC, int PART, typename real, typename vacc>
 __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R0,
-                                        const real* E, vacc* V)
+                                        cptr<real> E, vacc* V)
 {
 #define WB_MD_PRIM(a, b)                                                              \
     if constexpr (LA == a && LB == b) {                                               \
@@ -70,7 +86,7 @@ __device__ __forceinline__ void md_prim(real dx, real dy, real dz, const real* R
 // P - g, the E coefficients and every load; Vs holds NOP blocks)
 template <int LA, int LB, bool ONEC, int NOP, int PART, typename real, typename vacc>
 __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py, double pz,
-                                         const char* __restrict__ recs,
+                                         cptr<char> recs,
                                          const real* __restrict__ tab, vacc* Vs)
 {
     constexpr int L = LA + LB;
@@ -82,16 +98,16 @@ __device__ __forceinline__ void md_block(int pp0, int npp, double px, double py,
     // the first's (measured: exchange -1.2 %)
 #pragma unroll 2
     for (int m = pp0; m < pp0 + npp; ++m) {
-        const char* rec = recs + (size_t)m * RB;
-        const double* P = (const double*)rec;
-        const real* C = (const real*)(rec + 24);
+        const cptr<char> rec = recs + (size_t)m * RB;
+        const cptr<double> P = (cptr<double>)rec;
+        const cptr<real> C = (cptr<real>)(rec + 24);
         const real dx = (real)(P[0] - px);
         const real dy = (real)(P[1] - py);
         const real dz = (real)(P[2] - pz);
         const real r2 = fma(dx, dx, fma(dy, dy, dz * dz));
 #pragma unroll
         for (int o = 0; o < NOP; ++o) {
-            const real* Co = C + o * CRL;
+            const cptr<real> Co = C + o * CRL;
             real Fb[L + 1], R0[L + 1];
             if constexpr (F32) {
                 Boys<real>::template eval<L>(fmaf(Co[0], r2, Co[1] * r2), tab, Fb);
@@ -148,10 +164,10 @@ __device__ __forceinline__ const real* md_stage_boys(const real* __restrict__ bo
 template <int LA, int LB, int NOP, typename real, typename gacc>
 __device__ __forceinline__ void md_wave(
     const double* __restrict__ xyz, const real* __restrict__ F, gacc* __restrict__ G,
-    const size_t gstride, const size_t opstride, const int* __restrict__ counts,
-    const int* __restrict__ ids, const unsigned* __restrict__ skips, const int cap,
-    const PairGroup* __restrict__ groups, const int* __restrict__ bbao,
-    const char* __restrict__ recs, const int Ppad, const int nsub, const int nchunk,
+    const size_t gstride, const size_t opstride, const cptr<int> counts,
+    const cptr<int> ids, const cptr<unsigned> skips, const int cap,
+    const cptr<PairGroup> groups, const cptr<int> bbao,
+    const cptr<char> recs, const int Ppad, const int nsub, const int nchunk,
     const int zero_rows, unsigned* __restrict__ masks, const size_t mopstride, const int zrow,
     const int first, const real* __restrict__ tab, const int s, const int j)
 {
@@ -210,8 +226,8 @@ __device__ __forceinline__ void md_wave(
         if (tracked && first) store_mask();
         return;
     }
-    const int* tl = ids + (size_t)s * cap;
-    const unsigned* sl = skips + (size_t)s * cap;
+    const cptr<int> tl = ids + (size_t)s * cap;
+    const cptr<unsigned> sl = skips + (size_t)s * cap;
     const double px = xyz[g], py = xyz[Ppad + g], pz = xyz[2 * Ppad + g];
     // wait for the point's coordinates here, once: otherwise the first use
     // inside the item loop waits on the vector-memory counter in program
@@ -270,7 +286,7 @@ __device__ __forceinline__ void md_wave(
         for (int m = 0; m < NSI; ++m) fi_n[m] = F[(size_t)(x0 + m) * Ppad + g];
     }
     for (int t = t0; t < t1; ++t) {
-        const PairGroup* gr = groups + (item >> 8);
+        const cptr<PairGroup> gr = groups + (item >> 8);
         const unsigned mask = (unsigned)item & 0xffu;
         const unsigned skw = sl[t];                          // trailing primitive pairs skipped
         const int bblk = gr->bblk;
@@ -287,7 +303,7 @@ __device__ __forceinline__ void md_wave(
         int item_n = 0;
         if (t + 1 < t1) {                                    // uniform
             item_n = tl[t + 1];
-            const PairGroup* gn = groups + (item_n >> 8);
+            const cptr<PairGroup> gn = groups + (item_n >> 8);
             const int x0n = gn->x0;
 #pragma unroll
             for (int m = 0; m < NSI; ++m) fi_n[m] = F[(size_t)(x0n + m) * Ppad + g];
@@ -421,7 +437,8 @@ __device__ __forceinline__ void md_wave_job(const MdJob& a, const real* __restri
                                             int j)
 {
     md_wave<LA, LB, NOP, real, gacc>(a.xyz, (const real*)a.F, (gacc*)a.G, a.gstride, a.opstride,
-                                     a.counts, a.ids, a.skips, a.cap, a.groups, a.bbao, a.recs,
+                                     as_cptr(a.counts), as_cptr(a.ids), as_cptr(a.skips), a.cap,
+                                     as_cptr(a.groups), as_cptr(a.bbao), as_cptr(a.recs),
                                      a.Ppad, a.nsub, a.nchunk, a.zero_rows, a.masks, a.mopstride,
                                      a.zrow, a.first, tab, s, j);
 }
@@ -449,9 +466,10 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int worker = blockIdx.x * MD_WAVES + wave;
     if (worker >= nsub * nchunk) return;
-    md_wave<LA, LB, NOP, real, gacc>(xyz, F, G, gstride, opstride, counts, ids, skips, cap, groups,
-                                     bbao, recs, Ppad, nsub, nchunk, zero_rows, masks, mopstride,
-                                     zrow, first, tab, worker % nsub, worker / nsub);
+    md_wave<LA, LB, NOP, real, gacc>(xyz, F, G, gstride, opstride, as_cptr(counts), as_cptr(ids),
+                                     as_cptr(skips), cap, as_cptr(groups), as_cptr(bbao),
+                                     as_cptr(recs), Ppad, nsub, nchunk, zero_rows, masks,
+                                     mopstride, zrow, first, tab, worker % nsub, worker / nsub);
 }
 
 // one class, many molecules: a persistent grid whose waves take work items
