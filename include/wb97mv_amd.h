@@ -314,6 +314,15 @@ int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, cons
                 const double* kappa, double tile_tol, int precision, double* F, double* U,
                 double* W);
 
+/* Tile pruning of the last VV10 sum run through ctx (wb_vv10_sum, or the
+ * VV10 flow of wb_nlc_vxc / wb_cycle): out2 = {(wave, tile) pairs tested
+ * against the pruning bound, pairs skipped}; both 0 when the sum ran with
+ * tile_tol = 0 (nothing is tested).  Waits for the context's stream.  The
+ * reference prunes tile pairs on the host before its launch and reports
+ * nothing (gpu/vv10.py:147-164, :179-221); this exposes what the device
+ * skipped. */
+int wb_vv10_stats(wb_ctx_t ctx, long long* out2);
+
 /* AO values (deriv 0) or values + gradients (deriv 1), FP64, out
  * (1 + 3 deriv, npts, nao) as PySCF's eval_gto (gpu/ao_eval.py:186-222). */
 int wb_ao_eval(wb_ctx_t ctx, wb_basis_t basis, int npts, const double* coords, int deriv,

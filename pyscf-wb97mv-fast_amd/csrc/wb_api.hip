@@ -335,11 +335,15 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
         ClassData& C = B.cls[class_id(A.l, Bs.l)];
         PairRec pr = o.pr;
         pr.pp0 = C.nprims;
-        if (pr.shA >= 32768 || pr.shB >= 32768 || pr.pp0 >= (1 << 25) || pr.npp >= 64)
-            throw Error(WB_UNSUPPORTED, "basis too large for the packed screening records");
+        // packed: pp = pp0 << 9 | npp (npp <= 256: up to 16 x 16 primitives,
+        // the reference's envelope), pp0 < 2^22 per class
+        if (pr.shA >= 32768 || pr.shB >= 32768 || pr.pp0 >= (1 << 22) || pr.npp > 256)
+            throw Error(WB_UNSUPPORTED, "shell pair outside the packed screening records (more "
+                                        "than 256 primitive pairs in one pair, or 2^22 in its "
+                                        "class)");
         PairScreen ps = o.ps;
         ps.sh = pr.shA | pr.shB << 16;
-        ps.pp = pr.pp0 << 6 | pr.npp;
+        ps.pp = pr.pp0 << 9 | pr.npp;
         C.P.insert(C.P.end(), o.P.begin(), o.P.end());
         C.p.insert(C.p.end(), o.p.begin(), o.p.end());
         C.pref.insert(C.pref.end(), o.pref.begin(), o.pref.end());
@@ -396,10 +400,11 @@ static void build_basis(Basis& B, int nshell, const int* l, const int* nprim,
                 C.groups.push_back(gr);
             }
             const PairRec& pr = pairs[i];
-            if (pr.pp0 >= (1 << 23) || pr.npp >= 64)
+            if (pr.pp0 >= (1 << 22) || pr.npp < 1 || pr.npp > 256)
                 throw Error(WB_UNSUPPORTED, "shell-pair class too large for the packed task words");
             C.groups.back().pidx[pos] = i;
-            C.groups.back().pw[pos] = pr.pp0 << 8 | pr.npp << 2 | (pr.onec ? 2 : 0) | (pr.diag ? 1 : 0);
+            C.groups.back().pw[pos] = (int)((unsigned)pr.pp0 << 10 | (unsigned)(pr.npp - 1) << 2 |
+                                            (pr.onec ? 2u : 0u) | (pr.diag ? 1u : 0u));
             C.groups.back().npair++;
         }
         C.pairs.swap(pairs);
@@ -1037,6 +1042,7 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     double* wv = (double*)c.ws_vv.ensure(sizeof(double) * 5 * Ppad);
     vv10_flow(s, G.nsub, Ppad, G.d_xyz.as<double>(), G.d_w.as<double>(), G.origin, rho, b, Cc,
               tol, f64, c.ws_B, wv, wv + 4 * Ppad, d_out2, c.profile ? &c : nullptr);
+    c.vv_stat = (const unsigned long long*)(c.ws_B.as<char>() + vv_layout(G.nsub, Ppad, f64).stat);
     double* Bm = T;    // T is dead after the density: reuse for B
     launch_ao_proj(s, B, G, wv, Bm);
     double* V = (double*)c.ws_K.ensure(sizeof(double) * nao * nao);
@@ -1967,6 +1973,7 @@ static void cycle_dev(Ctx& c, Basis& B, Grid* kg, const double* d_qd, int nop,
         nlc_dev(sd, B, *vg, d_dm, b, Cc, vv_tol, prec, d_out2, d_vmat);
         WB_HIP(hipEventRecord(c.ev_join, sd.stream));
         WB_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+        c.vv_stat = sd.vv_stat;                       // ordered after the join
         return;
     }
     if (kg) k_build_dev(c, B, *kg, d_qd, nop, omegas, k_tol, prec, d_k);
@@ -2496,13 +2503,34 @@ int wb_vv10_sum(wb_ctx_t ctx, int n, const double* coords, const double* q, cons
         double* out = (double*)dout.ensure(sizeof(double) * 3 * (size_t)n);
         DevBuf dn, dws;
         upload(dn, &n, sizeof(int), c->stream);
-        dws.ensure(vv10_sym_bytes(n, f64, nullptr));
+        const size_t symb = vv10_sym_bytes(n, f64, nullptr);
+        dws.ensure(symb + 2 * sizeof(unsigned long long));
+        auto* stat = (unsigned long long*)(dws.as<char>() + symb);
         launch_vv10_sym(c->stream, dn.as<int>(), n, dp.p, dk.p, (const float4*)dt.p,
-                        (float)tile_tol, f64, dws.p, out, out + n, out + 2 * (size_t)n);
+                        (float)tile_tol, f64, dws.p, out, out + n, out + 2 * (size_t)n, stat);
+        WB_HIP(hipMemcpyAsync(c->vv_stat_host, stat, 2 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+        c->vv_stat = nullptr;
         WB_HIP(hipMemcpyAsync(F, out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         WB_HIP(hipMemcpyAsync(U, out + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         WB_HIP(hipMemcpyAsync(W, out + 2 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         WB_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int wb_vv10_stats(wb_ctx_t ctx, long long* out2)
+{
+    WB_TRY({
+        auto* c = (Ctx*)ctx;
+        if (!c || !out2) throw Error(WB_INVALID, "null argument");
+        unsigned long long v[2] = {c->vv_stat_host[0], c->vv_stat_host[1]};
+        if (c->vv_stat) {
+            WB_HIP(hipSetDevice(c->device));
+            WB_HIP(hipStreamSynchronize(c->stream));
+            WB_HIP(hipMemcpy(v, c->vv_stat, sizeof(v), hipMemcpyDeviceToHost));
+        }
+        out2[0] = (long long)v[0];
+        out2[1] = (long long)v[1];
     })
 }
 

@@ -59,8 +59,8 @@ __host__ __device__ constexpr bool md_fuse(int la, int lb) { return lb == 0 && l
 struct PairGroup {            // the pairs of one item shell and one block
     int shX, x0, bblk, npair; // item shell, its AO offset, block index, pairs
     int pidx[MD_BBMAX];       // pair index for the block position j, -1 if none
-    // the same pairs packed for the exchange kernels:
-    // pp0 << 8 | npp << 2 | one-centre << 1 | diag
+    // the same pairs packed for the exchange kernels (as unsigned bits):
+    // pp0 << 10 | (npp - 1) << 2 | one-centre << 1 | diag, npp 1..256
     int pw[MD_BBMAX];
 };
 
@@ -70,7 +70,7 @@ struct PairScreen {           // screening data per pair (pair order of its clas
     float beta;               // sum of per-primitive bound prefactors (rounded up)
     float q;                  // 0.75 * min p (rounded down)
     int sh;                   // shA | shB << 16 (the pair's shells)
-    int pp;                   // pp0 << 6 | npp (its primitive pairs)
+    int pp;                   // pp0 << 9 | npp (its primitive pairs, npp <= 256)
 };
 
 struct ClassData {
@@ -138,6 +138,10 @@ struct Ctx {
     size_t k_budget = 0;
     // statistics of the last K build
     long long tasks_total = 0, tasks_kept = 0;
+    // VV10 pruning counters of the last VV10 sum run through this context
+    // (device, in a workspace of the context that ran it; or host values)
+    const unsigned long long* vv_stat = nullptr;
+    unsigned long long vv_stat_host[2] = {0, 0};
     float ms_last = 0.f;
     int ncopy = 8;
     // classes whose two operators run in one pass (bit = class id; only where
@@ -336,11 +340,13 @@ void launch_grid_keep(hipStream_t s, long long n, const double* xyz, const doubl
 // VV10 double sum over n = *nptr <= ncap compacted points.  pts: (x, y, z,
 // W0) and pk: (kappa, q) per point, float or double (f64); tiles: per
 // 64-point tile two float4 (centre xyz, radius | W0min, kmin, qsum, 0); ws:
-// vv10_sym_bytes(ncap) bytes.  Out: F (times -3/2), U, W per point.
+// vv10_sym_bytes(ncap) bytes.  Out: F (times -3/2), U, W per point.  stat
+// (may be null; cleared here): with tol > 0, the (wave, tile) pairs tested
+// for pruning and those skipped.
 size_t vv10_sym_bytes(int ncap, bool f64, int* nsplit_out);
 void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, const void* pk,
                      const float4* tiles, float tol, bool f64, void* ws, double* F, double* U,
-                     double* W);
+                     double* W, unsigned long long* stat);
 
 void zero_bytes(hipStream_t s, void* p, size_t bytes);
 void transpose(hipStream_t s, int n, const double* a, double* b);
@@ -348,7 +354,7 @@ void symadd(hipStream_t s, int n, const double* a, double* c);
 // The VV10 part of nr_nlc_vxc after the density: pointwise terms, compaction,
 // double sum, potential factors wv = [4][Ppad] (the operand B = launch_ao_proj
 // of them; vmat = X^T B + (X^T B)^T) and sums = (nelec, excsum).
-struct VvLayout { size_t cnt, off, pt, pk, idx, aux, ts, F, U, W, sym, total; };
+struct VvLayout { size_t cnt, off, pt, pk, idx, aux, ts, F, U, W, sym, stat, total; };
 VvLayout vv_layout(int nsub, int Ppad, bool f64);
 // workspace bytes gemm_nt_fast takes (partials, plus the folded operand)
 size_t gemm_nt_ws_bytes(int M, int N, int K, bool fold);

@@ -357,7 +357,8 @@ __device__ __forceinline__ void md_wave(
 #pragma unroll
         for (int q = 0; q < BB; ++q) {
             if (!(mask >> q & 1u)) continue;                 // uniform
-            const int pp0 = pw[q] >> 8, npp = ((pw[q] >> 2) & 63) - (int)(skw >> (4 * q) & 15u);
+            const int pp0 = (int)((unsigned)pw[q] >> 10);
+            const int npp = (int)(((unsigned)pw[q] >> 2) & 255u) + 1 - (int)(skw >> (4 * q) & 15u);
             const bool onec = pw[q] & 2, diag = pw[q] & 1;
             const real* FaC = BA ? FkC[q] : FiC;
             const real* FbC = BA ? FiC : FkC[q];
@@ -759,7 +760,7 @@ void task_kernel(TaskLaunch tl, const float4* __restrict__ sub, int nsub,
                 const int p = gr.pidx[q];
                 if (q >= bb || p < 0) continue;
                 const PairScreen ps = scr[p];
-                const int pp0 = ps.pp >> 6, pnpp = ps.pp & 63;
+                const int pp0 = ps.pp >> 9, pnpp = ps.pp & 511;
                 bool keep = true;
                 int skip = 0;
                 if (tol > 0.f) {

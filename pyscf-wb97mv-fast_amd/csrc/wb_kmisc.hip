@@ -923,11 +923,14 @@ __device__ __forceinline__ void vs_pair(const VP4<real>& a, const VP2<real>& ka,
     }
 }
 
+// stat (tol > 0, may be null): [0] += (wave, tile) visits tested for
+// pruning, [1] += those skipped; one add per wave at the end
 template <typename real>
 __global__ __launch_bounds__(256)
 void vv10_sym_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__ pt,
                      const VP2<real>* __restrict__ pk, const float4* __restrict__ ts, float tol,
-                     int nsplit, size_t pcap, double* __restrict__ ipart, real* __restrict__ jrec)
+                     int nsplit, size_t pcap, double* __restrict__ ipart, real* __restrict__ jrec,
+                     unsigned long long* __restrict__ stat)
 {
     // the tile twice over, so that point (l + k) mod 64 is entry l + k
     __shared__ VP4<real> s_pt[128];
@@ -985,6 +988,7 @@ void vv10_sym_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__
     const int tot = np * VS_GT;
     const int lo = (int)((long long)tot * part / nsplit), hi = (int)((long long)tot * (part + 1) / nsplit);
     const int nown = part == 0 ? VS_GT : 0;
+    unsigned n_test = 0, n_skip = 0;             // wave-uniform pruning counts
     for (int it = -nown; it < hi - lo; ++it) {
         const bool own = it < 0;
         int t, d = 0, u = 0;
@@ -1018,6 +1022,8 @@ void vv10_sym_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__
             const float bound = fmaxf(fmaxf(bI, 2.f * bI / gI * fmaxf(1.f, rmax * rmax)),
                                       fmaxf(bJ, 2.f * bJ / gJ * fmaxf(1.f, rmax * rmax)));
             skip = bound < tol;              // uniform per wave
+            ++n_test;
+            n_skip += skip ? 1u : 0u;
         }
         if (own) {
             if (!skip) {
@@ -1077,6 +1083,10 @@ void vv10_sym_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__
             rec[q * 64 + slot] = (real)acc;
         }
     }
+    if (stat && tol > 0.f && lane == 0) {
+        atomicAdd(stat, (unsigned long long)n_test);
+        atomicAdd(stat + 1, (unsigned long long)n_skip);
+    }
     double* ip = ipart + (size_t)part * 3 * pcap;
     if (v0) { ip[i0] = F0; ip[pcap + i0] = U0; ip[2 * pcap + i0] = W0; }
     if (v1) { ip[i1] = F1; ip[pcap + i1] = U1; ip[2 * pcap + i1] = W1; }
@@ -1125,8 +1135,9 @@ size_t vv10_sym_bytes(int ncap, bool f64, int* nsplit_out)
 // vv10_sym_bytes(ncap) bytes
 void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, const void* pk,
                      const float4* tiles, float tol, bool f64, void* ws, double* F, double* U,
-                     double* W)
+                     double* W, unsigned long long* stat)
 {
+    if (stat) zero_bytes(s, stat, 2 * sizeof(unsigned long long));
     int nsplit = 1;
     vv10_sym_bytes(ncap, f64, &nsplit);
     const int Bm = vs_groups(ncap);
@@ -1137,13 +1148,13 @@ void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, 
     if (f64) {
         hipLaunchKernelGGL(vv10_sym_kernel<double>, grid, dim3(256), 0, s, nptr,
                            (const VP4<double>*)pts, (const VP2<double>*)pk, tiles, tol, nsplit,
-                           pcap, ipart, (double*)jrec);
+                           pcap, ipart, (double*)jrec, stat);
         hipLaunchKernelGGL(vv10_sym_reduce<double>, red, dim3(256), 0, s, nptr, nsplit, pcap,
                            ipart, (const double*)jrec, F, U, W);
     } else {
         hipLaunchKernelGGL(vv10_sym_kernel<float>, grid, dim3(256), 0, s, nptr,
                            (const VP4<float>*)pts, (const VP2<float>*)pk, tiles, tol, nsplit,
-                           pcap, ipart, (float*)jrec);
+                           pcap, ipart, (float*)jrec, stat);
         hipLaunchKernelGGL(vv10_sym_reduce<float>, red, dim3(256), 0, s, nptr, nsplit, pcap,
                            ipart, (const float*)jrec, F, U, W);
     }
@@ -1611,6 +1622,7 @@ VvLayout vv_layout(int nsub, int Ppad, bool f64)
     L.U = take(sizeof(double) * Ppad);
     L.W = take(sizeof(double) * Ppad);
     L.sym = take(vv10_sym_bytes(Ppad, f64, nullptr));
+    L.stat = take(2 * sizeof(unsigned long long));
     L.total = o;
     return L;
 }
@@ -1646,7 +1658,8 @@ void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const doubl
                            idx, aux);
         hipLaunchKernelGGL(vv_tile_kernel<double>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
                            nptr, pt, pk, (int)ntile_max, ts);
-        launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, true, base + o_sym, F, U, W);
+        launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, true, base + o_sym, F, U, W,
+                        (unsigned long long*)(base + L.stat));
     } else {
         auto* pt = (VP4<float>*)(base + o_pt);
         auto* pk = (VP2<float>*)(base + o_pk);
@@ -1655,7 +1668,8 @@ void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const doubl
                            idx, aux);
         hipLaunchKernelGGL(vv_tile_kernel<float>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
                            nptr, pt, pk, (int)ntile_max, ts);
-        launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, false, base + o_sym, F, U, W);
+        launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, false, base + o_sym, F, U, W,
+                        (unsigned long long*)(base + L.stat));
     }
     (void)pi;
     if (prof) prof->mark(PH_VV10);

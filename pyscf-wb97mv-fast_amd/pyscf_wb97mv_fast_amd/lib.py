@@ -95,6 +95,7 @@ _SIGS = {
                                    _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     'wb_cycle_dev': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i, _vp, _vp,
                           _vp]),
+    'wb_vv10_stats': (_i, [_vp, _vp]),
     'wb_cycle_graph_launch': (_i, [_vp]),
     'wb_cycle_graph_destroy': (_i, [_vp]),
 }
@@ -183,6 +184,13 @@ class Context:
     def trim(self):
         """Release the workspaces (they regrow on the next call)."""
         check(load().wb_ctx_trim(self.h))
+
+    def vv10_stats(self):
+        """(tile pairs tested, skipped) by the pruning of the last VV10 sum
+        on this context (both 0 without pruning)."""
+        out = numpy.zeros(2, dtype=numpy.int64)
+        check(load().wb_vv10_stats(self.h, ptr(out)))
+        return int(out[0]), int(out[1])
 
     def set_k_budget(self, nbytes):
         """Workspace budget of this context's K builds (0: no limit); a

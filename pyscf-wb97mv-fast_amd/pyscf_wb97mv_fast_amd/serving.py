@@ -35,9 +35,22 @@ budget is refused with GpuOOM.
 Latency is time from submission to the job's last cycle -- for SCF jobs the
 time to the converged energy -- (p50/p99 over the finished jobs), beside
 throughput in jobs/s.  The reference has no serving path.
+
+The host steps (``scf`` / ``update``: Fock matrices, DIIS, diagonalization)
+of the slots run concurrently on the host's cores.  A BLAS library that
+gives each of them every core oversubscribes the host (measured on a
+192-AO water cluster: 0.15-0.23 s per SCF step in the serving process
+against 2-9 ms of device cycle); ``host_threads`` (default: the process's
+thread budget, ``OMP_NUM_THREADS`` or the usable cores, divided by the
+slots) caps the BLAS thread pools while the server runs, through
+threadpoolctl when it is installed, and restores them at close().  BLAS
+results can round differently with another thread count, so a job's
+results equal bit for bit across servers with the same host_threads, and
+within one server alone and in any mix.
 """
 import collections
 import ctypes
+import os
 import threading
 import time
 from concurrent.futures import Future
@@ -100,6 +113,31 @@ class Request:
     _shared: bool = field(default=False, repr=False)  # started while other jobs ran
 
 
+def host_thread_budget():
+    """Threads this process may keep busy: OMP_NUM_THREADS when set, else the
+    cores it may run on."""
+    try:
+        n = int(os.environ.get('OMP_NUM_THREADS', '0'))
+    except ValueError:
+        n = 0
+    if n > 0:
+        return n
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def _limit_blas(n):
+    """Cap every loaded BLAS thread pool at n threads (process-wide); the
+    returned object restores them.  None without threadpoolctl."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return None
+    return threadpool_limits(limits=int(n), user_api='blas')
+
+
 def _percentile(xs, p):
     return float(numpy.percentile(numpy.asarray(xs), p)) if len(xs) else None
 
@@ -107,10 +145,12 @@ def _percentile(xs, p):
 class CycleServer:
     def __init__(self, device=0, slots=4, hbm_budget=200e9, omegas=(0.0, 0.3), k_tol=1e-11,
                  nlc=(6.0, 0.01), vv_tol=0.0, precision='mixed', ncopy=8, runner=None,
-                 contexts=None):
+                 contexts=None, host_threads=None):
         """runner / contexts: for tests of the scheduling logic (a callable
         runner(ctx, request) -> result, one context object per slot, and
-        request.est_bytes as the job's memory need)."""
+        request.est_bytes as the job's memory need).  host_threads: BLAS
+        threads per slot's host step (None: the thread budget / slots; 0:
+        leave the BLAS pools as they are)."""
         self.omegas = tuple(omegas)
         self.k_tol, self.nlc, self.vv_tol = float(k_tol), tuple(nlc), float(vv_tol)
         self.precision, self.ncopy = precision, int(ncopy)
@@ -120,6 +160,10 @@ class CycleServer:
         self._ctx = contexts if contexts is not None else \
             [lib.Context(device, ncopy=ncopy) for _ in range(slots)]
         n = len(self._ctx)
+        if host_threads is None:
+            host_threads = max(1, host_thread_budget() // max(1, n))
+        self.host_threads = int(host_threads)
+        self._blas_limit = _limit_blas(self.host_threads) if self.host_threads > 0 else None
         self._held = [0] * n
         self._q = collections.deque()
         self._cv = threading.Condition()
@@ -159,6 +203,9 @@ class CycleServer:
         if self._own:
             for c in self._ctx:
                 c.close()
+        if self._blas_limit is not None:
+            self._blas_limit.restore_original_limits()
+            self._blas_limit = None
 
     def stats(self):
         lat = [r.t_end - r.t_submit for r in self.done]

@@ -16,8 +16,10 @@ balances exchange with a Coulomb term the model does not have; the full
 exchange (x = -1) alone makes E concave in D on real grids and the
 iteration oscillates between near-degenerate orbitals (measured on water
 clusters and a def2-TZVP chain at level-1 grids; x = -0.5 still fails on
-the chain), so the default is x = -0.1, which converges every test
-molecule in 8-24 cycles.  Each call takes this cycle's device results,
+the chain, and x = -0.1 stalls at a gradient of 3e-4 on an 18-heavy-atom
+def2-SVP chain, level shifts of 0.2-0.5 or a longer DIIS history not
+helping), so the default is x = -0.05, which converges every molecule of
+the serving benchmark's mix in 8-21 cycles.  Each call takes this cycle's device results,
 returns the next density (Roothaan step on the DIIS-extrapolated Fock
 matrix in the canonically orthogonalized basis, virtual orbitals optionally
 level-shifted, the lowest nelec/2 orbitals doubly occupied), the energy of
@@ -69,7 +71,7 @@ class DIIS:
 
 class ModelSCF:
     def __init__(self, mol, rsh=(0.3, 1.0, 0.15), diis_space=8, lindep=1e-8, shift=0.0,
-                 x_scale=-0.1):
+                 x_scale=-0.05):
         self.shift = float(shift)
         self.x = float(x_scale)
         self.h, self.S = guess.huckel_hamiltonian(mol)

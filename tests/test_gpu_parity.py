@@ -238,6 +238,67 @@ def test_vv10_pruning_two_far_clusters(ctx):
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_vv10_pruning_counts_and_skips_far_tiles(ctx, precision):
+    """Two clusters 1000 bohr apart: every (wave, tile) pair across the gap
+    is skipped (the device's own count, wb_vv10_stats), none within a
+    cluster, and F, U, W stay those of the exact sum (FP64 to 1e-9; the
+    mixed form to 1e-4 here, since its FP32 coordinates are relative to the
+    centroid of both clusters, 500 bohr from every point: an ulp of 3e-5
+    bohr on distances of about 1 bohr); without pruning nothing is tested."""
+    rng = numpy.random.default_rng(6)
+    n = 1024                                  # two 512-point groups, 8 tiles each
+    coords = rng.normal(size=(n, 3))
+    coords[n // 2:] += 1000.0
+    q, W0, kappa = rng.uniform(0.01, 0.1, n), rng.uniform(0.5, 1.0, n), rng.uniform(0.3, 0.6, n)
+    ref = oracle.vv10_sum(coords, q, W0, kappa)
+    k = vv10.Vv10Kernel(ctx, tile_tol=1e-12, precision=precision)
+    got = k.kernel_sum(coords, q, W0, kappa)
+    tested, skipped = ctx.vv10_stats()
+    # points are kept in order, so each 512-point group (4 waves, 8 tiles) is
+    # one cluster; every unordered pair of groups is visited once, so group
+    # 0's waves test its own 8 tiles and group 1's 8, group 1's waves its own
+    assert tested == 4 * (8 + 8) + 4 * 8, tested
+    assert skipped == 4 * 8, skipped
+    for a, b in zip(got, ref):
+        assert numpy.allclose(a, b, rtol=1e-9 if precision == 'fp64' else 1e-4)
+    vv10.Vv10Kernel(ctx, tile_tol=0.0, precision=precision).kernel_sum(coords, q, W0, kappa)
+    assert ctx.vv10_stats() == (0, 0)
+
+
+def test_vv10_pruning_on_a_real_density(ctx):
+    """Mixed form, two (H2O)8 clusters 60 bohr apart, level-0 grid: with
+    tile_tol 1e-10 and 1e-8 the device skips tile pairs (more at the larger
+    tolerance), and E_nl moves by less than the bound the tolerance implies
+    (each skipped pair's contribution to any F_i, U_i or W_i is below tol,
+    so |dE_nl| <= 0.75 sum_i q_i * (tiles skipped per point) * tol).  The
+    bound takes each tile's smallest W0 and kappa and its whole charge, so
+    on real densities it prunes only pairs far from both tiles' dense
+    points."""
+    base = geometry.water_cluster(8, seed=2)
+    shift = 60.0 * mole.BOHR
+    atoms = base + [(s_, (x + shift, y, z)) for s_, (x, y, z) in base]
+    mol = mole.Mole(atoms, 'def2-svp')
+    g = grids.Grids(mol, level=0).build()
+    dm = guess.huckel_density(mol)
+    st = vv10.NlcState(ctx, mol)
+    ni = sgx.SyntheticNumInt()
+    n0, e0, v0 = vv10.gpu_nr_nlc_vxc(st, ni, mol, g, 'wb97m-v', dm, precision='mixed')
+    ntiles = (g.size + 63) // 64
+    last = 0
+    for tol in (1e-10, 1e-8):
+        n1, e1, v1 = vv10.gpu_nr_nlc_vxc(st, ni, mol, g, 'wb97m-v', dm, precision='mixed',
+                                         tile_tol=tol)
+        tested, skipped = ctx.vv10_stats()
+        print('tol %.0e: tile pairs tested %d, skipped %d (%.1f %%); dE_nl %.2e, max|dV| %.2e'
+              % (tol, tested, skipped, 100.0 * skipped / max(tested, 1), e1 - e0,
+                 numpy.abs(v1 - v0).max()))
+        assert skipped > last
+        last = skipped
+        assert abs(e1 - e0) <= 0.75 * n0 * ntiles * tol
+        assert abs(n1 - n0) == 0.0
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_nlc_vxc_matches_oracle(ctx, precision):
     mol, g, dm = _system(npts=3000)
     st = vv10.NlcState(ctx, mol)
