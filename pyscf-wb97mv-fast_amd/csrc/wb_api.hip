@@ -622,7 +622,7 @@ struct KPlan {
     int kchunk = 0, csub = 0, nchunk = 0;
     size_t nitem_sub = 0;           // item-list entries per sub-block, all classes
     size_t qdt = 0, x = 0, f = 0, fmax = 0, counts = 0, ids = 0, skips = 0, g = 0, mask = 0,
-           cp = 0, fold = 0, xyz = 0, w = 0, sub = 0, total = 0;
+           cp = 0, fold = 0, xyz = 0, w = 0, sub = 0, lst = 0, total = 0;
 };
 
 static KPlan k_plan(const Basis& B, const Grid& G, int nop, bool f32, bool g64, int ncopy,
@@ -648,6 +648,7 @@ static KPlan k_plan(const Basis& B, const Grid& G, int nop, bool f32, bool g64, 
     P.mask = take(zrow <= 32 * MD_MASK_WORDS ? 4 * (size_t)ncopy * P.csub * MD_MASK_WORDS * nop : 0);
     P.cp = take(8 * nsplit * nao * nao);
     P.fold = take(8 * nao * Pc);
+    P.lst = take(ao_lists_layout(B, P.csub).total);
     if (P.nchunk > 1) {
         P.xyz = take(8 * 3 * Pc);
         P.w = take(8 * Pc);
@@ -759,8 +760,8 @@ struct KChunk {
 // 2 NCLASS x u64; QDT: QD transposed)
 template <typename Mark>
 static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char* arena,
-                          unsigned long long* work, const double* QDT, int ci, bool f32, bool g64,
-                          double tol, KChunk& K, Mark mark)
+                          unsigned long long* work, const double* QD, const double* QDT, int ci,
+                          bool f32, bool g64, double tol, KChunk& K, Mark mark)
 {
     hipStream_t s = c.stream;
     const int nao = B.nao, ncopy = c.ncopy;
@@ -792,12 +793,19 @@ static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char*
         K.gv = &Gc;
     }
     K.X = (double*)(arena + P.x);
-    launch_ao_eval(s, B, *K.gv, 0, K.X);
-    mark(PH_AO);
     // F and G carry MD_SPARE spare rows after the AO rows for the padding
     // positions of the exchange kernels' B blocks
     K.F = arena + P.f;
-    gemm_nn_fast(s, nao, Ppad, nao, QDT, K.X, K.F, f32);
+    // F = QD^T X over each 128-point tile's significant AOs only (the rows
+    // of QD and X gathered by the tile's AO list); the dense GEMM when the
+    // shapes do not allow it
+    const bool gather = gemm_nn_gather_ok(nao, Ppad, QD, K.X);
+    const AoLists AL = ao_lists_layout(B, ns);
+    if (gather) launch_ao_eval_lists(s, B, *K.gv, K.X, arena + P.lst, AL);
+    else launch_ao_eval(s, B, *K.gv, 0, K.X);
+    mark(PH_AO);
+    if (gather) gemm_nn_gather(s, nao, Ppad, QD, K.X, K.F, f32, arena + P.lst, AL);
+    else gemm_nn_fast(s, nao, Ppad, nao, QDT, K.X, K.F, f32);
     mark(PH_FGEMM);
     // screening and task lists
     float* fmax = (float*)(arena + P.fmax);
@@ -926,7 +934,7 @@ static void k_build_arena(Ctx& c, Basis& B, Grid& G, const double* d_QD, int nop
     for (auto& C : B.cls) ntask += (size_t)C.npairs * G.nsub;
     for (int ci = 0; ci < P.nchunk; ++ci) {
         KChunk K;
-        k_chunk_setup(c, B, G, P, arena, work, QDT, ci, f32, g64, tol, K, mark);
+        k_chunk_setup(c, B, G, P, arena, work, d_QD, QDT, ci, f32, g64, tol, K, mark);
         for (int k = 0; k < NCLASS; ++k) {
             MdJob js[8];
             int nj = 0;
@@ -1031,10 +1039,16 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     // AO values only: the density and the potential operand evaluate the
     // gradients on the fly instead of storing (and twice reading) 3 planes
     double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane);
-    launch_ao_eval(s, B, G, 0, X);
+    // T = D X (D symmetric) over each 128-point tile's significant AOs
+    const bool gather = gemm_nn_gather_ok(nao, Ppad, d_dm, X);
+    const AoLists AL = ao_lists_layout(B, G.nsub);
+    void* lws = gather ? c.ws_list.ensure(AL.total) : nullptr;
+    if (gather) launch_ao_eval_lists(s, B, G, X, lws, AL);
+    else launch_ao_eval(s, B, G, 0, X);
     c.mark(PH_NLC_AO);
     double* T = (double*)c.ws_T.ensure(sizeof(double) * plane);
-    gemm_nn_fast(s, nao, Ppad, nao, d_dm, X, T, false);   // T = D X (D symmetric)
+    if (gather) gemm_nn_gather(s, nao, Ppad, d_dm, X, T, false, lws, AL);
+    else gemm_nn_fast(s, nao, Ppad, nao, d_dm, X, T, false);
     double* rho = (double*)c.ws_rho.ensure(sizeof(double) * 4 * Ppad);
     double* part = (double*)c.ws_red.ensure(ao_rho_bytes(B, Ppad));
     launch_ao_rho(s, B, G, T, part, rho);
@@ -1662,6 +1676,7 @@ static void job_needs(const Ctx& w, const wb_cycle_job& j, int nop, int prec,
         // or has no K build
         size_t* v = (j.kgrid && !(j.flags & WB_JOB_SEQUENTIAL) && !w.profile) ? side : need;
         at_least(v, WS_XV, 8 * nao * P);
+        at_least(v, WS_LIST, ao_lists_layout(B, G.nsub).total);
         at_least(v, WS_T, 8 * nao * P);
         at_least(v, WS_RHO, 8 * 4 * P);
         at_least(v, WS_RED, ao_rho_bytes(B, (int)P));
@@ -2064,8 +2079,8 @@ static void k_build_queue_group(Ctx& c, const std::vector<QueueJob>& g, const wb
         char* a = arena + g[i].arena;
         double* QDT = (double*)(a + g[i].P.qdt);
         transpose(s, B.nao, qd, QDT);
-        k_chunk_setup(c, *g[i].B, *g[i].G, g[i].P, a, work + 2 * NCLASS * i, QDT, 0, f32, g64,
-                      tol, *K[i], [](int) {});
+        k_chunk_setup(c, *g[i].B, *g[i].G, g[i].P, a, work + 2 * NCLASS * i, qd, QDT, 0, f32,
+                      g64, tol, *K[i], [](int) {});
     }
     // the class launches: every molecule's shares of class k in job order
     // (each molecule's in the order its own build launches them)

@@ -123,7 +123,7 @@ struct Grid {
 // --------------------------------------------------------------- context ---
 // the context's grow-only workspaces, in a fixed order (Ctx::bufs)
 enum WsBuf { WS_X, WS_XV, WS_KARENA, WS_KP, WS_K, WS_T, WS_RHO, WS_VV, WS_B, WS_RED, WS_IO,
-             WS_WORK, WS_COUNT };
+             WS_WORK, WS_LIST, WS_COUNT };
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -133,6 +133,7 @@ struct Ctx {
     // (k_plan), in one allocation
     DevBuf ws_X, ws_Xv, ws_karena, ws_Kp, ws_K;
     DevBuf ws_T, ws_rho, ws_vv, ws_B, ws_red, ws_io;
+    DevBuf ws_list;                    // AO lists of the VV10 flow's density GEMM
     // device bytes a K build may take for its workspace (0: no limit; the
     // grid is then processed in one chunk)
     size_t k_budget = 0;
@@ -179,7 +180,7 @@ struct Ctx {
     size_t ws_bytes() const;
     void trim();
     DevBuf* bufs[WS_COUNT] = {&ws_X, &ws_Xv, &ws_karena, &ws_Kp, &ws_K, &ws_T, &ws_rho,
-                              &ws_vv, &ws_B, &ws_red, &ws_io, &ws_work};
+                              &ws_vv, &ws_B, &ws_red, &ws_io, &ws_work, &ws_list};
 };
 
 enum Phase { PH_AO = 0, PH_FGEMM, PH_SCREEN, PH_MD, PH_KGEMM, PH_NLC_AO, PH_NLC_RHO,
@@ -187,6 +188,18 @@ enum Phase { PH_AO = 0, PH_FGEMM, PH_SCREEN, PH_MD, PH_KGEMM, PH_NLC_AO, PH_NLC_
 
 // --------------------------------------------------------- launchers --------
 void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X);
+// AO values below this magnitude at every point of a 64-point sub-block
+// leave the shell out of that sub-block's AO list (the reference prunes AO
+// blocks with PySCF's non0tab the same way, gpu/ao_eval.py:134-153)
+constexpr double AO_SIG_THR = 1e-16;
+// AO lists of the 128-point tiles of a grid (two sub-blocks each), in one
+// workspace: sig [nsub][nshell] bytes, counts [ntile], lists [ntile][cap]
+struct AoLists { size_t sig = 0, counts = 0, lists = 0, total = 0; int cap = 0; };
+AoLists ao_lists_layout(const Basis& b, int nsub);
+// X (values) as launch_ao_eval, plus the lists of the AOs significant in
+// each tile, into ws (ao_lists_layout(b, g.nsub).total bytes)
+void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, double* X, void* ws,
+                          const AoLists& L);
 void launch_ao_eval_pts(hipStream_t s, const Basis& b, const double* xyz, int Ppad,
                         int deriv, double* X);
 
@@ -213,6 +226,12 @@ void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A,
 // row0, the first row of each row's shell, and nsub): the exchange kernels'
 // written-shell masks of the copies (MdMasks layout, one operator): rows of a
 // copy's sub-block region that were never written are skipped (K = nsub * 64)
+// C[M][N] = A^T B over the AO rows listed per 128-point n tile (A [K][M]
+// and B [K][N] row-major by AO; launch_ao_eval_lists's ws and layout); M
+// even, N a multiple of 64, A and B 16-byte aligned (gemm_nn_gather_ok)
+bool gemm_nn_gather_ok(int M, int N, const double* A, const double* B);
+void gemm_nn_gather(hipStream_t s, int M, int N, const double* A, const double* B, void* C,
+                    bool out_f32, const void* ws, const AoLists& L);
 void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B,
                   void* C, bool out_f32);
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B,

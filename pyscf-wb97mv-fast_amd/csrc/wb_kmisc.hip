@@ -64,8 +64,10 @@ __device__ __forceinline__ void ao_shell_eval(double dx, double dy, double dz, d
 //   GGA potential factors at g), the operand of the potential matrix.
 enum AoMode { AO_STORE0 = 0, AO_STORE1, AO_RHO, AO_PROJ };
 
+// returns, for AO_STORE0, whether any value of the shell at g exceeds
+// AO_SIG_THR in magnitude
 template <int l, int MODE>
-__device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double rad,
+__device__ __forceinline__ bool ao_shell(double dx, double dy, double dz, double rad,
                                          double drad, double* __restrict__ X, size_t plane,
                                          int ao0, int Ppad, int g, const double* __restrict__ T,
                                          const double (&v)[4], double (&acc)[4])
@@ -74,11 +76,13 @@ __device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double
     constexpr int DERIV = MODE != AO_STORE0;
     double sv[NS], gs[3][NS];
     ao_shell_eval<l, DERIV>(dx, dy, dz, rad, drad, sv, gs);
+    bool big = false;
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         const size_t o = (size_t)(ao0 + m) * Ppad + g;
         if constexpr (MODE == AO_STORE0 || MODE == AO_STORE1) {
             X[o] = sv[m];
+            big = big || fabs(sv[m]) > AO_SIG_THR;
             if constexpr (MODE == AO_STORE1) {
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) X[(ax + 1) * plane + o] = gs[ax][m];
@@ -92,17 +96,20 @@ __device__ __forceinline__ void ao_shell(double dx, double dy, double dz, double
             X[o] = v[0] * sv[m] + v[1] * gs[0][m] + v[2] * gs[1][m] + v[3] * gs[2][m];
         }
     }
+    return big;
 }
 
 // grid: (points / 256, shell groups of spb); AO_RHO writes the group's
-// partial sums part[group][4][Ppad]
+// partial sums part[group][4][Ppad].  sig (AO_STORE0, may be null):
+// sig[sub-block][shell] = 1 if a value of the shell exceeds AO_SIG_THR at a
+// point of the 64-point sub-block (one wave = one sub-block), else 0.
 template <int MODE>
 __global__ __launch_bounds__(256)
 void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
                const double* __restrict__ exps, const double* __restrict__ coefs,
                const double* __restrict__ xyz, int Ppad, double* __restrict__ X, size_t plane,
                const double* __restrict__ T, const double* __restrict__ vg,
-               double* __restrict__ part)
+               double* __restrict__ part, unsigned char* __restrict__ sig)
 {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= Ppad) return;
@@ -127,12 +134,20 @@ void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
                 drad -= 2.0 * a * ce;
             }
         }
+        bool big;
         switch (sh.l) {
-        case 0: ao_shell<0, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
-        case 1: ao_shell<1, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
-        case 2: ao_shell<2, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
-        default: ao_shell<3, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        case 0: big = ao_shell<0, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        case 1: big = ao_shell<1, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        case 2: big = ao_shell<2, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
+        default: big = ao_shell<3, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
         }
+        if constexpr (MODE == AO_STORE0) {
+            if (sig) {                                // uniform
+                const bool any = __ballot(big) != 0ull;
+                if ((threadIdx.x & 63) == 0) sig[(size_t)(g >> 6) * nshell + is] = any ? 1 : 0;
+            }
+        }
+        (void)big;
     }
     if constexpr (MODE == AO_RHO) {
 #pragma unroll
@@ -142,12 +157,13 @@ void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
 
 template <int MODE>
 static void launch_ao_pass(hipStream_t s, const Basis& b, const double* xyz, int Ppad, int spb,
-                           double* X, const double* T, const double* vg, double* part)
+                           double* X, const double* T, const double* vg, double* part,
+                           unsigned char* sig = nullptr)
 {
     dim3 grid((Ppad + 255) / 256, (b.nshell + spb - 1) / spb);
     hipLaunchKernelGGL(ao_kernel<MODE>, grid, dim3(256), 0, s, b.d_shells.as<ShellInfo>(),
                        b.nshell, spb, b.d_exps.as<double>(), b.d_coefs.as<double>(), xyz, Ppad, X,
-                       (size_t)b.nao * Ppad, T, vg, part);
+                       (size_t)b.nao * Ppad, T, vg, part, sig);
     WB_HIP(hipGetLastError());
 }
 
@@ -227,6 +243,72 @@ void launch_ao_proj(hipStream_t s, const Basis& b, const Grid& g, const double* 
 void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X)
 {
     launch_ao_eval_pts(s, b, g.d_xyz.as<double>(), g.Ppad, deriv, X);
+}
+
+// ------------------------------------------------ AO significance lists ---
+// For a 128-point tile t (sub-blocks 2t, 2t + 1): the AOs of the shells
+// flagged significant in either sub-block, ascending, in lists[t][0 ..
+// counts[t]).  One wave per tile; a wave-wide prefix sum of the shells' AO
+// counts places each flagged shell's AOs.
+__global__ __launch_bounds__(256)
+void ao_list_kernel(int nsub, int nshell, const unsigned char* __restrict__ sig,
+                    const int* __restrict__ sh_ao0, const int* __restrict__ sh_nao, int cap,
+                    int* __restrict__ counts, int* __restrict__ lists)
+{
+    const int t = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ntile = (nsub + 1) / 2;
+    if (t >= ntile) return;
+    const int lane = threadIdx.x & 63;
+    const unsigned char* s0 = sig + (size_t)(2 * t) * nshell;
+    const unsigned char* s1 = sig + (size_t)min(2 * t + 1, nsub - 1) * nshell;
+    int* out = lists + (size_t)t * cap;
+    int n = 0;
+    for (int b0 = 0; b0 < nshell; b0 += 64) {
+        const int sh = b0 + lane;
+        int na = 0, a0 = 0;
+        if (sh < nshell && (s0[sh] | s1[sh])) {
+            na = sh_nao[sh];
+            a0 = sh_ao0[sh];
+        }
+        int incl = na;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        const int off = n + incl - na;
+        for (int m = 0; m < na; ++m) out[off + m] = a0 + m;
+        n += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) counts[t] = n;
+}
+
+AoLists ao_lists_layout(const Basis& b, int nsub)
+{
+    AoLists L;
+    const int ntile = (nsub + 1) / 2;
+    L.cap = b.nao;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { const size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
+    L.sig = take((size_t)nsub * b.nshell);
+    L.counts = take(sizeof(int) * (size_t)ntile);
+    L.lists = take(sizeof(int) * (size_t)ntile * L.cap);
+    L.total = o;
+    return L;
+}
+
+void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, double* X, void* ws,
+                          const AoLists& L)
+{
+    char* base = (char*)ws;
+    unsigned char* sig = (unsigned char*)(base + L.sig);
+    launch_ao_pass<AO_STORE0>(s, b, g.d_xyz.as<double>(), g.Ppad, 16, X, nullptr, nullptr,
+                              nullptr, sig);
+    const int ntile = (g.nsub + 1) / 2;
+    hipLaunchKernelGGL(ao_list_kernel, dim3((ntile + 3) / 4), dim3(256), 0, s, g.nsub, b.nshell,
+                       sig, b.d_shao.as<int>(), b.d_shao.as<int>() + b.nshell, L.cap,
+                       (int*)(base + L.counts), (int*)(base + L.lists));
+    WB_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------- FP64 GEMMs ----
@@ -581,6 +663,92 @@ void gemm_nn_t_kernel(int M, int N, int K, const double* __restrict__ A,
     store_tile<WM, WN>(C, N, M, N, m0, n0, wm, wn, acc);
 }
 
+// Row-gathered NN GEMM over the AO lists of the 128-point n tiles:
+//   C[m][n] = sum_{j < counts[t]} A[k_j][m] B[k_j][n],  k_j = lists[t][j],
+// t = n / 128: A (rows: AOs, e.g. QD or D) and B (rows: AOs, the AO values
+// X) are both read a whole row piece at a time, so the gathered loads stay
+// 16-byte, 8 lanes per 128-byte line; rows past the list end are zero, and
+// a last tile of 64 points reads clamped columns it never stores.  The
+// sum runs over the AOs that are significant somewhere in the tile only
+// (the others' values are below AO_SIG_THR): about a quarter of them on
+// the benchmark molecules.  lda and ldb even, A and B 16-byte aligned.
+template <int BR>
+__device__ __forceinline__ void fetch_rows_gather(const double* __restrict__ A, int lda, int cols,
+                                                  const int* __restrict__ lst, int cnt, int k0,
+                                                  int c0, dv2 (&v)[BR / 32])
+{
+    constexpr int CPR = BR / 2;
+#pragma unroll
+    for (int i = 0; i < BR / 32; ++i) {
+        const int id = i * 256 + threadIdx.x;
+        const int k = k0 + id / CPR, c = min(c0 + 2 * (id % CPR), cols - 2);
+        dv2 t = (dv2){0.0, 0.0};
+        if (k < cnt) t = *(const dv2*)(A + (size_t)lst[k] * lda + c);
+        v[i] = t;
+    }
+}
+
+template <int WM, typename OutT>
+__global__ __launch_bounds__(256, 2)
+void gemm_nn_gather_kernel(int M, int N, const double* __restrict__ A, int lda,
+                           const double* __restrict__ B, int ldb, OutT* __restrict__ C, int mt,
+                           const int* __restrict__ counts, const int* __restrict__ lists,
+                           int cap)
+{
+    constexpr int WN = 4;
+    constexpr int BM = 32 * WM, BN = 32 * WN, PA = BM + 4, PB = BN + 4;
+    __shared__ double As[2][T_K * PA];
+    __shared__ double Bs[2][T_K * PB];
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int nt = w / mt;
+    const int m0 = (w % mt) * BM, n0 = nt * BN;
+    const int cnt = counts[nt];
+    const int* __restrict__ lst = lists + (size_t)nt * cap;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
+    const int c = lane & 15, kq = lane >> 4;
+    d4 acc[WM][WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = (d4){0, 0, 0, 0};
+    dv2 va[BM / 32], vb[BN / 32];
+    fetch_rows_gather<BM>(A, lda, M, lst, cnt, 0, m0, va);
+    fetch_rows_gather<BN>(B, ldb, N, lst, cnt, 0, n0, vb);
+    int buf = 0;
+    for (int k0 = 0; k0 < cnt; k0 += T_K) {
+#pragma unroll
+        for (int i = 0; i < BM / 32; ++i) {
+            const int id = i * 256 + threadIdx.x;
+            *(dv2*)(&As[buf][(id / (BM / 2)) * PA + 2 * (id % (BM / 2))]) = va[i];
+        }
+#pragma unroll
+        for (int i = 0; i < BN / 32; ++i) {
+            const int id = i * 256 + threadIdx.x;
+            *(dv2*)(&Bs[buf][(id / (BN / 2)) * PB + 2 * (id % (BN / 2))]) = vb[i];
+        }
+        __syncthreads();
+        if (k0 + T_K < cnt) {
+            fetch_rows_gather<BM>(A, lda, M, lst, cnt, k0 + T_K, m0, va);
+            fetch_rows_gather<BN>(B, ldb, N, lst, cnt, k0 + T_K, n0, vb);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            double a[WM], b[WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i) a[i] = As[buf][(4 * kq + s) * PA + wm + 16 * i + c];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) b[j] = Bs[buf][(4 * kq + s) * PB + wn + 16 * j + c];
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[i][j] = mfma64(a[i], b[j], acc[i][j]);
+        }
+        buf ^= 1;
+    }
+    store_tile<WM, WN>(C, N, M, N, m0, n0, wm, wn, acc);
+}
+
 // Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k] (both k-contiguous, K
 // even); tiles fastest within a split (they share the A and B slices)
 template <int WM, int WN>
@@ -735,6 +903,30 @@ void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const dou
     if (wm == 3) { if (out_f32) WB_NN(3, float); else WB_NN(3, double); }
     else { if (out_f32) WB_NN(4, float); else WB_NN(4, double); }
 #undef WB_NN
+    WB_HIP(hipGetLastError());
+}
+
+bool gemm_nn_gather_ok(int M, int N, const double* A, const double* B)
+{
+    return M % 2 == 0 && M >= 2 && N % 64 == 0 && N >= 64 && aligned16(A) && aligned16(B);
+}
+
+void gemm_nn_gather(hipStream_t s, int M, int N, const double* A, const double* B, void* C,
+                    bool out_f32, const void* ws, const AoLists& L)
+{
+    if (!gemm_nn_gather_ok(M, N, A, B))
+        throw Error(WB_INVALID, "gathered GEMM: M even, N a multiple of 64, aligned operands");
+    const char* base = (const char*)ws;
+    const int* counts = (const int*)(base + L.counts);
+    const int* lists = (const int*)(base + L.lists);
+    const int wm = wave_tiles(M);
+    const int mt = (M + 32 * wm - 1) / (32 * wm), nt = (N + 127) / 128;
+    dim3 grid(mt * nt);
+#define WB_NG(WMv, T) hipLaunchKernelGGL((gemm_nn_gather_kernel<WMv, T>), grid, dim3(256), 0, s, \
+                                          M, N, A, M, B, N, (T*)C, mt, counts, lists, L.cap)
+    if (wm == 3) { if (out_f32) WB_NG(3, float); else WB_NG(3, double); }
+    else { if (out_f32) WB_NG(4, float); else WB_NG(4, double); }
+#undef WB_NG
     WB_HIP(hipGetLastError());
 }
 
