@@ -648,7 +648,7 @@ static KPlan k_plan(const Basis& B, const Grid& G, int nop, bool f32, bool g64, 
     P.mask = take(zrow <= 32 * MD_MASK_WORDS ? 4 * (size_t)ncopy * P.csub * MD_MASK_WORDS * nop : 0);
     P.cp = take(8 * nsplit * nao * nao);
     P.fold = take(8 * nao * Pc);
-    P.lst = take(ao_lists_layout(B, P.csub).total);
+    P.lst = take(ao_lists_layout(B, P.csub, P.kchunk).total);
     if (P.nchunk > 1) {
         P.xyz = take(8 * 3 * Pc);
         P.w = take(8 * Pc);
@@ -753,6 +753,10 @@ struct KChunk {
     char* Gb = nullptr;
     unsigned* masks = nullptr;
     size_t gstride = 0, opstride = 0, gsz = 0, mopstride = 0;
+    // the AO lists of the chunk (gathered F and K GEMMs), when gathering
+    bool gather = false;
+    AoLists AL;
+    char* lws = nullptr;
 };
 
 // chunk ci of the build planned by P on arena: its points, X, F = QD^T X,
@@ -800,11 +804,14 @@ static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char*
     // of QD and X gathered by the tile's AO list); the dense GEMM when the
     // shapes do not allow it
     const bool gather = gemm_nn_gather_ok(nao, Ppad, QD, K.X);
-    const AoLists AL = ao_lists_layout(B, ns);
-    if (gather) launch_ao_eval_lists(s, B, *K.gv, K.X, arena + P.lst, AL);
+    const AoLists AL = ao_lists_layout(B, ns, P.kchunk);
+    K.gather = gather;
+    K.AL = AL;
+    K.lws = arena + P.lst;
+    if (gather) launch_ao_eval_lists(s, B, *K.gv, K.X, K.lws, AL);
     else launch_ao_eval(s, B, *K.gv, 0, K.X);
     mark(PH_AO);
-    if (gather) gemm_nn_gather(s, nao, Ppad, QD, K.X, K.F, f32, arena + P.lst, AL);
+    if (gather) gemm_nn_gather(s, nao, Ppad, QD, K.X, K.F, f32, K.lws, AL);
     else gemm_nn_fast(s, nao, Ppad, nao, QDT, K.X, K.F, f32);
     mark(PH_FGEMM);
     // screening and task lists
@@ -900,11 +907,14 @@ static void k_chunk_finish(Ctx& c, Basis& B, const KPlan& P, char* arena, const 
         else zero_bytes(s, K.Gb, K.gsz * K.opstride * nop);
     }
     mark(PH_MD);
+    // K = X^T (w o G) over each point split's significant AO rows of X
+    const NtGather ng = K.gather ? nt_gather(K.AL, K.lws) : NtGather{};
     for (int op = 0; op < nop; ++op) {
         gemm_nt_core(s, nao, nao, K.Ppad, P.kchunk, K.X, K.Gb + K.gsz * K.opstride * op, !g64,
                      c.ncopy, K.gstride, K.gv->d_w.as<double>(), d_K + (size_t)op * nn,
                      accumulate, (double*)(arena + P.cp), (double*)(arena + P.fold),
-                     K.tracked ? K.masks + K.mopstride * op : nullptr, B.d_row0.as<int>(), K.ns);
+                     K.tracked ? K.masks + K.mopstride * op : nullptr, B.d_row0.as<int>(), K.ns,
+                     K.gather ? &ng : nullptr);
         mark(PH_KGEMM);
     }
 }
@@ -1041,7 +1051,9 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane);
     // T = D X (D symmetric) over each 128-point tile's significant AOs
     const bool gather = gemm_nn_gather_ok(nao, Ppad, d_dm, X);
-    const AoLists AL = ao_lists_layout(B, G.nsub);
+    int kc = 0;
+    nt_split(Ppad, kc);
+    const AoLists AL = ao_lists_layout(B, G.nsub, kc);
     void* lws = gather ? c.ws_list.ensure(AL.total) : nullptr;
     if (gather) launch_ao_eval_lists(s, B, G, X, lws, AL);
     else launch_ao_eval(s, B, G, 0, X);
@@ -1060,7 +1072,10 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     double* Bm = T;    // T is dead after the density: reuse for B
     launch_ao_proj(s, B, G, wv, Bm);
     double* V = (double*)c.ws_K.ensure(sizeof(double) * nao * nao);
-    gemm_nt_fast(s, nao, nao, Ppad, X, Bm, false, 1, 0, nullptr, V, c.ws_Kp);
+    // X B^T over each point split's significant AO rows of X
+    const NtGather ng = gather ? nt_gather(AL, lws) : NtGather{};
+    gemm_nt_fast(s, nao, nao, Ppad, X, Bm, false, 1, 0, nullptr, V, c.ws_Kp, nullptr, nullptr, 0,
+                 gather ? &ng : nullptr);
     symadd(s, nao, V, d_vmat);
     c.mark(PH_NLC_FOCK);
     collect(c);
@@ -1676,7 +1691,11 @@ static void job_needs(const Ctx& w, const wb_cycle_job& j, int nop, int prec,
         // or has no K build
         size_t* v = (j.kgrid && !(j.flags & WB_JOB_SEQUENTIAL) && !w.profile) ? side : need;
         at_least(v, WS_XV, 8 * nao * P);
-        at_least(v, WS_LIST, ao_lists_layout(B, G.nsub).total);
+        {
+            int kc = 0;
+            nt_split((int)P, kc);
+            at_least(v, WS_LIST, ao_lists_layout(B, G.nsub, kc).total);
+        }
         at_least(v, WS_T, 8 * nao * P);
         at_least(v, WS_RHO, 8 * 4 * P);
         at_least(v, WS_RED, ao_rho_bytes(B, (int)P));

@@ -192,10 +192,16 @@ void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, dou
 // leave the shell out of that sub-block's AO list (the reference prunes AO
 // blocks with PySCF's non0tab the same way, gpu/ao_eval.py:134-153)
 constexpr double AO_SIG_THR = 1e-16;
-// AO lists of the 128-point tiles of a grid (two sub-blocks each), in one
-// workspace: sig [nsub][nshell] bytes, counts [ntile], lists [ntile][cap]
-struct AoLists { size_t sig = 0, counts = 0, lists = 0, total = 0; int cap = 0; };
-AoLists ao_lists_layout(const Basis& b, int nsub);
+// AO lists of a grid in one workspace: sig [nsub][nshell] bytes; for the
+// 128-point tiles (two sub-blocks each) of the gathered NN GEMM counts
+// [ntile], lists [ntile][cap]; for the point splits of the split NT GEMM
+// (kchunk points, spt sub-blocks each) scounts [nsplit], slists and sinv
+// (the position of each AO in its split's list, or -1) [nsplit][cap]
+struct AoLists {
+    size_t sig = 0, counts = 0, lists = 0, scounts = 0, slists = 0, sinv = 0, total = 0;
+    int cap = 0, spt = 0, nsplit = 0;
+};
+AoLists ao_lists_layout(const Basis& b, int nsub, int kchunk);
 // X (values) as launch_ao_eval, plus the lists of the AOs significant in
 // each tile, into ws (ao_lists_layout(b, g.nsub).total bytes)
 void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, double* X, void* ws,
@@ -221,22 +227,36 @@ void gemm_nt_reduce(hipStream_t s, int M, int N, int K, const double* A,
                     const void* B, bool b_f32, int nb, size_t bstride,
                     const double* w, double* C, DevBuf& ws);
 
-// MFMA tile versions (the production paths); gemm_nt_fast folds the copies
-// and weights of B into one FP64 operand first when needed.  masks (with
-// row0, the first row of each row's shell, and nsub): the exchange kernels'
-// written-shell masks of the copies (MdMasks layout, one operator): rows of a
-// copy's sub-block region that were never written are skipped (K = nsub * 64)
 // C[M][N] = A^T B over the AO rows listed per 128-point n tile (A [K][M]
 // and B [K][N] row-major by AO; launch_ao_eval_lists's ws and layout); M
 // even, N a multiple of 64, A and B 16-byte aligned (gemm_nn_gather_ok)
 bool gemm_nn_gather_ok(int M, int N, const double* A, const double* B);
 void gemm_nn_gather(hipStream_t s, int M, int N, const double* A, const double* B, void* C,
                     bool out_f32, const void* ws, const AoLists& L);
+// The split NT GEMM's rows of A (AOs) restricted per point split to the
+// split's list of significant AOs (ao_lists_layout's slists / scounts /
+// sinv): the split's partial holds those rows only, and the reduce adds each
+// where its AO appears
+struct NtGather {
+    const int* counts = nullptr;
+    const int* lists = nullptr;
+    const int* inv = nullptr;
+    int cap = 0, nsplit = 0;
+};
+NtGather nt_gather(const AoLists& L, const void* ws);
+
+// MFMA tile versions (the production paths); gemm_nt_fast folds the copies
+// and weights of B into one FP64 operand first when needed.  masks (with
+// row0, the first row of each row's shell, and nsub): the exchange kernels'
+// written-shell masks of the copies (MdMasks layout, one operator): rows of a
+// copy's sub-block region that were never written are skipped (K = nsub * 64).
+// gather: A's rows per split as NtGather lists (null: every row)
 void gemm_nn_fast(hipStream_t s, int M, int N, int K, const double* A, const double* B,
                   void* C, bool out_f32);
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B,
                   bool b_f32, int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
-                  const unsigned* masks = nullptr, const int* row0 = nullptr, int nsub = 0);
+                  const unsigned* masks = nullptr, const int* row0 = nullptr, int nsub = 0,
+                  const NtGather* gather = nullptr);
 
 // fmax[s][sh] = max over sub-block s of w_g sum_{mu in sh} |F[mu][g]|
 void launch_fmax(hipStream_t s, const void* F, bool f32, const double* w, int nao,
@@ -389,7 +409,8 @@ bool gemm_nt_folds(bool b_f32, int nb, const double* w, const void* B, const uns
 // that accumulates them in order gives the one-call result bit for bit.
 void gemm_nt_core(hipStream_t s, int M, int N, int K, int kchunk, const double* A, const void* B,
                   bool b_f32, int nb, size_t bstride, const double* w, double* C, bool accumulate,
-                  double* Cp, double* Bf, const unsigned* masks, const int* row0, int nsub);
+                  double* Cp, double* Bf, const unsigned* masks, const int* row0, int nsub,
+                  const NtGather* gather = nullptr);
 struct Ctx;
 void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
                const double* origin, const double* rho, double b, double C, double tol, bool f64,

@@ -246,29 +246,33 @@ void launch_ao_eval(hipStream_t s, const Basis& b, const Grid& g, int deriv, dou
 }
 
 // ------------------------------------------------ AO significance lists ---
-// For a 128-point tile t (sub-blocks 2t, 2t + 1): the AOs of the shells
-// flagged significant in either sub-block, ascending, in lists[t][0 ..
-// counts[t]).  One wave per tile; a wave-wide prefix sum of the shells' AO
-// counts places each flagged shell's AOs.
+// For tile t of spt sub-blocks (t spt .. min((t + 1) spt, nsub) - 1): the
+// AOs of the shells flagged significant in any of its sub-blocks, ascending,
+// in lists[t][0 .. counts[t]); with inv, inv[t][mu] = the position of AO mu
+// in that list, or -1.  One wave per tile; a wave-wide prefix sum of the
+// shells' AO counts places each flagged shell's AOs.
 __global__ __launch_bounds__(256)
-void ao_list_kernel(int nsub, int nshell, const unsigned char* __restrict__ sig,
+void ao_list_kernel(int nsub, int spt, int nshell, const unsigned char* __restrict__ sig,
                     const int* __restrict__ sh_ao0, const int* __restrict__ sh_nao, int cap,
-                    int* __restrict__ counts, int* __restrict__ lists)
+                    int* __restrict__ counts, int* __restrict__ lists, int* __restrict__ inv)
 {
     const int t = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ntile = (nsub + 1) / 2;
+    const int ntile = (nsub + spt - 1) / spt;
     if (t >= ntile) return;
     const int lane = threadIdx.x & 63;
-    const unsigned char* s0 = sig + (size_t)(2 * t) * nshell;
-    const unsigned char* s1 = sig + (size_t)min(2 * t + 1, nsub - 1) * nshell;
+    const int sb0 = t * spt, sb1 = min(nsub, sb0 + spt);
     int* out = lists + (size_t)t * cap;
+    int* iv = inv ? inv + (size_t)t * cap : nullptr;
     int n = 0;
     for (int b0 = 0; b0 < nshell; b0 += 64) {
         const int sh = b0 + lane;
-        int na = 0, a0 = 0;
-        if (sh < nshell && (s0[sh] | s1[sh])) {
-            na = sh_nao[sh];
+        int na = 0, a0 = 0, nall = 0;
+        if (sh < nshell) {
             a0 = sh_ao0[sh];
+            nall = sh_nao[sh];
+            unsigned char f = 0;
+            for (int sb = sb0; sb < sb1; ++sb) f |= sig[(size_t)sb * nshell + sh];
+            na = f ? nall : 0;
         }
         int incl = na;
 #pragma unroll
@@ -278,21 +282,28 @@ void ao_list_kernel(int nsub, int nshell, const unsigned char* __restrict__ sig,
         }
         const int off = n + incl - na;
         for (int m = 0; m < na; ++m) out[off + m] = a0 + m;
+        if (iv)
+            for (int m = 0; m < nall; ++m) iv[a0 + m] = na ? off + m : -1;
         n += __shfl(incl, 63, 64);
     }
     if (lane == 0) counts[t] = n;
 }
 
-AoLists ao_lists_layout(const Basis& b, int nsub)
+AoLists ao_lists_layout(const Basis& b, int nsub, int kchunk)
 {
     AoLists L;
     const int ntile = (nsub + 1) / 2;
     L.cap = b.nao;
+    L.spt = std::max(1, kchunk / SUB);
+    L.nsplit = (nsub + L.spt - 1) / L.spt;
     size_t o = 0;
     auto take = [&](size_t bytes) { const size_t r = o; o += (bytes + 255) / 256 * 256; return r; };
     L.sig = take((size_t)nsub * b.nshell);
     L.counts = take(sizeof(int) * (size_t)ntile);
     L.lists = take(sizeof(int) * (size_t)ntile * L.cap);
+    L.scounts = take(sizeof(int) * (size_t)L.nsplit);
+    L.slists = take(sizeof(int) * (size_t)L.nsplit * L.cap);
+    L.sinv = take(sizeof(int) * (size_t)L.nsplit * L.cap);
     L.total = o;
     return L;
 }
@@ -305,9 +316,12 @@ void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, double* 
     launch_ao_pass<AO_STORE0>(s, b, g.d_xyz.as<double>(), g.Ppad, 16, X, nullptr, nullptr,
                               nullptr, sig);
     const int ntile = (g.nsub + 1) / 2;
-    hipLaunchKernelGGL(ao_list_kernel, dim3((ntile + 3) / 4), dim3(256), 0, s, g.nsub, b.nshell,
-                       sig, b.d_shao.as<int>(), b.d_shao.as<int>() + b.nshell, L.cap,
-                       (int*)(base + L.counts), (int*)(base + L.lists));
+    hipLaunchKernelGGL(ao_list_kernel, dim3((ntile + 3) / 4), dim3(256), 0, s, g.nsub, 2,
+                       b.nshell, sig, b.d_shao.as<int>(), b.d_shao.as<int>() + b.nshell, L.cap,
+                       (int*)(base + L.counts), (int*)(base + L.lists), (int*)nullptr);
+    hipLaunchKernelGGL(ao_list_kernel, dim3((L.nsplit + 3) / 4), dim3(256), 0, s, g.nsub, L.spt,
+                       b.nshell, sig, b.d_shao.as<int>(), b.d_shao.as<int>() + b.nshell, L.cap,
+                       (int*)(base + L.scounts), (int*)(base + L.slists), (int*)(base + L.sinv));
     WB_HIP(hipGetLastError());
 }
 
@@ -750,12 +764,30 @@ void gemm_nn_gather_kernel(int M, int N, const double* __restrict__ A, int lda,
 }
 
 // Cp[z][M][N] = sum_{k in split z} A[m][k] B[n][k] (both k-contiguous, K
-// even); tiles fastest within a split (they share the A and B slices)
-template <int WM, int WN>
+// even); tiles fastest within a split (they share the A and B slices).
+// GATHER: the rows of A are those of split z's AO list (glist[z][0 ..
+// gcnt[z]), significant AOs only), and Cp[z] holds them compacted: row p of
+// Cp[z] is AO glist[z][p]; m tiles past the count have nothing to do.
+template <int BR>
+__device__ __forceinline__ void fetch_kslice_rows(const double* __restrict__ A, int lda,
+                                                  const int* __restrict__ rows, int nrows,
+                                                  int kend, int r0, int k0, dv2 (&v)[BR / 32])
+{
+#pragma unroll
+    for (int i = 0; i < BR / 32; ++i) {
+        const int id = i * 256 + threadIdx.x;
+        const int r = rows[min(r0 + (id >> 3), nrows - 1)], k = k0 + 2 * (id & 7);
+        const dv2 t = *(const dv2*)(A + (size_t)r * lda + min(k, kend - 2));
+        v[i] = k < kend ? t : (dv2){0.0, 0.0};
+    }
+}
+
+template <int WM, int WN, bool GATHER>
 __global__ __launch_bounds__(256, 2)
 void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
                       const double* __restrict__ A, const double* __restrict__ B,
-                      double* __restrict__ Cp)
+                      double* __restrict__ Cp, const int* __restrict__ gcnt,
+                      const int* __restrict__ glist, int gcap)
 {
     constexpr int BM = 32 * WM, BN = 32 * WN;
     __shared__ dv2 As[2][BM * 8];
@@ -764,6 +796,13 @@ void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
     const int tile = w % ntile, z = w / ntile;
     const int m0 = (tile % mt) * BM, n0 = (tile / mt) * BN;
     const int kb = z * kchunk, ke = min(K, kb + kchunk);
+    int Mz = M;
+    const int* __restrict__ lz = nullptr;
+    if constexpr (GATHER) {
+        Mz = gcnt[z];
+        if (m0 >= Mz) return;                     // uniform per block
+        lz = glist + (size_t)z * gcap;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
     const int c = lane & 15, kq = lane >> 4;
@@ -773,7 +812,11 @@ void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
 #pragma unroll
         for (int j = 0; j < WN; ++j) acc[i][j] = (d4){0, 0, 0, 0};
     dv2 va[BM / 32], vb[BN / 32];
-    fetch_kslice<BM>(A, K, M, ke, m0, kb, va);
+    auto fetch_a = [&](int k0) {
+        if constexpr (GATHER) fetch_kslice_rows<BM>(A, K, lz, Mz, ke, m0, k0, va);
+        else fetch_kslice<BM>(A, K, M, ke, m0, k0, va);
+    };
+    fetch_a(kb);
     fetch_kslice<BN>(B, K, N, ke, n0, kb, vb);
     int buf = 0;
     for (int k0 = kb; k0 < ke; k0 += T_K) {
@@ -781,7 +824,7 @@ void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
         store_kslice<BN>(Bs[buf], vb);
         __syncthreads();
         if (k0 + T_K < ke) {
-            fetch_kslice<BM>(A, K, M, ke, m0, k0 + T_K, va);
+            fetch_a(k0 + T_K);
             fetch_kslice<BN>(B, K, N, ke, n0, k0 + T_K, vb);
         }
         dv2 a[WM][2], b[WN][2];
@@ -798,7 +841,25 @@ void gemm_nt_t_kernel(int M, int N, int K, int kchunk, int mt, int ntile,
                     acc[i][j] = mfma64(a[i][s >> 1][s & 1], b[j][s >> 1][s & 1], acc[i][j]);
         buf ^= 1;
     }
-    store_tile<WM, WN>(Cp + (size_t)z * M * N, N, M, N, m0, n0, wm, wn, acc);
+    store_tile<WM, WN>(Cp + (size_t)z * M * N, N, Mz, N, m0, n0, wm, wn, acc);
+}
+
+// C = (accumulate ? C : 0) + the compacted partials of the gathered split
+// GEMM, in split order: row m of C takes row ginv[z][m] of Cp[z] where that
+// is >= 0 (the AO was in split z's list), nothing from the others
+__global__ void split_reduce_gather_kernel(int M, int N, int nsplit, const double* __restrict__ Cp,
+                                           double* __restrict__ C, int accumulate,
+                                           const int* __restrict__ ginv, int gcap)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= M * N) return;
+    const int m = i / N, n = i - m * N;
+    double v = accumulate ? C[i] : 0.0;
+    for (int z = 0; z < nsplit; ++z) {
+        const int p = ginv[(size_t)z * gcap + m];
+        if (p >= 0) v += Cp[(size_t)z * M * N + (size_t)p * N + n];
+    }
+    C[i] = v;
 }
 
 // out[i] = w[i % K] * sum_c src[c][i]  (the G copies and the grid weights
@@ -950,7 +1011,7 @@ size_t gemm_nt_ws_bytes(int M, int N, int K, bool fold)
 
 void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const void* B, bool b_f32,
                   int nb, size_t bstride, const double* w, double* C, DevBuf& ws,
-                  const unsigned* masks, const int* row0, int nsub)
+                  const unsigned* masks, const int* row0, int nsub, const NtGather* gather)
 {
     if (masks && (K != nsub * SUB || !w || bstride % 4 || !aligned16(B) || !aligned16(w)))
         throw Error(WB_INVALID, "masked fold: B must be sub-block copies of 16-byte rows");
@@ -965,7 +1026,19 @@ void gemm_nt_fast(hipStream_t s, int M, int N, int K, const double* A, const voi
     const size_t part_bytes = sizeof(double) * (size_t)nsplit * M * N;
     char* base = (char*)ws.ensure(part_bytes + (fold ? sizeof(double) * (size_t)N * K : 0));
     gemm_nt_core(s, M, N, K, kchunk, A, B, b_f32, nb, bstride, w, C, false, (double*)base,
-                 fold ? (double*)(base + part_bytes) : nullptr, masks, row0, nsub);
+                 fold ? (double*)(base + part_bytes) : nullptr, masks, row0, nsub, gather);
+}
+
+NtGather nt_gather(const AoLists& L, const void* ws)
+{
+    const char* base = (const char*)ws;
+    NtGather g;
+    g.counts = (const int*)(base + L.scounts);
+    g.lists = (const int*)(base + L.slists);
+    g.inv = (const int*)(base + L.sinv);
+    g.cap = L.cap;
+    g.nsplit = L.nsplit;
+    return g;
 }
 
 bool gemm_nt_folds(bool b_f32, int nb, const double* w, const void* B, const unsigned* masks)
@@ -975,7 +1048,8 @@ bool gemm_nt_folds(bool b_f32, int nb, const double* w, const void* B, const uns
 
 void gemm_nt_core(hipStream_t s, int M, int N, int K, int kchunk, const double* A, const void* B,
                   bool b_f32, int nb, size_t bstride, const double* w, double* C, bool accumulate,
-                  double* Cp, double* Bf, const unsigned* masks, const int* row0, int nsub)
+                  double* Cp, double* Bf, const unsigned* masks, const int* row0, int nsub,
+                  const NtGather* gather)
 {
     if (K % 2 || !aligned16(A) || kchunk % T_K)
         throw Error(WB_INVALID, "split GEMM: K even, A 16-byte aligned, splits of whole slices");
@@ -1011,16 +1085,29 @@ void gemm_nt_core(hipStream_t s, int M, int N, int K, int kchunk, const double* 
         Bd = Bf;
     }
     dim3 grid(ntile * nsplit);
-#define WB_NT(a, b) hipLaunchKernelGGL((gemm_nt_t_kernel<a, b>), grid, dim3(256), 0, s, M, N, K, \
-                                        kchunk, mt, ntile, A, Bd, Cp)
-    if (wm == 3 && wn == 3) WB_NT(3, 3);
-    else if (wm == 3) WB_NT(3, 4);
-    else if (wn == 3) WB_NT(4, 3);
-    else WB_NT(4, 4);
+    const bool g = gather != nullptr;
+    if (g && (gather->nsplit != nsplit || gather->cap < M))
+        throw Error(WB_INVALID, "split GEMM: AO lists for another split or row count");
+    const int* gc = g ? gather->counts : nullptr;
+    const int* gl = g ? gather->lists : nullptr;
+    const int gcap = g ? gather->cap : 0;
+#define WB_NT(a, b)                                                                              \
+    if (g) hipLaunchKernelGGL((gemm_nt_t_kernel<a, b, true>), grid, dim3(256), 0, s, M, N, K,    \
+                              kchunk, mt, ntile, A, Bd, Cp, gc, gl, gcap);                       \
+    else hipLaunchKernelGGL((gemm_nt_t_kernel<a, b, false>), grid, dim3(256), 0, s, M, N, K,     \
+                            kchunk, mt, ntile, A, Bd, Cp, gc, gl, gcap);
+    if (wm == 3 && wn == 3) { WB_NT(3, 3) }
+    else if (wm == 3) { WB_NT(3, 4) }
+    else if (wn == 3) { WB_NT(4, 3) }
+    else { WB_NT(4, 4) }
 #undef WB_NT
     WB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
-                       nsplit, Cp, C, accumulate ? 1 : 0);
+    if (g)
+        hipLaunchKernelGGL(split_reduce_gather_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s,
+                           M, N, nsplit, Cp, C, accumulate ? 1 : 0, gather->inv, gcap);
+    else
+        hipLaunchKernelGGL(split_reduce_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s, M * N,
+                           nsplit, Cp, C, accumulate ? 1 : 0);
     WB_HIP(hipGetLastError());
 }
 
