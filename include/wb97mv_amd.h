@@ -133,6 +133,12 @@ int wb_basis_create(wb_ctx_t ctx, int nshell, const int* l, const int* nprim,
 /* out4 = {nao, nshell, kept shell pairs, primitive pairs} */
 int wb_basis_info(wb_basis_t basis, long long* out4);
 int wb_basis_destroy(wb_basis_t basis);
+/* Device bytes the basis holds, plus the primitive-pair records a K build of
+ * nop operators would still add to it (both precisions; 0 operators: what
+ * it holds now).  No counterpart in the reference (its per-mol builder,
+ * gpu/sgx_k.py:649-660, holds its arrays in CuPy's pool); for admission by
+ * device footprint. */
+int wb_basis_bytes(wb_basis_t basis, int nop, long long* out);
 
 /* Integration grid (bohr), resident on the device; points are reordered
  * internally into spatially compact 64-point sub-blocks.  Replaces
@@ -142,6 +148,8 @@ int wb_grid_create(wb_ctx_t ctx, int npts, const double* coords, const double* w
 /* out3 = {npts, padded points, sub-blocks} */
 int wb_grid_info(wb_grid_t grid, long long* out3);
 int wb_grid_destroy(wb_grid_t grid);
+/* Device bytes the grid holds (coordinates, weights, sub-block spheres). */
+int wb_grid_bytes(wb_grid_t grid, long long* out);
 
 /* COSX exchange for nop operators (omega = 0: 1/r, > 0: erf(omega r)/r):
  * K_op = X^T (w o G_op), G_op[g] = A_op(g) F[g], F = X qd, with qd = Q dm

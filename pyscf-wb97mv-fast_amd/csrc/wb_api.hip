@@ -1303,6 +1303,53 @@ int wb_basis_destroy(wb_basis_t basis)
     WB_TRY({ delete (Basis*)basis; })
 }
 
+int wb_basis_bytes(wb_basis_t basis, int nop, long long* out)
+{
+    WB_TRY({
+        auto* b = (Basis*)basis;
+        if (!b || !out) throw Error(WB_INVALID, "null argument");
+        if (nop < 0 || nop > 8) throw Error(WB_INVALID, "nop out of range");
+        size_t n = b->d_shells.bytes + b->d_exps.bytes + b->d_coefs.bytes + b->d_shao.bytes +
+                   b->d_row0.bytes;
+        for (auto& C : b->cls) {
+            n += C.d_pairs.bytes + C.d_scr.bytes + C.d_groups.bytes + C.d_bbao.bytes +
+                 C.d_pbc.bytes;
+            std::lock_guard<std::mutex> lock(C.opc_mu);
+            // records for nop operators cached: one set for all of them (a
+            // fused class), or one set per operator
+            int sets1 = 0;
+            bool setn = false;
+            for (const auto& kv : C.opc) {
+                n += kv.second.first->bytes + kv.second.second->bytes;
+                if (kv.first.size() == 1) ++sets1;
+                if ((int)kv.first.size() == nop) setn = true;
+            }
+            const bool have = setn || sets1 >= nop;
+            if (!have && nop > 0 && C.nprims > 0) {
+                // what a K build of nop operators adds: one record set for
+                // the operators together where the class runs them fused,
+                // else one per operator, each in FP32 and FP64
+                const size_t one = (size_t)md_rec_bytes(C.la, C.lb, true, 1) +
+                                   md_rec_bytes(C.la, C.lb, false, 1);
+                const size_t two = (size_t)md_rec_bytes(C.la, C.lb, true, 2) +
+                                   md_rec_bytes(C.la, C.lb, false, 2);
+                n += (size_t)C.nprims * ((nop == 2 && md_fuse(C.la, C.lb)) ? std::max(two, 2 * one)
+                                                                          : nop * one);
+            }
+        }
+        *out = (long long)n;
+    })
+}
+
+int wb_grid_bytes(wb_grid_t grid, long long* out)
+{
+    WB_TRY({
+        auto* g = (Grid*)grid;
+        if (!g || !out) throw Error(WB_INVALID, "null argument");
+        *out = (long long)(g->d_xyz.bytes + g->d_w.bytes + g->d_sub.bytes);
+    })
+}
+
 int wb_grid_create(wb_ctx_t ctx, int npts, const double* coords, const double* weights,
                    wb_grid_t* out)
 {

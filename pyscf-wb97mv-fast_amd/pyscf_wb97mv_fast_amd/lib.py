@@ -96,6 +96,8 @@ _SIGS = {
     'wb_cycle_dev': (_i, [_vp, _vp, _vp, _vp, _i, _vp, _d, _vp, _vp, _d, _d, _d, _i, _vp, _vp,
                           _vp]),
     'wb_vv10_stats': (_i, [_vp, _vp]),
+    'wb_basis_bytes': (_i, [_vp, _i, _vp]),
+    'wb_grid_bytes': (_i, [_vp, _vp]),
     'wb_cycle_graph_launch': (_i, [_vp]),
     'wb_cycle_graph_destroy': (_i, [_vp]),
 }
@@ -229,6 +231,13 @@ class DeviceBasis:
         check(lib.wb_basis_info(self.h, ptr(info)))
         self.nao, self.nshell, self.npairs, self.nprims = (int(v) for v in info)
 
+    def device_bytes(self, nop=0):
+        """Device bytes held, plus the records a K build of nop operators
+        would still add."""
+        v = ctypes.c_longlong()
+        check(load().wb_basis_bytes(self.h, int(nop), ctypes.byref(v)))
+        return int(v.value)
+
     def __del__(self):
         try:
             if getattr(self, 'h', None):
@@ -253,6 +262,11 @@ class DeviceGrid:
         info = numpy.zeros(3, dtype=numpy.int64)
         check(lib.wb_grid_info(self.h, ptr(info)))
         self.npts, self.ppad, self.nsub = (int(v) for v in info)
+
+    def device_bytes(self):
+        v = ctypes.c_longlong()
+        check(load().wb_grid_bytes(self.h, ctypes.byref(v)))
+        return int(v.value)
 
     def __del__(self):
         try:

@@ -108,6 +108,7 @@ class Request:
     t_start: float = field(default=None, repr=False)
     t_end: float = field(default=None, repr=False)
     est_bytes: int = field(default=0, repr=False)     # device bytes the job needs from empty
+    state_bytes: int = field(default=0, repr=False)   # of which its basis, records and grids
     k_budget: int = field(default=0, repr=False)      # K workspace budget set to fit (0: none)
     _seq: int = field(default=0, repr=False)          # arrival number
     _shared: bool = field(default=False, repr=False)  # started while other jobs ran
@@ -346,25 +347,37 @@ class CycleServer:
                 if b > alone:
                     alone, big = b, (kg, prec)
             return after, alone, big
+        # the job's own device state (its basis with the operator records its
+        # K builds add, every stage's grid) is held while it runs, on top of
+        # the slot's workspaces
+        grids_seen = {}
+        for kg, vg, _ in stages:
+            for g in (kg, vg):
+                if g is not None:
+                    grids_seen[id(g)] = g
+        state_bytes = basis.device_bytes(len(self.omegas)) + sum(
+            g.device_bytes() for g in grids_seen.values())
+        room = self.budget - state_bytes         # what the workspaces may take
         ctx.set_k_budget(0)
         after, alone, big = worst()
-        if alone > self.budget and big is not None and big[0] is not None:
+        if alone > room and big is not None and big[0] is not None:
             arena = ctx.k_plan(basis, big[0], len(self.omegas), big[1])[0]
-            kb = arena - (alone - self.budget)
+            kb = arena - (alone - room)
             while kb > 0:
                 ctx.set_k_budget(kb)
                 try:
                     after, alone, _ = worst()
                 except GpuOOM:                   # below one split's workspace
                     break
-                if alone <= self.budget:
+                if alone <= room:
                     req.k_budget = kb
                     break
-                kb -= max(alone - self.budget, arena // 64)
+                kb -= max(alone - room, arena // 64)
             else:
                 ctx.set_k_budget(0)
-        req.est_bytes = int(alone)
-        return int(after), int(alone)
+        req.state_bytes = int(state_bytes)
+        req.est_bytes = int(alone + state_bytes)
+        return int(after + state_bytes), int(alone + state_bytes)
 
     def _job_bytes(self, ctx, req, basis, kg, vg, precision):
         job = lib.CycleJob()

@@ -119,9 +119,22 @@ def test_workspace_accounting(systems):
     out = srv.submit(r).result(timeout=120)
     srv.close()
     held = c.mem_bytes
-    print('held', held, 'job bytes', r.est_bytes, flush=True)
-    # the job's workspace bytes (wb_job_bytes) are exactly what it allocated
-    assert held == r.est_bytes > 0
+    print('held', held, 'job bytes', r.est_bytes, 'of which its state', r.state_bytes, flush=True)
+    # the job's workspace bytes (wb_job_bytes) are exactly what it allocated;
+    # its basis (with the operator records its K builds add) and grids come
+    # on top, at least what they hold after a K build of both operators
+    assert held == r.est_bytes - r.state_bytes > 0
+    b = lib.DeviceBasis(c, pk)
+    g = lib.DeviceGrid(c, coords, weights)
+    need0 = b.device_bytes(2)
+    assert b.device_bytes(0) < need0
+    kk = numpy.empty((2, b.nao, b.nao))
+    lib.check(lib.load().wb_k_build(c.h, b.h, g.h, lib.ptr(lib.f64(dm)), 2,
+                                    lib.ptr(numpy.array([0.0, 0.3])), 1e-11, lib.PREC['mixed'],
+                                    lib.ptr(kk)))
+    assert b.device_bytes(2) == b.device_bytes(0) <= need0
+    assert b.device_bytes(0) + g.device_bytes() <= r.state_bytes
+    del b, g
     c.trim()
     assert c.mem_bytes == 0
     k, o2, v = _alone(pk, coords, weights, dm)
