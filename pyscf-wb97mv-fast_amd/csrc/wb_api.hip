@@ -808,7 +808,7 @@ static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char*
     K.gather = gather;
     K.AL = AL;
     K.lws = arena + P.lst;
-    if (gather) launch_ao_eval_lists(s, B, *K.gv, K.X, K.lws, AL);
+    if (gather) launch_ao_eval_lists(s, B, *K.gv, 0, K.X, K.lws, AL);
     else launch_ao_eval(s, B, *K.gv, 0, K.X);
     mark(PH_AO);
     if (gather) gemm_nn_gather(s, nao, Ppad, QD, K.X, K.F, f32, K.lws, AL);
@@ -1055,7 +1055,7 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     nt_split(Ppad, kc);
     const AoLists AL = ao_lists_layout(B, G.nsub, kc);
     void* lws = gather ? c.ws_list.ensure(AL.total) : nullptr;
-    if (gather) launch_ao_eval_lists(s, B, G, X, lws, AL);
+    if (gather) launch_ao_eval_lists(s, B, G, 0, X, lws, AL);
     else launch_ao_eval(s, B, G, 0, X);
     c.mark(PH_NLC_AO);
     double* T = (double*)c.ws_T.ensure(sizeof(double) * plane);
@@ -1618,16 +1618,28 @@ static void xc_rho_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, bool mgga,
     hipStream_t s = c.stream;
     const size_t plane = (size_t)nao * Ppad;
     double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane * (mgga ? 4 : 1));
-    launch_ao_eval(s, B, G, mgga ? 1 : 0, X);
+    // the GEMMs over each tile's significant AOs (values, and gradients when
+    // stored), as in the VV10 flow
+    const bool gather = gemm_nn_gather_ok(nao, Ppad, d_dm, X);
+    int kc = 0;
+    nt_split(Ppad, kc);
+    const AoLists AL = ao_lists_layout(B, G.nsub, kc);
+    void* lws = gather ? c.ws_list.ensure(AL.total) : nullptr;
+    if (gather) launch_ao_eval_lists(s, B, G, mgga ? 1 : 0, X, lws, AL);
+    else launch_ao_eval(s, B, G, mgga ? 1 : 0, X);
+    auto nn = [&](const double* Xp, double* out) {             // out = D Xp (D symmetric)
+        if (gather) gemm_nn_gather(s, nao, Ppad, d_dm, Xp, out, false, lws, AL);
+        else gemm_nn_fast(s, nao, Ppad, nao, d_dm, Xp, out, false);
+    };
     double* T = (double*)c.ws_T.ensure(sizeof(double) * plane);
-    gemm_nn_fast(s, nao, Ppad, nao, d_dm, X, T, false);         // T = D X (D symmetric)
+    nn(X, T);
     double* part = (double*)c.ws_red.ensure(ao_rho_bytes(B, Ppad));
     launch_ao_rho(s, B, G, T, part, rho);
     if (mgga) {
         // tau = 1/2 sum_d sum_mu (D dX_d)_mu dX_d,mu
         for (int d = 0; d < 3; ++d) {
             const double* Xd = X + (size_t)(d + 1) * plane;
-            gemm_nn_fast(s, nao, Ppad, nao, d_dm, Xd, T, false);
+            nn(Xd, T);
             launch_dot_planes(s, nao, Ppad, T, Xd, 0.5, d > 0, rho + 4 * (size_t)Ppad);
         }
     } else {
@@ -1643,21 +1655,33 @@ static void xc_vmat_dev(Ctx& c, Basis& B, Grid& G, const double* v, bool mgga, d
     hipStream_t s = c.stream;
     const size_t plane = (size_t)nao * Ppad;
     double* X = (double*)c.ws_Xv.ensure(sizeof(double) * plane * (mgga ? 4 : 1));
-    launch_ao_eval(s, B, G, mgga ? 1 : 0, X);
+    // the GEMMs over each point split's significant AO rows (values, and
+    // gradients when stored)
+    const bool gather = gemm_nn_gather_ok(nao, Ppad, X, X);
+    int kc = 0;
+    nt_split(Ppad, kc);
+    const AoLists AL = ao_lists_layout(B, G.nsub, kc);
+    void* lws = gather ? c.ws_list.ensure(AL.total) : nullptr;
+    if (gather) launch_ao_eval_lists(s, B, G, mgga ? 1 : 0, X, lws, AL);
+    else launch_ao_eval(s, B, G, mgga ? 1 : 0, X);
+    const NtGather ng = gather ? nt_gather(AL, lws) : NtGather{};
+    const NtGather* gp = gather ? &ng : nullptr;
     // B = (w v0 / 2) X + w v_d dX_d (weights and the 1/2 of the symmetric sum)
     double* wv = (double*)c.ws_vv.ensure(sizeof(double) * 5 * Ppad);
     launch_xc_weights(s, Ppad, G.d_w.as<double>(), v, wv);
     double* Bm = (double*)c.ws_T.ensure(sizeof(double) * plane);
     launch_ao_proj(s, B, G, wv, Bm);
     double* V = (double*)c.ws_K.ensure(sizeof(double) * (size_t)nao * nao * (mgga ? 2 : 1));
-    gemm_nt_fast(s, nao, nao, Ppad, X, Bm, false, 1, 0, nullptr, V, c.ws_Kp);
+    gemm_nt_fast(s, nao, nao, Ppad, X, Bm, false, 1, 0, nullptr, V, c.ws_Kp, nullptr, nullptr, 0,
+                 gp);
     symadd(s, nao, V, d_vmat);                                  // vmat = V + V^T
     if (mgga) {
         // + sum_d dX_d diag(w v4 / 2) dX_d^T (added once: already symmetric)
         double* Vt = V + (size_t)nao * nao;
         for (int d = 0; d < 3; ++d) {
             const double* Xd = X + (size_t)(d + 1) * plane;
-            gemm_nt_fast(s, nao, nao, Ppad, Xd, Xd, false, 1, 0, wv + 4 * (size_t)Ppad, Vt, c.ws_Kp);
+            gemm_nt_fast(s, nao, nao, Ppad, Xd, Xd, false, 1, 0, wv + 4 * (size_t)Ppad, Vt, c.ws_Kp,
+                         nullptr, nullptr, 0, gp);
             launch_axpy(s, (size_t)nao * nao, 1.0, Vt, d_vmat);
         }
     }

@@ -202,10 +202,11 @@ struct AoLists {
     int cap = 0, spt = 0, nsplit = 0;
 };
 AoLists ao_lists_layout(const Basis& b, int nsub, int kchunk);
-// X (values) as launch_ao_eval, plus the lists of the AOs significant in
-// each tile, into ws (ao_lists_layout(b, g.nsub).total bytes)
-void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, double* X, void* ws,
-                          const AoLists& L);
+// X (values; with deriv, the gradient planes too) as launch_ao_eval, plus
+// the lists of the AOs significant (value or stored gradient) in each tile
+// and each point split, into ws (ao_lists_layout(...).total bytes)
+void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X,
+                          void* ws, const AoLists& L);
 void launch_ao_eval_pts(hipStream_t s, const Basis& b, const double* xyz, int Ppad,
                         int deriv, double* X);
 

@@ -64,8 +64,9 @@ __device__ __forceinline__ void ao_shell_eval(double dx, double dy, double dz, d
 //   GGA potential factors at g), the operand of the potential matrix.
 enum AoMode { AO_STORE0 = 0, AO_STORE1, AO_RHO, AO_PROJ };
 
-// returns, for AO_STORE0, whether any value of the shell at g exceeds
-// AO_SIG_THR in magnitude
+// returns, for AO_STORE0 / AO_STORE1, whether any value (or, with the
+// gradients, any gradient component) of the shell at g exceeds AO_SIG_THR in
+// magnitude
 template <int l, int MODE>
 __device__ __forceinline__ bool ao_shell(double dx, double dy, double dz, double rad,
                                          double drad, double* __restrict__ X, size_t plane,
@@ -85,7 +86,10 @@ __device__ __forceinline__ bool ao_shell(double dx, double dy, double dz, double
             big = big || fabs(sv[m]) > AO_SIG_THR;
             if constexpr (MODE == AO_STORE1) {
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) X[(ax + 1) * plane + o] = gs[ax][m];
+                for (int ax = 0; ax < 3; ++ax) {
+                    X[(ax + 1) * plane + o] = gs[ax][m];
+                    big = big || fabs(gs[ax][m]) > AO_SIG_THR;
+                }
             }
         } else if constexpr (MODE == AO_RHO) {
             const double t = T[o];
@@ -100,9 +104,10 @@ __device__ __forceinline__ bool ao_shell(double dx, double dy, double dz, double
 }
 
 // grid: (points / 256, shell groups of spb); AO_RHO writes the group's
-// partial sums part[group][4][Ppad].  sig (AO_STORE0, may be null):
-// sig[sub-block][shell] = 1 if a value of the shell exceeds AO_SIG_THR at a
-// point of the 64-point sub-block (one wave = one sub-block), else 0.
+// partial sums part[group][4][Ppad].  sig (AO_STORE0 / AO_STORE1, may be
+// null): sig[sub-block][shell] = 1 if a value (or stored gradient) of the
+// shell exceeds AO_SIG_THR at a point of the 64-point sub-block (one wave =
+// one sub-block), else 0.
 template <int MODE>
 __global__ __launch_bounds__(256)
 void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
@@ -141,7 +146,7 @@ void ao_kernel(const ShellInfo* __restrict__ shells, int nshell, int spb,
         case 2: big = ao_shell<2, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
         default: big = ao_shell<3, MODE>(dx, dy, dz, rad, drad, X, plane, sh.ao0, Ppad, g, T, v, acc); break;
         }
-        if constexpr (MODE == AO_STORE0) {
+        if constexpr (MODE == AO_STORE0 || MODE == AO_STORE1) {
             if (sig) {                                // uniform
                 const bool any = __ballot(big) != 0ull;
                 if ((threadIdx.x & 63) == 0) sig[(size_t)(g >> 6) * nshell + is] = any ? 1 : 0;
@@ -308,13 +313,17 @@ AoLists ao_lists_layout(const Basis& b, int nsub, int kchunk)
     return L;
 }
 
-void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, double* X, void* ws,
-                          const AoLists& L)
+void launch_ao_eval_lists(hipStream_t s, const Basis& b, const Grid& g, int deriv, double* X,
+                          void* ws, const AoLists& L)
 {
     char* base = (char*)ws;
     unsigned char* sig = (unsigned char*)(base + L.sig);
-    launch_ao_pass<AO_STORE0>(s, b, g.d_xyz.as<double>(), g.Ppad, 16, X, nullptr, nullptr,
-                              nullptr, sig);
+    if (deriv)
+        launch_ao_pass<AO_STORE1>(s, b, g.d_xyz.as<double>(), g.Ppad, 16, X, nullptr, nullptr,
+                                  nullptr, sig);
+    else
+        launch_ao_pass<AO_STORE0>(s, b, g.d_xyz.as<double>(), g.Ppad, 16, X, nullptr, nullptr,
+                                  nullptr, sig);
     const int ntile = (g.nsub + 1) / 2;
     hipLaunchKernelGGL(ao_list_kernel, dim3((ntile + 3) / 4), dim3(256), 0, s, g.nsub, 2,
                        b.nshell, sig, b.d_shao.as<int>(), b.d_shao.as<int>() + b.nshell, L.cap,
