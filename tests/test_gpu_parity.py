@@ -89,6 +89,31 @@ def test_k_mixed_matches_oracle(ctx, basis, omega):
         assert abs(dE) < 3e-7
 
 
+@pytest.mark.parametrize('atoms,odd_nao', [(None, False),
+                                           ([('F', (0.0, 0.0, 0.0)), ('H', (0.0, 0.0, 0.92)),
+                                             ('O', (2.8, 0.0, 0.0)), ('H', (3.3, 0.8, 0.0)),
+                                             ('H', (3.3, -0.8, 0.0))], True)])
+def test_k_and_vv10_on_ragged_shapes(ctx, atoms, odd_nao):
+    """The FP64 GEMMs run over per-tile AO lists (128-point tiles) and per
+    point-split lists; an odd number of 64-point sub-blocks leaves a last
+    tile of 64 points, and an odd AO count (HF + H2O, 43 AOs) takes the dense
+    GEMMs instead.  K (both operators) and the VV10 flow in FP64 against the
+    oracle either way."""
+    mol, g, dm = _system(npts=1600, atoms=atoms)          # 25 sub-blocks
+    assert (g.size + 63) // 64 % 2 == 1
+    assert (mol.ao_loc_nr()[-1] % 2 == 1) == odd_nao
+    ob = oracle.OracleBasis.from_mol(mol)
+    K = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='fp64').k_raw(g, dm, [0.0, 0.3])
+    for op, omega in enumerate((0.0, 0.3)):
+        ref = oracle.k_build(ob, g.coords, g.weights, dm, omega, pair_cut=1e-14)
+        assert numpy.allclose(K[op], ref, rtol=0, atol=1e-10 * numpy.abs(ref).max()), omega
+    n_g, e_g, v_g = vv10.gpu_nr_nlc_vxc(vv10.NlcState(ctx, mol), sgx.SyntheticNumInt(), mol, g,
+                                        'wb97m-v', dm, precision='fp64')
+    n_o, e_o, v_o = oracle.nr_nlc_vxc(ob, g.coords, g.weights, dm)
+    assert abs(n_g - n_o) < 1e-10 and abs(e_g - e_o) < 1e-10
+    assert numpy.abs(v_g - v_o).max() < 1e-9
+
+
 def test_k_two_operators_one_call_and_repeatable(ctx):
     mol, g, dm = _system(npts=1200)
     kb = sgx_k.GpuKBuilder(mol, ctx, tile_tol=0.0, precision='mixed')
