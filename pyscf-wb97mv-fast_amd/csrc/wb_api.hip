@@ -621,8 +621,8 @@ static void collect(Ctx& c)
 struct KPlan {
     int kchunk = 0, csub = 0, nchunk = 0;
     size_t nitem_sub = 0;           // item-list entries per sub-block, all classes
-    size_t qdt = 0, x = 0, f = 0, fmax = 0, counts = 0, ids = 0, skips = 0, g = 0, mask = 0,
-           cp = 0, fold = 0, xyz = 0, w = 0, sub = 0, lst = 0, total = 0;
+    size_t qdt = 0, x = 0, f = 0, fmax = 0, counts = 0, wsub = 0, order = 0, ids = 0, skips = 0,
+           g = 0, mask = 0, cp = 0, fold = 0, xyz = 0, w = 0, sub = 0, lst = 0, total = 0;
 };
 
 static KPlan k_plan(const Basis& B, const Grid& G, int nop, bool f32, bool g64, int ncopy,
@@ -642,6 +642,8 @@ static KPlan k_plan(const Basis& B, const Grid& G, int nop, bool f32, bool g64, 
     P.f = take((f32 ? 4 : 8) * (nao + MD_SPARE) * Pc);
     P.fmax = take(4 * (size_t)P.csub * B.nshell);
     P.counts = take(4 * (size_t)NCLASS * P.csub);
+    P.wsub = take(4 * (size_t)NCLASS * P.csub);
+    P.order = take(4 * (size_t)NCLASS * P.csub);
     P.ids = take(4 * std::max<size_t>(P.nitem_sub * P.csub, 1));
     P.skips = take(4 * std::max<size_t>(P.nitem_sub * P.csub, 1));
     P.g = take((g64 ? 8 : 4) * (zrow + 1) * Pc * ncopy * nop);
@@ -747,6 +749,7 @@ struct KChunk {
     double* X = nullptr;
     void* F = nullptr;
     int* counts = nullptr;
+    int* order = nullptr;             // [class][sub-block], heavy first (null: identity)
     int* ids = nullptr;
     unsigned* skips = nullptr;
     size_t cls_off[NCLASS] = {};
@@ -822,6 +825,8 @@ static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char*
     zero_bytes(s, K.counts, sizeof(int) * (size_t)NCLASS * ns);
     K.ids = (int*)(arena + P.ids);
     K.skips = (unsigned*)(arena + P.skips);
+    // the exchange launches take their sub-blocks heaviest first
+    K.order = (int*)(arena + P.order);
     size_t toff = 0;
     TaskLaunch tl;
     for (int k = 0; k < NCLASS; ++k) {
@@ -836,12 +841,15 @@ static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char*
             t.ids = K.ids + toff;
             t.skips = K.skips + toff;
             t.work = work + 2 * k;
+            t.wsub = (int*)(arena + P.wsub) + (size_t)k * ns;
+            t.order = K.order ? K.order + (size_t)k * ns : nullptr;
             t.ngroups = C.ngroups;
             t.bb = md_bb(C.la, C.lb);
         }
         toff += (size_t)C.ngroups * ns;
     }
     launch_tasks(s, tl, *K.gv, fmax, B.nshell, (float)tol);
+    if (K.order) launch_order(s, tl, ns);
     // rows: nao, MD_SPARE spare rows (padding positions of shell blocks),
     // one zero row (read for a shell's first update in a region)
     K.gstride = (size_t)(K.zrow + 1) * Ppad;
@@ -870,6 +878,7 @@ static int k_class_jobs(const Ctx& c, const Basis& B, KChunk& K, const KRecs& R,
     const int* cnt = K.counts + (size_t)k * ns;
     const int* ids_k = K.ids + K.cls_off[k];
     const unsigned* skips_k = K.skips + K.cls_off[k];
+    const int* ord = K.order ? K.order + (size_t)k * ns : nullptr;
     const int zrows = K.first ? K.zrow : 0;
     // waves per sub-block of this class (the first class launched
     // initializes every copy's region, so it uses them all)
@@ -880,6 +889,7 @@ static int k_class_jobs(const Ctx& c, const Basis& B, KChunk& K, const KRecs& R,
         MdMasks mk{K.masks, K.mopstride, K.zrow, K.first};
         out[njob++] = md_job(C, *K.gv, K.F, K.Gb, K.gstride, K.opstride, nch, cnt, ids_k, skips_k,
                              R.rec2[k], zrows, mk);
+        out[njob - 1].order = ord;
         nopv = 2;
     } else {
         for (int op = 0; op < nop; ++op) {
@@ -887,6 +897,7 @@ static int k_class_jobs(const Ctx& c, const Basis& B, KChunk& K, const KRecs& R,
                        K.first};
             out[njob++] = md_job(C, *K.gv, K.F, K.Gb + K.gsz * K.opstride * op, K.gstride,
                                  K.opstride, nch, cnt, ids_k, skips_k, R.rec1[k][op], zrows, mk);
+            out[njob - 1].order = ord;
         }
     }
     K.first = false;

@@ -272,6 +272,8 @@ struct TaskClass {
     int* ids;
     unsigned* skips;
     unsigned long long* work;
+    int* wsub;                 // kept primitive pairs per sub-block (its waves' work)
+    int* order;                // sub-blocks by descending wsub (launch_order); may be null
     int ngroups, bb;
 };
 struct TaskLaunch {
@@ -283,6 +285,11 @@ struct TaskLaunch {
 // for every class of tl in one launch
 void launch_tasks(hipStream_t s, const TaskLaunch& tl, const Grid& g, const float* fmax,
                   int nshell, float tol);
+// per class of tl: order = the sub-blocks sorted by descending wsub (binned;
+// ties in any order), so that an exchange launch dispatches its longest
+// waves first and its tail holds short ones.  Results do not depend on it:
+// every wave writes only its own (copy, sub-block) region.
+void launch_order(hipStream_t s, const TaskLaunch& tl, int nsub);
 // f32: FP32 integrals (else FP64); g64: accumulate G in FP64 (always with FP64 integrals)
 // F_0..F_L at n values T (device arrays), out [L+1][n]
 void launch_boys_eval(hipStream_t s, int L, int n, const double* T, bool f32,
@@ -321,6 +328,7 @@ struct MdJob {
     const int* bbao;
     const char* recs;
     unsigned* masks;
+    const int* order;          // sub-block of each rank (null: identity order)
     size_t mopstride;
     int cap, Ppad, nsub, nchunk, zero_rows, zrow, first, pad_;
 };

@@ -431,6 +431,21 @@ __device__ __forceinline__ void md_wave(
     if (tracked) store_mask();
 }
 
+// worker w of a class launch (nsub sub-blocks, nchunk waves each) -> its
+// sub-block s and chunk j: chunk-major (w = j nsub + rank), and within each
+// chunk's pass the sub-blocks by rank in order (the launch's order: most kept
+// primitive pairs first, so the tail of the launch holds short waves), else in
+// grid order.  Measured on (H2O)27 def2-SVP, exchange both operators: grid
+// order 19.70 ms, this order 18.61 ms; rank-major (a sub-block's chunks back
+// to back) 20.72 ms in grid order; mapping consecutive blocks to one XCD
+// 20.12 ms (it gathers the dense regions' waves on a few XCDs).
+__device__ __forceinline__ void md_worker(long long w, int nsub, cptr<int> order, int& s, int& j)
+{
+    const int r = (int)(w % nsub);
+    j = (int)(w / nsub);
+    s = order != nullptr ? order[r] : r;
+}
+
 // md_wave on the molecule a describes (its pointers passed as the
 // __restrict__ parameters)
 template <int LA, int LB, int NOP, typename real, typename gacc>
@@ -444,7 +459,7 @@ __device__ __forceinline__ void md_wave_job(const MdJob& a, const real* __restri
                                      a.zrow, a.first, tab, s, j);
 }
 
-// one class, one molecule: worker = chunk x nsub + sub-block.  The pointers
+// one class, one molecule: one wave per worker (md_worker).  The pointers
 // are kernel arguments of their own (not fields of a by-value MdJob): with the
 // job as one struct argument the same body took 30-90 more VGPRs per class
 // (the (d,d) kernel one wave per SIMD instead of three) and the exchange
@@ -459,7 +474,7 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
                  const char* __restrict__ recs,
                  const real* __restrict__ boys_g, int Ppad, int nsub, int nchunk,
                  int zero_rows, unsigned* __restrict__ masks, size_t mopstride, int zrow,
-                 int first)
+                 int first, const int* __restrict__ order)
 {
     constexpr int TAB = sizeof(real) == 4 ? B32_NODES * b32_row(LA + LB) : 1;
     __shared__ __align__(16) real s_tab[TAB];
@@ -467,10 +482,12 @@ void md_k_kernel(const double* __restrict__ xyz, const real* __restrict__ F,
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int worker = blockIdx.x * MD_WAVES + wave;
     if (worker >= nsub * nchunk) return;
+    int s, j;
+    md_worker(worker, nsub, as_cptr(order), s, j);
     md_wave<LA, LB, NOP, real, gacc>(xyz, F, G, gstride, opstride, as_cptr(counts), as_cptr(ids),
                                      as_cptr(skips), cap, as_cptr(groups), as_cptr(bbao),
                                      as_cptr(recs), Ppad, nsub, nchunk, zero_rows, masks,
-                                     mopstride, zrow, first, tab, worker % nsub, worker / nsub);
+                                     mopstride, zrow, first, tab, s, j);
 }
 
 // one class, many molecules: a persistent grid whose waves take work items
@@ -503,8 +520,9 @@ void md_queue_kernel(const MdJob* __restrict__ jobs, const long long* __restrict
             else hi = mid - 1;
         }
         const MdJob a = jobs[lo];
-        const long long local = item - first[lo];
-        md_wave_job<LA, LB, NOP, real, gacc>(a, tab, (int)(local % a.nsub), (int)(local / a.nsub));
+        int s, j;
+        md_worker(item - first[lo], a.nsub, as_cptr(a.order), s, j);
+        md_wave_job<LA, LB, NOP, real, gacc>(a, tab, s, j);
     }
 }
 
@@ -559,7 +577,7 @@ static void md_launch_t(hipStream_t s, const MdJob* j, const MdJob* jobs, const 
                            dim3(64 * MD_WAVES), 0, s, a.xyz, (const real*)a.F, (gacc*)a.G,
                            a.gstride, a.opstride, a.counts, a.ids, a.skips, a.cap, a.groups,
                            a.bbao, a.recs, (const real*)boys, a.Ppad, a.nsub, a.nchunk,
-                           a.zero_rows, a.masks, a.mopstride, a.zrow, a.first);
+                           a.zero_rows, a.masks, a.mopstride, a.zrow, a.first, a.order);
     } else {
         auto kern = md_queue_kernel<LA, LB, NOP, real, gacc>;
         static const int resident = queue_blocks(kern);     // one device kind per process
@@ -811,9 +829,58 @@ void task_kernel(TaskLaunch tl, const float4* __restrict__ sub, int nsub,
     }
     if (lane == 0) {
         counts[s] = n;
+        if (tc.wsub) tc.wsub[s] = (int)(npp < 0x7fffffffull ? npp : 0x7fffffffull);
         atomicAdd(work, ntask);
         atomicAdd(work + 1, npp);
     }
+}
+
+// one block per class of tl: a counting sort of the sub-blocks into
+// ORD_BINS bins of descending weight (linear in wsub / max wsub); within a
+// bin the order is that of the LDS atomics
+constexpr int ORD_BINS = 256;
+__global__ __launch_bounds__(1024)
+void order_kernel(TaskLaunch tl, int nsub)
+{
+    const TaskClass& tc = tl.c[blockIdx.x];
+    const int* __restrict__ w = tc.wsub;
+    int* __restrict__ order = tc.order;
+    __shared__ int s_max;
+    __shared__ int s_pos[ORD_BINS];
+    const int t = threadIdx.x;
+    if (t == 0) s_max = 0;
+    for (int b = t; b < ORD_BINS; b += 1024) s_pos[b] = 0;
+    __syncthreads();
+    int m = 0;
+    for (int i = t; i < nsub; i += 1024) m = max(m, w[i]);
+    atomicMax(&s_max, m);
+    __syncthreads();
+    const long long mx = s_max;
+    auto bin = [&](int x) {
+        return mx > 0 ? (ORD_BINS - 1) - (int)((long long)x * (ORD_BINS - 1) / mx) : 0;
+    };
+    for (int i = t; i < nsub; i += 1024) atomicAdd(&s_pos[bin(w[i])], 1);
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int b = 0; b < ORD_BINS; ++b) {
+            const int c = s_pos[b];
+            s_pos[b] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < nsub; i += 1024) order[atomicAdd(&s_pos[bin(w[i])], 1)] = i;
+}
+
+void launch_order(hipStream_t s, const TaskLaunch& tl, int nsub)
+{
+    if (tl.n == 0 || nsub == 0) return;
+    for (int k = 0; k < tl.n; ++k)
+        if (!tl.c[k].wsub || !tl.c[k].order)
+            throw Error(WB_INVALID, "launch_order: every class needs wsub and order");
+    hipLaunchKernelGGL(order_kernel, dim3(tl.n), dim3(1024), 0, s, tl, nsub);
+    WB_HIP(hipGetLastError());
 }
 
 void launch_tasks(hipStream_t s, const TaskLaunch& tl, const Grid& g, const float* fmax,
