@@ -817,6 +817,7 @@ static void k_chunk_setup(Ctx& c, Basis& B, const Grid& G, const KPlan& P, char*
     if (gather) gemm_nn_gather(s, nao, Ppad, QD, K.X, K.F, f32, K.lws, AL);
     else gemm_nn_fast(s, nao, Ppad, nao, QDT, K.X, K.F, f32);
     mark(PH_FGEMM);
+    if (c.rec_gate && ci == 0) WB_HIP(hipEventRecord(c.ev_gate, s));
     // screening and task lists
     float* fmax = (float*)(arena + P.fmax);
     launch_fmax(s, K.F, f32, K.gv->d_w.as<double>(), nao, Ppad, ns, B.d_shao.as<int>(),
@@ -1162,6 +1163,7 @@ Ctx::~Ctx()
     side.reset();
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_gate) (void)hipEventDestroy(ev_gate);
     if (t0) (void)hipEventDestroy(t0);
     if (own_stream && stream) {
         (void)hipStreamSynchronize(stream);
@@ -2068,6 +2070,7 @@ static Ctx& side_ctx(Ctx& c)
     if (!c.side) c.side = make_sub_ctx(c);
     if (!c.ev_fork) WB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
     if (!c.ev_join) WB_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    if (!c.ev_gate) WB_HIP(hipEventCreateWithFlags(&c.ev_gate, hipEventDisableTiming));
     c.side->ncopy = c.ncopy;
     c.side->fuse_mask = c.fuse_mask;
     c.side->g64 = c.g64;
@@ -2083,9 +2086,24 @@ static void cycle_dev(Ctx& c, Basis& B, Grid* kg, const double* d_qd, int nop,
 {
     if (kg && vg && !c.profile) {
         Ctx& sd = side_ctx(c);
+        // The VV10 branch starts at the cycle start in the FP64 form, and
+        // after the K build's F GEMM in the mixed form (measured on (H2O)27
+        // def2-SVP: two gathered FP64 GEMMs side by side take longer than
+        // one after the other; mixed 45.6 -> 44.9 ms, FP64 96.1 -> 97.8 ms
+        // if it waited there too).  The whole K build is enqueued before
+        // the branch waits for its start events (the fork orders it after
+        // everything before the cycle, also when no gate was recorded).
+        struct Gate {
+            Ctx& c;
+            ~Gate() { c.rec_gate = false; }
+        } gate{c};
+        const bool late = prec == WB_PREC_MIXED;
+        c.rec_gate = late;
         WB_HIP(hipEventRecord(c.ev_fork, c.stream));
-        WB_HIP(hipStreamWaitEvent(sd.stream, c.ev_fork, 0));
         k_build_dev(c, B, *kg, d_qd, nop, omegas, k_tol, prec, d_k);
+        c.rec_gate = false;
+        WB_HIP(hipStreamWaitEvent(sd.stream, c.ev_fork, 0));
+        if (late) WB_HIP(hipStreamWaitEvent(sd.stream, c.ev_gate, 0));
         nlc_dev(sd, B, *vg, d_dm, b, Cc, vv_tol, prec, d_out2, d_vmat);
         WB_HIP(hipEventRecord(c.ev_join, sd.stream));
         WB_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));

@@ -171,6 +171,11 @@ struct Ctx {
     // that fork it from and join it to this context's stream
     std::unique_ptr<Ctx> side;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // with rec_gate set, the K build records ev_gate on this stream after
+    // its (first chunk's) F GEMM: where a concurrent cycle's VV10 branch
+    // starts in the mixed form
+    hipEvent_t ev_gate = nullptr;
+    bool rec_gate = false;
     Ctx() = default;
     Ctx(const Ctx&) = delete;
     Ctx& operator=(const Ctx&) = delete;

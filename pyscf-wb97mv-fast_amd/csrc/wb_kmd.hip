@@ -839,7 +839,8 @@ void task_kernel(TaskLaunch tl, const float4* __restrict__ sub, int nsub,
 // ORD_BINS bins of descending weight (linear in wsub / max wsub); within a
 // bin the order is that of the LDS atomics
 constexpr int ORD_BINS = 256;
-__global__ __launch_bounds__(1024)
+constexpr int ORD_THREADS = 256;      // a block the size of an exchange block
+__global__ __launch_bounds__(ORD_THREADS)
 void order_kernel(TaskLaunch tl, int nsub)
 {
     const TaskClass& tc = tl.c[blockIdx.x];
@@ -849,17 +850,17 @@ void order_kernel(TaskLaunch tl, int nsub)
     __shared__ int s_pos[ORD_BINS];
     const int t = threadIdx.x;
     if (t == 0) s_max = 0;
-    for (int b = t; b < ORD_BINS; b += 1024) s_pos[b] = 0;
+    for (int b = t; b < ORD_BINS; b += ORD_THREADS) s_pos[b] = 0;
     __syncthreads();
     int m = 0;
-    for (int i = t; i < nsub; i += 1024) m = max(m, w[i]);
+    for (int i = t; i < nsub; i += ORD_THREADS) m = max(m, w[i]);
     atomicMax(&s_max, m);
     __syncthreads();
     const long long mx = s_max;
     auto bin = [&](int x) {
         return mx > 0 ? (ORD_BINS - 1) - (int)((long long)x * (ORD_BINS - 1) / mx) : 0;
     };
-    for (int i = t; i < nsub; i += 1024) atomicAdd(&s_pos[bin(w[i])], 1);
+    for (int i = t; i < nsub; i += ORD_THREADS) atomicAdd(&s_pos[bin(w[i])], 1);
     __syncthreads();
     if (t == 0) {
         int acc = 0;
@@ -870,7 +871,7 @@ void order_kernel(TaskLaunch tl, int nsub)
         }
     }
     __syncthreads();
-    for (int i = t; i < nsub; i += 1024) order[atomicAdd(&s_pos[bin(w[i])], 1)] = i;
+    for (int i = t; i < nsub; i += ORD_THREADS) order[atomicAdd(&s_pos[bin(w[i])], 1)] = i;
 }
 
 void launch_order(hipStream_t s, const TaskLaunch& tl, int nsub)
@@ -879,7 +880,7 @@ void launch_order(hipStream_t s, const TaskLaunch& tl, int nsub)
     for (int k = 0; k < tl.n; ++k)
         if (!tl.c[k].wsub || !tl.c[k].order)
             throw Error(WB_INVALID, "launch_order: every class needs wsub and order");
-    hipLaunchKernelGGL(order_kernel, dim3(tl.n), dim3(1024), 0, s, tl, nsub);
+    hipLaunchKernelGGL(order_kernel, dim3(tl.n), dim3(ORD_THREADS), 0, s, tl, nsub);
     WB_HIP(hipGetLastError());
 }
 

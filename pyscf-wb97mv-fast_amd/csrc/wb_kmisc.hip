@@ -1465,27 +1465,27 @@ void vv_count_kernel(int nsub, int Ppad, const double* __restrict__ rho,
     if ((threadIdx.x & 63) == 0) cnt[s] = __popcll(b);
 }
 
-// exclusive scan of cnt[0..n) into off[0..n], off[n] = total (one block)
-__global__ __launch_bounds__(1024)
+// exclusive scan of cnt[0..n) into off[0..n], off[n] = total, by one wave:
+// a block of 1024 threads waits for a CU with 16 free wave slots, which a
+// concurrent exchange launch on the other stream keeps from it for
+// milliseconds (measured: 3.2 ms of the VV10 flow); one wave takes the first
+// free slot
+__global__ __launch_bounds__(64)
 void scan_kernel(int n, const int* __restrict__ cnt, int* __restrict__ off)
 {
-    __shared__ int part[1024];
     const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
+    const int per = (n + 63) / 64;
+    const int b = min(n, t * per), e = min(n, b + per);
     int sum = 0;
     for (int i = b; i < e; ++i) sum += cnt[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    int inc = sum;                                   // inclusive wave scan
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (t >= o) inc += v;
     }
-    int run = part[t] - sum;
+    int run = inc - sum;
     for (int i = b; i < e; ++i) { off[i] = run; run += cnt[i]; }
-    if (t == 1023) off[n] = part[1023];
+    if (t == 63) off[n] = inc;
 }
 
 template <typename real>
@@ -1934,7 +1934,7 @@ void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const doubl
     const double thresh = 1e-8;
     const int wb = (nsub + 3) / 4;
     hipLaunchKernelGGL(vv_count_kernel, dim3(wb), dim3(256), 0, s, nsub, Ppad, rho, w, thresh, cnt);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, nsub, cnt, off);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(64), 0, s, nsub, cnt, off);
     const int* nptr = off + nsub;
     const double pi = M_PI;
     const double beta = pow(3.0 / (b * b), 0.75) / 32.0;
