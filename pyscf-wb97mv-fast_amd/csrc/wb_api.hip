@@ -1079,7 +1079,8 @@ static void nlc_dev(Ctx& c, Basis& B, Grid& G, const double* d_dm, double b, dou
     c.mark(PH_NLC_RHO);
     double* wv = (double*)c.ws_vv.ensure(sizeof(double) * 5 * Ppad);
     vv10_flow(s, G.nsub, Ppad, G.d_xyz.as<double>(), G.d_w.as<double>(), G.origin, rho, b, Cc,
-              tol, f64, c.ws_B, wv, wv + 4 * Ppad, d_out2, c.profile ? &c : nullptr);
+              tol, f64, c.ws_B, wv, wv + 4 * Ppad, d_out2, c.profile ? &c : nullptr,
+              c.vv_cap16);
     c.vv_stat = (const unsigned long long*)(c.ws_B.as<char>() + vv_layout(G.nsub, Ppad, f64).stat);
     double* Bm = T;    // T is dead after the density: reuse for B
     launch_ao_proj(s, B, G, wv, Bm);
@@ -2093,11 +2094,17 @@ static void cycle_dev(Ctx& c, Basis& B, Grid* kg, const double* d_qd, int nop,
         // if it waited there too).  The whole K build is enqueued before
         // the branch waits for its start events (the fork orders it after
         // everything before the cycle, also when no gate was recorded).
+        // The VV10 pair sum runs on a capped grid beside the K build (4
+        // blocks per CU in FP32, 3 per 4 CUs in FP64), so that exchange
+        // blocks keep finding room: mixed 44.7 -> 44.3 ms, FP64 94.7 -> 87.0
+        // ms (uncapped, the FP64 pair sum's long blocks held most CUs).
         struct Gate {
             Ctx& c;
-            ~Gate() { c.rec_gate = false; }
-        } gate{c};
+            Ctx& sd;
+            ~Gate() { c.rec_gate = false; sd.vv_cap16 = 0; }
+        } gate{c, sd};
         const bool late = prec == WB_PREC_MIXED;
+        sd.vv_cap16 = late ? 64 : 12;
         c.rec_gate = late;
         WB_HIP(hipEventRecord(c.ev_fork, c.stream));
         k_build_dev(c, B, *kg, d_qd, nop, omegas, k_tol, prec, d_k);

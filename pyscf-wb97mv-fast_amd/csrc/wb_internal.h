@@ -176,6 +176,9 @@ struct Ctx {
     // starts in the mixed form
     hipEvent_t ev_gate = nullptr;
     bool rec_gate = false;
+    // on the VV10 side context of a concurrent cycle: the pair sum's grid,
+    // in blocks per 16 CUs (0: one block per work unit)
+    int vv_cap16 = 0;
     Ctx() = default;
     Ctx(const Ctx&) = delete;
     Ctx& operator=(const Ctx&) = delete;
@@ -397,9 +400,12 @@ void launch_grid_keep(hipStream_t s, long long n, const double* xyz, const doubl
 // (may be null; cleared here): with tol > 0, the (wave, tile) pairs tested
 // for pruning and those skipped.
 size_t vv10_sym_bytes(int ncap, bool f64, int* nsplit_out);
+// cap16 > 0: a grid of cap16 blocks per 16 CUs, each block taking work units
+// (group, part) in turn, so that a kernel running beside it keeps CU room;
+// the same sums either way
 void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, const void* pk,
                      const float4* tiles, float tol, bool f64, void* ws, double* F, double* U,
-                     double* W, unsigned long long* stat);
+                     double* W, unsigned long long* stat, int cap16 = 0);
 
 void zero_bytes(hipStream_t s, void* p, size_t bytes);
 void transpose(hipStream_t s, int n, const double* a, double* b);
@@ -428,6 +434,6 @@ void gemm_nt_core(hipStream_t s, int M, int N, int K, int kchunk, const double* 
 struct Ctx;
 void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
                const double* origin, const double* rho, double b, double C, double tol, bool f64,
-               DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof);
+               DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof, int cap16 = 0);
 
 }  // namespace wb

@@ -1226,158 +1226,164 @@ void vv10_sym_kernel(const int* __restrict__ nptr, const VP4<real>* __restrict__
     __shared__ real s_j[4][3][4][64];        // [wave][quantity][flush][slot]
     const int n = *nptr;
     const int B = vs_groups(n);
-    const int b = blockIdx.x / nsplit, part = blockIdx.x % nsplit;
-    if (b >= B) return;                      // uniform per block
     const int ntiles = (n + 63) / 64;
     const int hmax = vs_hmax(B);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i0 = b * VS_IG + wave * 128 + lane, i1 = i0 + 64;
-    const bool v0 = i0 < n, v1 = i1 < n;
-    // padding points carry q = 0 (and kappa = 1): they add nothing to anyone
-    auto ld4 = [&](int i, bool v) { VP4<real> r{0, 0, 0, 0}; if (v) r = pt[i]; return r; };
-    auto ld2 = [&](int i, bool v) { VP2<real> r{1, 0}; if (v) r = pk[i]; return r; };
-    const VP4<real> a = ld4(i0, v0), c = ld4(i1, v1);
-    const VP2<real> ka = ld2(i0, v0), kc = ld2(i1, v1);
-    // wave statistics of its points (pruning)
-    float cx = 0, cy = 0, cz = 0, rI = 0, wminI = 0, kminI = 0, qI = 0;
-    if (tol > 0.f) {
-        const float cnt = (float)__popcll(__ballot(v0)) + (float)__popcll(__ballot(v1));
-        float sx = (v0 ? (float)a.x : 0.f) + (v1 ? (float)c.x : 0.f);
-        float sy = (v0 ? (float)a.y : 0.f) + (v1 ? (float)c.y : 0.f);
-        float sz = (v0 ? (float)a.z : 0.f) + (v1 ? (float)c.z : 0.f);
-        float sq = (float)ka.q + (float)kc.q;
-        for (int o = 32; o > 0; o >>= 1) {
-            sx += __shfl_xor(sx, o, 64);
-            sy += __shfl_xor(sy, o, 64);
-            sz += __shfl_xor(sz, o, 64);
-            sq += __shfl_xor(sq, o, 64);
-        }
-        const float cn = fmaxf(cnt, 1.f);
-        cx = sx / cn; cy = sy / cn; cz = sz / cn; qI = sq * 1.0001f;
-        auto dist = [&](const VP4<real>& p) {
-            const float ex = (float)p.x - cx, ey = (float)p.y - cy, ez = (float)p.z - cz;
-            return sqrtf(ex * ex + ey * ey + ez * ez);
-        };
-        float r = fmaxf(v0 ? dist(a) : 0.f, v1 ? dist(c) : 0.f);
-        float wm = fminf(v0 ? (float)a.w : 3e38f, v1 ? (float)c.w : 3e38f);
-        float km = fminf(v0 ? (float)ka.k : 3e38f, v1 ? (float)kc.k : 3e38f);
-        for (int o = 32; o > 0; o >>= 1) {
-            r = fmaxf(r, __shfl_xor(r, o, 64));
-            wm = fminf(wm, __shfl_xor(wm, o, 64));
-            km = fminf(km, __shfl_xor(km, o, 64));
-        }
-        rI = r * 1.0001f + 1e-5f; wminI = wm * 0.9999f; kminI = km * 0.9999f;
-    }
-    double F0 = 0, U0 = 0, W0 = 0, F1 = 0, U1 = 0, W1 = 0;
-
-    // the block's work list: own tiles (part 0, one-sided), then its share
-    // of the partner tiles
-    const int np = vs_npartner(B, b);
-    const int tot = np * VS_GT;
-    const int lo = (int)((long long)tot * part / nsplit), hi = (int)((long long)tot * (part + 1) / nsplit);
-    const int nown = part == 0 ? VS_GT : 0;
     unsigned n_test = 0, n_skip = 0;             // wave-uniform pruning counts
-    for (int it = -nown; it < hi - lo; ++it) {
-        const bool own = it < 0;
-        int t, d = 0, u = 0;
-        if (own) {
-            t = b * VS_GT + (it + VS_GT);
-        } else {
-            const int x = lo + it;
-            d = x / VS_GT + 1;
-            u = x % VS_GT;
-            t = ((b + d) % B) * VS_GT + u;
-        }
-        if (t >= ntiles) continue;           // uniform per block
-        __syncthreads();
-        if (threadIdx.x < 128) {
-            const int j = t * 64 + (threadIdx.x & 63);
-            s_pt[threadIdx.x] = ld4(j, j < n);
-            s_pk[threadIdx.x] = ld2(j, j < n);
-        }
-        __syncthreads();
-        bool skip = false;
+    // work unit vb = (group b, part): one per block, or several per block
+    // when the grid is smaller than B x nsplit (in order, stride gridDim.x)
+    for (int vb = blockIdx.x; vb < B * nsplit; vb += gridDim.x) {
+        const int b = vb / nsplit, part = vb % nsplit;
+        const int i0 = b * VS_IG + wave * 128 + lane, i1 = i0 + 64;
+        const bool v0 = i0 < n, v1 = i1 < n;
+        // padding points carry q = 0 (and kappa = 1): they add nothing to anyone
+        auto ld4 = [&](int i, bool v) { VP4<real> r{0, 0, 0, 0}; if (v) r = pt[i]; return r; };
+        auto ld2 = [&](int i, bool v) { VP2<real> r{1, 0}; if (v) r = pk[i]; return r; };
+        const VP4<real> a = ld4(i0, v0), c = ld4(i1, v1);
+        const VP2<real> ka = ld2(i0, v0), kc = ld2(i1, v1);
+        // wave statistics of its points (pruning)
+        float cx = 0, cy = 0, cz = 0, rI = 0, wminI = 0, kminI = 0, qI = 0;
         if (tol > 0.f) {
-            const float4 tc = ts[2 * t], st = ts[2 * t + 1];   // c.xyz r | wmin kmin qsum
-            const float dd = sqrtf((tc.x - cx) * (tc.x - cx) + (tc.y - cy) * (tc.y - cy) +
-                                   (tc.z - cz) * (tc.z - cz));
-            const float rmin = fmaxf(dd - rI - tc.w, 0.f);
-            const float rmax = dd + rI + tc.w;
-            const float gI = rmin * rmin * wminI + kminI;
-            const float gJ = rmin * rmin * st.x + st.y;
-            const float den = gI * gJ * (gI + gJ);
-            const float bI = st.z / den, bJ = qI / den;
-            const float bound = fmaxf(fmaxf(bI, 2.f * bI / gI * fmaxf(1.f, rmax * rmax)),
-                                      fmaxf(bJ, 2.f * bJ / gJ * fmaxf(1.f, rmax * rmax)));
-            skip = bound < tol;              // uniform per wave
-            ++n_test;
-            n_skip += skip ? 1u : 0u;
+            const float cnt = (float)__popcll(__ballot(v0)) + (float)__popcll(__ballot(v1));
+            float sx = (v0 ? (float)a.x : 0.f) + (v1 ? (float)c.x : 0.f);
+            float sy = (v0 ? (float)a.y : 0.f) + (v1 ? (float)c.y : 0.f);
+            float sz = (v0 ? (float)a.z : 0.f) + (v1 ? (float)c.z : 0.f);
+            float sq = (float)ka.q + (float)kc.q;
+            for (int o = 32; o > 0; o >>= 1) {
+                sx += __shfl_xor(sx, o, 64);
+                sy += __shfl_xor(sy, o, 64);
+                sz += __shfl_xor(sz, o, 64);
+                sq += __shfl_xor(sq, o, 64);
+            }
+            const float cn = fmaxf(cnt, 1.f);
+            cx = sx / cn; cy = sy / cn; cz = sz / cn; qI = sq * 1.0001f;
+            auto dist = [&](const VP4<real>& p) {
+                const float ex = (float)p.x - cx, ey = (float)p.y - cy, ez = (float)p.z - cz;
+                return sqrtf(ex * ex + ey * ey + ez * ez);
+            };
+            float r = fmaxf(v0 ? dist(a) : 0.f, v1 ? dist(c) : 0.f);
+            float wm = fminf(v0 ? (float)a.w : 3e38f, v1 ? (float)c.w : 3e38f);
+            float km = fminf(v0 ? (float)ka.k : 3e38f, v1 ? (float)kc.k : 3e38f);
+            for (int o = 32; o > 0; o >>= 1) {
+                r = fmaxf(r, __shfl_xor(r, o, 64));
+                wm = fminf(wm, __shfl_xor(wm, o, 64));
+                km = fminf(km, __shfl_xor(km, o, 64));
+            }
+            rI = r * 1.0001f + 1e-5f; wminI = wm * 0.9999f; kminI = km * 0.9999f;
         }
-        if (own) {
-            if (!skip) {
+        double F0 = 0, U0 = 0, W0 = 0, F1 = 0, U1 = 0, W1 = 0;
+
+        // the block's work list: own tiles (part 0, one-sided), then its share
+        // of the partner tiles
+        const int np = vs_npartner(B, b);
+        const int tot = np * VS_GT;
+        const int lo = (int)((long long)tot * part / nsplit);
+        const int hi = (int)((long long)tot * (part + 1) / nsplit);
+        const int nown = part == 0 ? VS_GT : 0;
+        for (int it = -nown; it < hi - lo; ++it) {
+            const bool own = it < 0;
+            int t, d = 0, u = 0;
+            if (own) {
+                t = b * VS_GT + (it + VS_GT);
+            } else {
+                const int x = lo + it;
+                d = x / VS_GT + 1;
+                u = x % VS_GT;
+                t = ((b + d) % B) * VS_GT + u;
+            }
+            if (t >= ntiles) continue;           // uniform per block
+            __syncthreads();
+            if (threadIdx.x < 128) {
+                const int j = t * 64 + (threadIdx.x & 63);
+                s_pt[threadIdx.x] = ld4(j, j < n);
+                s_pk[threadIdx.x] = ld2(j, j < n);
+            }
+            __syncthreads();
+            bool skip = false;
+            if (tol > 0.f) {
+                const float4 tc = ts[2 * t], st = ts[2 * t + 1];   // c.xyz r | wmin kmin qsum
+                const float dd = sqrtf((tc.x - cx) * (tc.x - cx) + (tc.y - cy) * (tc.y - cy) +
+                                       (tc.z - cz) * (tc.z - cz));
+                const float rmin = fmaxf(dd - rI - tc.w, 0.f);
+                const float rmax = dd + rI + tc.w;
+                const float gI = rmin * rmin * wminI + kminI;
+                const float gJ = rmin * rmin * st.x + st.y;
+                const float den = gI * gJ * (gI + gJ);
+                const float bI = st.z / den, bJ = qI / den;
+                const float bound = fmaxf(fmaxf(bI, 2.f * bI / gI * fmaxf(1.f, rmax * rmax)),
+                                          fmaxf(bJ, 2.f * bJ / gJ * fmaxf(1.f, rmax * rmax)));
+                skip = bound < tol;              // uniform per wave
+                ++n_test;
+                n_skip += skip ? 1u : 0u;
+            }
+            if (own) {
+                if (!skip) {
+#pragma unroll 1
+                    for (int m = 0; m < 64 / VS_SUB; ++m) {
+                        real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0, z = 0;
+                        const VP4<real>* tp = s_pt + ((lane + m * VS_SUB) & 63);
+                        const VP2<real>* tk = s_pk + ((lane + m * VS_SUB) & 63);
+#pragma unroll
+                        for (int kk = 0; kk < VS_SUB; ++kk) {
+                            const VP4<real> p = tp[kk];
+                            const VP2<real> kp = tk[kk];
+                            vs_pair<real, false>(a, ka, p, kp, f0, u0, w0, z, z, z);
+                            vs_pair<real, false>(c, kc, p, kp, f1, u1, w1, z, z, z);
+                        }
+                        F0 += (double)f0; U0 += (double)u0; W0 += (double)w0;
+                        F1 += (double)f1; U1 += (double)u1; W1 += (double)w1;
+                    }
+                }
+                continue;
+            }
+            real (*sj)[4][64] = s_j[wave];
+            if (skip) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    sj[0][m][lane] = 0; sj[1][m][lane] = 0; sj[2][m][lane] = 0;
+                }
+            } else {
 #pragma unroll 1
                 for (int m = 0; m < 64 / VS_SUB; ++m) {
-                    real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0, z = 0;
+                    real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0;
+                    real cf = 0, cu = 0, cw = 0;     // accumulators of point (lane + k) mod 64
                     const VP4<real>* tp = s_pt + ((lane + m * VS_SUB) & 63);
                     const VP2<real>* tk = s_pk + ((lane + m * VS_SUB) & 63);
 #pragma unroll
                     for (int kk = 0; kk < VS_SUB; ++kk) {
                         const VP4<real> p = tp[kk];
                         const VP2<real> kp = tk[kk];
-                        vs_pair<real, false>(a, ka, p, kp, f0, u0, w0, z, z, z);
-                        vs_pair<real, false>(c, kc, p, kp, f1, u1, w1, z, z, z);
+                        vs_pair<real, true>(a, ka, p, kp, f0, u0, w0, cf, cu, cw);
+                        vs_pair<real, true>(c, kc, p, kp, f1, u1, w1, cf, cu, cw);
+                        cf = vs_rot(cf); cu = vs_rot(cu); cw = vs_rot(cw);
                     }
+                    // lane l now holds the sums of point (l + 16 (m + 1)) mod 64
+                    const int slot = (lane + (m + 1) * VS_SUB) & 63;
+                    sj[0][m][slot] = cf; sj[1][m][slot] = cu; sj[2][m][slot] = cw;
                     F0 += (double)f0; U0 += (double)u0; W0 += (double)w0;
                     F1 += (double)f1; U1 += (double)u1; W1 += (double)w1;
                 }
             }
-            continue;
-        }
-        real (*sj)[4][64] = s_j[wave];
-        if (skip) {
+            __syncthreads();
+            if (threadIdx.x < 192) {
+                const int q = threadIdx.x >> 6, slot = threadIdx.x & 63;
+                double acc = 0;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) { sj[0][m][lane] = 0; sj[1][m][lane] = 0; sj[2][m][lane] = 0; }
-        } else {
-#pragma unroll 1
-            for (int m = 0; m < 64 / VS_SUB; ++m) {
-                real f0 = 0, u0 = 0, w0 = 0, f1 = 0, u1 = 0, w1 = 0;
-                real cf = 0, cu = 0, cw = 0;     // accumulators of point (lane + k) mod 64
-                const VP4<real>* tp = s_pt + ((lane + m * VS_SUB) & 63);
-                const VP2<real>* tk = s_pk + ((lane + m * VS_SUB) & 63);
+                for (int wv = 0; wv < 4; ++wv)
 #pragma unroll
-                for (int kk = 0; kk < VS_SUB; ++kk) {
-                    const VP4<real> p = tp[kk];
-                    const VP2<real> kp = tk[kk];
-                    vs_pair<real, true>(a, ka, p, kp, f0, u0, w0, cf, cu, cw);
-                    vs_pair<real, true>(c, kc, p, kp, f1, u1, w1, cf, cu, cw);
-                    cf = vs_rot(cf); cu = vs_rot(cu); cw = vs_rot(cw);
-                }
-                // lane l now holds the sums of point (l + 16 (m + 1)) mod 64
-                const int slot = (lane + (m + 1) * VS_SUB) & 63;
-                sj[0][m][slot] = cf; sj[1][m][slot] = cu; sj[2][m][slot] = cw;
-                F0 += (double)f0; U0 += (double)u0; W0 += (double)w0;
-                F1 += (double)f1; U1 += (double)u1; W1 += (double)w1;
+                    for (int m = 0; m < 4; ++m) acc += (double)s_j[wv][q][m][slot];
+                real* rec = jrec + (((size_t)b * hmax + (d - 1)) * VS_GT + u) * 192;
+                rec[q * 64 + slot] = (real)acc;
             }
         }
-        __syncthreads();
-        if (threadIdx.x < 192) {
-            const int q = threadIdx.x >> 6, slot = threadIdx.x & 63;
-            double acc = 0;
-#pragma unroll
-            for (int wv = 0; wv < 4; ++wv)
-#pragma unroll
-                for (int m = 0; m < 4; ++m) acc += (double)s_j[wv][q][m][slot];
-            real* rec = jrec + (((size_t)b * hmax + (d - 1)) * VS_GT + u) * 192;
-            rec[q * 64 + slot] = (real)acc;
-        }
+        double* ip = ipart + (size_t)part * 3 * pcap;
+        if (v0) { ip[i0] = F0; ip[pcap + i0] = U0; ip[2 * pcap + i0] = W0; }
+        if (v1) { ip[i1] = F1; ip[pcap + i1] = U1; ip[2 * pcap + i1] = W1; }
     }
     if (stat && tol > 0.f && lane == 0) {
         atomicAdd(stat, (unsigned long long)n_test);
         atomicAdd(stat + 1, (unsigned long long)n_skip);
     }
-    double* ip = ipart + (size_t)part * 3 * pcap;
-    if (v0) { ip[i0] = F0; ip[pcap + i0] = U0; ip[2 * pcap + i0] = W0; }
-    if (v1) { ip[i1] = F1; ip[pcap + i1] = U1; ip[2 * pcap + i1] = W1; }
 }
 
 // per point: i-side parts in part order, then the records of the groups
@@ -1423,7 +1429,7 @@ size_t vv10_sym_bytes(int ncap, bool f64, int* nsplit_out)
 // vv10_sym_bytes(ncap) bytes
 void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, const void* pk,
                      const float4* tiles, float tol, bool f64, void* ws, double* F, double* U,
-                     double* W, unsigned long long* stat)
+                     double* W, unsigned long long* stat, int cap16)
 {
     if (stat) zero_bytes(s, stat, 2 * sizeof(unsigned long long));
     int nsplit = 1;
@@ -1432,7 +1438,14 @@ void launch_vv10_sym(hipStream_t s, const int* nptr, int ncap, const void* pts, 
     const size_t pcap = (size_t)Bm * VS_IG;
     double* ipart = (double*)ws;
     void* jrec = (char*)ws + (sizeof(double) * 3 * pcap * nsplit + 255) / 256 * 256;
-    const dim3 grid(Bm * nsplit), red((ncap + 255) / 256);
+    int blocks = Bm * nsplit;
+    if (cap16 > 0) {
+        int dev = 0, cus = 0;
+        WB_HIP(hipGetDevice(&dev));
+        WB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        blocks = std::min(blocks, std::max(1, cus * cap16 / 16));
+    }
+    const dim3 grid(blocks), red((ncap + 255) / 256);
     if (f64) {
         hipLaunchKernelGGL(vv10_sym_kernel<double>, grid, dim3(256), 0, s, nptr,
                            (const VP4<double>*)pts, (const VP2<double>*)pk, tiles, tol, nsplit,
@@ -1917,7 +1930,7 @@ VvLayout vv_layout(int nsub, int Ppad, bool f64)
 
 void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const double* w,
                const double* origin, const double* rho, double b, double C, double tol, bool f64,
-               DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof)
+               DevBuf& ws, double* wv, double* e, double* sums, Ctx* prof, int cap16)
 {
     // workspace layout
     const VvLayout L = vv_layout(nsub, Ppad, f64);
@@ -1947,7 +1960,7 @@ void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const doubl
         hipLaunchKernelGGL(vv_tile_kernel<double>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
                            nptr, pt, pk, (int)ntile_max, ts);
         launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, true, base + o_sym, F, U, W,
-                        (unsigned long long*)(base + L.stat));
+                        (unsigned long long*)(base + L.stat), cap16);
     } else {
         auto* pt = (VP4<float>*)(base + o_pt);
         auto* pk = (VP2<float>*)(base + o_pk);
@@ -1957,7 +1970,7 @@ void vv10_flow(hipStream_t s, int nsub, int Ppad, const double* xyz, const doubl
         hipLaunchKernelGGL(vv_tile_kernel<float>, dim3((ntile_max + 3) / 4), dim3(256), 0, s,
                            nptr, pt, pk, (int)ntile_max, ts);
         launch_vv10_sym(s, nptr, Ppad, pt, pk, ts, (float)tol, false, base + o_sym, F, U, W,
-                        (unsigned long long*)(base + L.stat));
+                        (unsigned long long*)(base + L.stat), cap16);
     }
     (void)pi;
     if (prof) prof->mark(PH_VV10);

@@ -14,12 +14,12 @@ OMEGAS = numpy.array([0.0, 0.3])
 NLC = (6.0, 0.01)
 
 
-def _system():
+def _system(nwater=3, npts=6000, level=0):
     from pyscf_wb97mv_fast_amd.systems import geometry, grids, guess, mole
-    mol = mole.Mole(geometry.water_cluster(3, seed=2), 'def2-svp')
-    g = grids.Grids(mol, level=0).build()
+    mol = mole.Mole(geometry.water_cluster(nwater, seed=2), 'def2-svp')
+    g = grids.Grids(mol, level=level).build()
     rng = numpy.random.default_rng(5)
-    sel = numpy.sort(rng.choice(g.size, min(g.size, 6000), replace=False))
+    sel = numpy.sort(rng.choice(g.size, min(g.size, npts), replace=False))
     dm = guess.huckel_density(mol)
     return mol, g.coords[sel], g.weights[sel], dm
 
@@ -117,11 +117,16 @@ def test_cycle_graph_k_only_and_argument_checks():
 
 
 @pytest.mark.parametrize('precision', ['mixed', 'fp64'])
-def test_concurrent_cycle_equals_sequential(precision):
+@pytest.mark.parametrize('size', ['small', 'large'])
+def test_concurrent_cycle_equals_sequential(precision, size):
+    """large: 60 000 points, 118 VV10 point groups x 16 parts = 1888 work
+    units, more than the concurrent cycle's capped VV10 grid holds blocks
+    (4 per CU mixed, 3 per 4 CUs FP64), so its blocks take several units."""
     import torch
     from pyscf_wb97mv_fast_amd import lib
     from pyscf_wb97mv_fast_amd.basis import pack_mol
-    mol, coords, weights, dm0 = _system()
+    mol, coords, weights, dm0 = _system(*((3, 6000, 0) if size == 'small' else (8, 60000, 1)))
+    assert size == 'small' or len(coords) == 60000
     nao = mol.nao
     ctx = lib.Context(0)
     L = lib.load()
@@ -132,7 +137,9 @@ def test_concurrent_cycle_equals_sequential(precision):
     c_k, c_e, c_v = torch.zeros((2, nao, nao), **f64), torch.zeros(2, **f64), torch.zeros((nao, nao), **f64)
     s_k, s_e, s_v = torch.zeros_like(c_k), torch.zeros_like(c_e), torch.zeros_like(c_v)
     prec = lib.PREC[precision]
-    for rep, dm in enumerate(_densities(dm0) * 2):
+    dens = _densities(dm0) * 2 if size == 'small' else [dm0, dm0]
+    half = len(dens) // 2
+    for rep, dm in enumerate(dens):
         d = torch.from_numpy(numpy.ascontiguousarray(dm)).to(dev)
         torch.cuda.synchronize()
         lib.check(L.wb_k_build_dev(ctx.h, basis.h, grid.h, d.data_ptr(), 2, lib.ptr(OMEGAS),
@@ -140,7 +147,7 @@ def test_concurrent_cycle_equals_sequential(precision):
         lib.check(L.wb_nlc_vxc_dev(ctx.h, basis.h, grid.h, d.data_ptr(), NLC[0], NLC[1], 0.0,
                                    prec, s_e.data_ptr(), s_v.data_ptr()))
         # phase timing on for the second half: the flows then run in sequence
-        lib.check(L.wb_ctx_set_profile(ctx.h, 1 if rep >= 3 else 0))
+        lib.check(L.wb_ctx_set_profile(ctx.h, 1 if rep >= half else 0))
         lib.check(L.wb_cycle_dev(ctx.h, basis.h, grid.h, d.data_ptr(), 2, lib.ptr(OMEGAS), 1e-11,
                                  grid.h, d.data_ptr(), NLC[0], NLC[1], 0.0, prec,
                                  c_k.data_ptr(), c_e.data_ptr(), c_v.data_ptr()))
