@@ -229,9 +229,10 @@ int wb_cycle_graph_destroy(wb_graph_t graph);
  * (device pointers; wb_sync waits): the K build on kgrid (d_qd -> d_k, nop
  * operators) and/or the VV10 term on nlcgrid (d_dm -> d_out2 = (nelec, exc),
  * d_vmat).  With both, the VV10 flow runs concurrently on a second stream of
- * the context (its own workspaces), forked and joined with events, so its
- * FP32 pair sum overlaps the K build's FP64 GEMMs; the outputs equal those
- * of wb_k_build_dev followed by wb_nlc_vxc_dev bit for bit.  Replaces the
+ * the context (its own workspaces), forked and joined with events (in the
+ * mixed form from the end of the K build's F GEMM), its pair sum on a grid
+ * narrow enough to leave the exchange kernels CU room; the outputs equal
+ * those of wb_k_build_dev followed by wb_nlc_vxc_dev bit for bit.  Replaces the
  * get_k_only calls (gpu/sgx_k.py:983-991) and the nr_nlc_vxc call
  * (gpu/vv10.py:268-342) of one cycle for callers that issue them together
  * (device-resident SCF loops, serving). */
