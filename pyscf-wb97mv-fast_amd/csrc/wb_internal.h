@@ -31,7 +31,8 @@ constexpr int MD_BBMAX = 8;
 constexpr int MD_SPARE = 7;
 __host__ __device__ constexpr bool md_block_a(int la, int lb) { return la > lb; }
 // (sizes measured on water27 def2-SVP: for (p, s) 4 beats 8 by 7 % of the
-// exchange time -- 8 p shells in registers halve the occupancy)
+// exchange time -- 8 p shells in registers halve the occupancy -- and 3 and
+// 2 by 1 % and 3 %)
 __host__ __device__ constexpr int md_bb(int la, int lb) {
     return la == lb ? (la <= 1 ? 8 : 1)
                     : (la == 1 ? 4 : (la == 2 ? 2 : (lb == 0 ? 4 : (lb == 1 ? 2 : 1))));
